@@ -1,0 +1,191 @@
+/*
+ * rmt3d.h -- C ABI of the MI355X-native RMT3D fluid-structure step.
+ *
+ * One context (rmt3d_ctx) is one simulation: the fields live in HBM, the
+ * context owns them and its HIP stream.  A context is not thread-safe;
+ * distinct contexts are independent (this is the serving unit).  No C++
+ * exception crosses this boundary: every call returns an int status
+ * (0 = ok, >0 = the reference's step error codes, <0 = library error) and
+ * rmt3d_last_error() gives the message.
+ *
+ * Index order of every grid array handed across the boundary is the
+ * reference's padded order: cell (i,j,k) of the (m+4)(n+4)(o+4) grid with
+ * two ghost layers sits at i + (m+4)*(j + (n+4)*k)   (fluid_3d_utils.cc:89-91,
+ * fluid_3d.cc:45 u_mem / rm_mem).
+ */
+#ifndef RMT3D_H
+#define RMT3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes of fluid_3d::step_forward (Utilities/defs.hh:47-50) ---- */
+#define RMT3D_NORMAL_EXIT        0
+#define RMT3D_REFMAP_UPDATE_ERR  1
+#define RMT3D_PROJECTION_ERR     2
+#define RMT3D_STEADY_SOLN_EXIT   100
+/* library-side errors */
+#define RMT3D_ERR_ARG           (-1)
+#define RMT3D_ERR_HIP           (-2)
+#define RMT3D_ERR_CFG           (-3)
+#define RMT3D_ERR_FATAL         (-4)   /* a reference p_fatal_error path */
+#define RMT3D_ERR_NOMEM         (-5)
+
+/* ---- object types supported on this path (Utilities/defs.hh:62-75) ---- */
+#define RMT3D_BASIC_SPHERE 0
+#define RMT3D_SPHERE       1
+
+/* ---- sim types (Execs/run_sim.cc:13-32) ---- */
+#define RMT3D_SIM_FTEST   0
+#define RMT3D_SIM_STEST   1
+#define RMT3D_SIM_OBJECTS 2
+#define RMT3D_SIM_MILLS   3
+
+#define RMT3D_MAX_VEL_PROF 8
+#define RMT3D_N_EXTRA_SPECS 10
+
+/* Arithmetic precision of a context. */
+#define RMT3D_PREC_FP64  0   /* every stage in FP64 (the parity path)           */
+#define RMT3D_PREC_MIXED 1   /* FP32 stencils + FP32 MG preconditioner inside an
+                                FP64 defect-correction loop                      */
+
+/* One immersed object: the per-object keys of the .cfg
+ * (object_list, basic_specs, extra_specs, srho, shear_mod;
+ * Sim/sim_params.cc:275-314). */
+typedef struct rmt3d_object {
+    int    type;                           /* object_list              */
+    double basic[4];                       /* centre x,y,z, radius     */
+    double extra[RMT3D_N_EXTRA_SPECS];     /* extra_specs              */
+    double srho;                           /* solid density            */
+    double shear_mod;                      /* G                        */
+} rmt3d_object;
+
+/* POD mirror of the sim_params fields used on the step path
+ * (Sim/sim_params.hh, defaults at Sim/sim_params.cc:10-41). */
+typedef struct rmt3d_config {
+    int    sys_size[3];          /* m n o (cells)                          */
+    int    prd[3];               /* x_prd y_prd z_prd after BC consolidation */
+    double xyz_bounds[6];        /* ax bx ay by az bz                      */
+    int    bct[3][6];            /* bct[component][face]                   */
+    double bcv[3][6];            /* bcv[component][face]                   */
+    int    walls[6];
+    double wall_pos[6];
+    double wall_dist;
+    double fmu, frho, fdt_pad;
+    double dt;                   /* user dt (before CFL)                   */
+    double T;
+    int    frames;
+    int    stop_short;
+    int    num_iters;
+    int    nlayers;
+    int    max_extrap_rs;
+    double wt_n, ex_visc_mult, ev_trans_mult, sdt_pad, dt_ex_pad, weight_fac;
+    double gravity;
+    int    godunov, impl, has_exact;
+    int    sim_type;             /* RMT3D_SIM_*                            */
+    int    vel_prof_num;
+    double vel_profile[2 * RMT3D_MAX_VEL_PROF];
+    int    dump_code, ntracers, debug_flag, chk_num, chk_step;
+    int    n_obj;
+} rmt3d_config;
+
+typedef struct rmt3d_ctx rmt3d_ctx;
+
+/* Host view of the state, in the reference's padded index order.
+ * Every pointer may be NULL (that field is skipped).  Grid arrays have
+ * (m+4)(n+4)(o+4) entries; the band arrays have nband entries. */
+typedef struct rmt3d_state {
+    double  *u, *v, *w;          /* field::vel                  (fields.hh:36)  */
+    double  *p, *dp, *q;         /* nodal p, dp; MAC q          (fields.hh:38-40)*/
+    double  *dvel;               /* 3 x Ng, component-major     (fields.hh:42)  */
+    double  *fvel;               /* 18 x Ng, face*3+comp major  (fields.hh:46)  */
+    double  *sigma;              /* 9 x Ng, lower-face*3+comp   (fields.hh:55)  */
+    uint32_t *rm_c;              /* ref_map::c                  (fields.hh:287) */
+    double  *rm_x;               /* 3 x Ng                                      */
+    double  *rm_xpred;           /* 3 x Ng                                      */
+    /* band ("extraps" multimap, extrap_mmap.hh:14-169), sorted by cell,
+     * entries of one cell in multimap order */
+    int      nband;              /* in: capacity of the arrays; out: count     */
+    int32_t *band_cell;
+    uint32_t *band_c;
+    double  *band_x;             /* 3 x nband                                   */
+    double  *band_xpred;         /* 3 x nband                                   */
+} rmt3d_state;
+
+/* ---------------------------------------------------------------------- */
+/* .cfg parsing: sim_params::sim_params(const char*) (Sim/sim_params.cc:10-410).
+ * Fills cfg and up to max_obj objects; returns 0, or RMT3D_ERR_CFG on an
+ * unknown keyword / unreadable file (the reference exits(1) there). */
+int rmt3d_parse_cfg(const char *path, rmt3d_config *cfg,
+                    rmt3d_object *objs, int max_obj);
+/* Defaults of sim_params (Sim/sim_params.cc:10-41) without a file. */
+void rmt3d_default_config(rmt3d_config *cfg);
+
+/* fluid_3d::fluid_3d(geometry*, sim_manager*, sim_params*) together with
+ * sim_manager_alloc (run_sim.cc:13-32, fluid_3d.cc:12-77).  precision is one
+ * of RMT3D_PREC_*.  Allocates every device buffer of the context. */
+int rmt3d_create(const rmt3d_config *cfg, const rmt3d_object *objs,
+                 int precision, rmt3d_ctx **out);
+void rmt3d_destroy(rmt3d_ctx *ctx);
+const char *rmt3d_last_error(const rmt3d_ctx *ctx);
+
+/* fluid_3d::initialize (fluid_3d.cc:192-233): returns init_err (1 = from
+ * scratch) or a negative library error. */
+int rmt3d_initialize(rmt3d_ctx *ctx);
+/* The run_sim.cc:98-106 sequence between initialize() and run(): clamp
+ * spars.dt to dt_reg, build the stepper's dt (stepper.hh:211-225), then
+ * fluid_3d::init_iter (fluid_3d.cc:235-287).  Returns the stepper dt in *dt. */
+int rmt3d_prepare_run(rmt3d_ctx *ctx, int init_err, double *dt, int *total_steps);
+/* fluid_3d::init_iter alone (fluid_3d.cc:235-287). */
+int rmt3d_init_iter(rmt3d_ctx *ctx, int init_err);
+/* fluid_3d::step_forward (fluid_3d.cc:429-577): returns 0, 1, 2 or <0. */
+int rmt3d_step(rmt3d_ctx *ctx, int debug);
+/* fluid_3d::set_dt (fluid_3d.hh:452-455) and the public dt / time members. */
+int rmt3d_set_dt(rmt3d_ctx *ctx, double dt);
+double rmt3d_get_dt(const rmt3d_ctx *ctx);
+double rmt3d_get_time(const rmt3d_ctx *ctx);
+int rmt3d_set_time(rmt3d_ctx *ctx, double t);
+
+/* State transfer in the reference's index order (the checkpoint path,
+ * fluid_3d_chk_pt.cc:74-163 reads the same fields).  upload replaces the
+ * fields whose pointers are non-NULL; download fills them. */
+int rmt3d_upload_state(rmt3d_ctx *ctx, const rmt3d_state *st);
+int rmt3d_download_state(rmt3d_ctx *ctx, rmt3d_state *st);
+/* Number of band entries currently held (to size rmt3d_state arrays). */
+int rmt3d_band_count(rmt3d_ctx *ctx);
+
+/* Multigrid solve behind mg_prob::run (Utilities/mg_probs.cc:3-83):
+ * which = 0 MAC (cell-centred 7-point, simp_mac, mg_probs.hh:497-530),
+ * which = 1 FEM (nodal 27-point, simp_fem, mg_probs.hh:458-495).
+ * rhs and x are dense arrays in i + j*M + k*M*N order of that problem's
+ * unknowns (MAC: m x n x o; FEM: m or m+1 per dimension).  x is the initial
+ * guess on entry and the solution on exit.  tol <= 0 selects the reference's
+ * max-norm tolerance.  *ncycles receives the number of V-cycles run, *resid
+ * the final max-norm residual. */
+int rmt3d_mg_solve(rmt3d_ctx *ctx, int which, const double *rhs, double *x,
+                   double tol, int *ncycles, double *resid);
+/* Unknown counts of the two problems (for sizing rmt3d_mg_solve arrays). */
+int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
+
+/* Diagnostics: which = 0 -> div_u L2 (fluid_3d_utils.cc:507-653),
+ * 1 -> number of V-cycles in the last MAC solve, 2 -> last FEM solve,
+ * 3 -> last MAC residual, 4 -> last FEM residual, 5 -> dt_reg,
+ * 6 -> eps (transition zone width), 7 -> max |det F - 1| on primary faces */
+int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
+
+/* Per-stage device time accumulated by the step (ms), in the order of the
+ * reference's timer buckets "Extrapolation, Pressure Poisson, Communication,
+ * Stresses, MAC" (fluid_3d.cc:59) followed by "Advection".  n <= 6. */
+int rmt3d_stage_times(rmt3d_ctx *ctx, double *ms, int n);
+
+/* Stream the context's kernels run on (a hipStream_t). */
+void *rmt3d_stream(rmt3d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

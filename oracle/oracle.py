@@ -1,0 +1,201 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of oracle/liboracle.so, the FP64 C restatement of the
+reference's step.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from rmt3d_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_api.c"]
+
+_lib = None
+
+
+def build(force=False):
+    """Compile the oracle with gcc (plain C11, no FMA contraction)."""
+    import subprocess
+    srcs = [os.path.join(HERE, s) for s in SOURCES]
+    if not force and os.path.exists(LIB_PATH):
+        newest = max(os.path.getmtime(s) for s in srcs + [os.path.join(HERE, "orc.h")])
+        if os.path.getmtime(LIB_PATH) >= newest:
+            return LIB_PATH
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-D_GNU_SOURCE",
+           "-ffp-contract=off", "-Wall", "-o", LIB_PATH] + srcs + ["-lm"]
+    subprocess.check_call(cmd)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = C.c_double
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object)]
+        L.oracle_destroy.argtypes = [P]
+        for f in ("oracle_initialize",):
+            getattr(L, f).argtypes = [P]
+        L.oracle_init_iter.argtypes = [P, C.c_int]
+        L.oracle_prepare_run.argtypes = [P, C.c_int, dp, ip]
+        L.oracle_step.argtypes = [P, C.c_int]
+        for f in ("oracle_get_dt", "oracle_get_time", "oracle_div_u"):
+            getattr(L, f).argtypes = [P]
+            getattr(L, f).restype = D
+        L.oracle_set_dt.argtypes = [P, D]
+        L.oracle_set_time.argtypes = [P, D]
+        L.oracle_band_count.argtypes = [P]
+        L.oracle_compute_stress.argtypes = [P]
+        L.oracle_scalar.argtypes = [P, C.c_int]
+        L.oracle_scalar.restype = D
+        L.oracle_download.argtypes = [P, C.POINTER(_abi.State)]
+        L.oracle_upload.argtypes = [P, C.POINTER(_abi.State)]
+        L.oracle_mg_solve.argtypes = [P, C.c_int, dp, dp, D, ip, dp]
+        L.oracle_mg_apply.argtypes = [P, C.c_int, dp, dp]
+        L.oracle_mg_levels.argtypes = [P, C.c_int, ip]
+        L.oracle_mono_diff.argtypes = [D] * 5
+        L.oracle_mono_diff.restype = D
+        L.oracle_godunov_vel.argtypes = [dp]
+        L.oracle_godunov_refmap.argtypes = [D, D, dp]
+        L.oracle_mat_inverse.argtypes = [dp, dp]
+        L.oracle_mat_aat.argtypes = [dp, dp]
+        L.oracle_mat_ata.argtypes = [dp, dp]
+        L.oracle_heaviside.argtypes = [P, D]
+        L.oracle_heaviside.restype = D
+        L.oracle_lsq.argtypes = [C.c_int, dp, dp, dp, D, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleSim:
+    """One oracle simulation (fluid_3d + sim_manager restated in C)."""
+
+    def __init__(self, cfg, objs):
+        self.cfg = cfg
+        n = max(1, len(objs))
+        arr = (_abi.Object * n)(*objs) if objs else (_abi.Object * 1)()
+        self._objs = arr
+        self.h = lib().oracle_create(C.byref(cfg), arr)
+        m, nn, o = cfg.sys_size
+        self.dims = (m, nn, o)
+        self.Ng = (m + 4) * (nn + 4) * (o + 4)
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self):
+        return lib().oracle_initialize(self.h)
+
+    def init_iter(self, e):
+        return lib().oracle_init_iter(self.h, e)
+
+    def prepare_run(self, init_err):
+        dt = C.c_double()
+        tot = C.c_int()
+        rc = lib().oracle_prepare_run(self.h, init_err, C.byref(dt), C.byref(tot))
+        return rc, dt.value, tot.value
+
+    def step(self, debug=0):
+        return lib().oracle_step(self.h, debug)
+
+    @property
+    def dt(self):
+        return lib().oracle_get_dt(self.h)
+
+    @property
+    def time(self):
+        return lib().oracle_get_time(self.h)
+
+    def set_dt(self, dt):
+        lib().oracle_set_dt(self.h, dt)
+
+    def set_time(self, t):
+        lib().oracle_set_time(self.h, t)
+
+    def scalar(self, which):
+        return lib().oracle_scalar(self.h, which)
+
+    def div_u(self):
+        return lib().oracle_div_u(self.h)
+
+    def download(self, band=True):
+        Ng = self.Ng
+        f = {k: np.zeros(Ng) for k in ("u", "v", "w", "p", "dp", "q")}
+        f["dvel"] = np.zeros(3 * Ng)
+        f["fvel"] = np.zeros(18 * Ng)
+        f["sigma"] = np.zeros(9 * Ng)
+        f["rm_c"] = np.zeros(Ng, dtype=np.uint32)
+        f["rm_x"] = np.zeros(3 * Ng)
+        f["rm_xpred"] = np.zeros(3 * Ng)
+        st = _abi.State()
+        for k in ("u", "v", "w", "p", "dp", "q", "dvel", "fvel", "sigma", "rm_x", "rm_xpred"):
+            setattr(st, k, _dp(f[k]))
+        st.rm_c = f["rm_c"].ctypes.data_as(C.POINTER(C.c_uint32))
+        if band:
+            nb = lib().oracle_band_count(self.h)
+            f["band_cell"] = np.zeros(nb, dtype=np.int32)
+            f["band_c"] = np.zeros(nb, dtype=np.uint32)
+            f["band_x"] = np.zeros(3 * nb)
+            f["band_xpred"] = np.zeros(3 * nb)
+            st.nband = nb
+            st.band_cell = f["band_cell"].ctypes.data_as(C.POINTER(C.c_int32))
+            st.band_c = f["band_c"].ctypes.data_as(C.POINTER(C.c_uint32))
+            st.band_x = _dp(f["band_x"])
+            st.band_xpred = _dp(f["band_xpred"])
+        lib().oracle_download(self.h, C.byref(st))
+        return f
+
+    def upload(self, **fields):
+        st = _abi.State()
+        keep = []
+        for k, a in fields.items():
+            if k == "rm_c":
+                a = np.ascontiguousarray(a, dtype=np.uint32)
+                st.rm_c = a.ctypes.data_as(C.POINTER(C.c_uint32))
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                setattr(st, k, _dp(a))
+            keep.append(a)
+        lib().oracle_upload(self.h, C.byref(st))
+
+    def mg_levels(self, which):
+        d = (C.c_int * 48)()
+        n = lib().oracle_mg_levels(self.h, which, d)
+        return [tuple(d[3 * l:3 * l + 3]) for l in range(n)]
+
+    def mg_solve(self, which, rhs, x0, tol=0.0):
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        nc = C.c_int()
+        res = C.c_double()
+        lib().oracle_mg_solve(self.h, which, _dp(r), _dp(x), tol, C.byref(nc), C.byref(res))
+        return x, nc.value, res.value
+
+    def mg_apply(self, which, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        lib().oracle_mg_apply(self.h, which, _dp(x), _dp(y))
+        return y

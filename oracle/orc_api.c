@@ -1,0 +1,159 @@
+/*
+ * ORACLE -- test infrastructure only (see orc.h).
+ *
+ * Flat entry points loaded by tests/ and bench.py's cpu_baseline leg through
+ * ctypes.  State arrays use the rmt3d_state layout of include/rmt3d.h.
+ */
+#include <stdlib.h>
+#include <string.h>
+#include "orc.h"
+
+double orc_mono_diff(double u0, double u1, double u2, double u3, double u4);
+void orc_godunov_set_velocity(double *nl, double *vl, double *wl, double *nr, double *vr, double *wr);
+void orc_godunov_set_refmap(double nl, double nr, double *ml, double *mvl, double *mwl,
+                            double *mr, double *mvr, double *mwr);
+void orc_mat_inverse(const double v[9], double m2[9]);
+void orc_mat_aat(const double F[9], double s[6]);
+void orc_mat_ata(const double F[9], double s[6]);
+
+orc_sim *oracle_create(const rmt3d_config *cfg, const rmt3d_object *objs) { return orc_create(cfg, objs); }
+void oracle_destroy(orc_sim *S) { orc_destroy(S); }
+int oracle_initialize(orc_sim *S) { return orc_initialize(S); }
+int oracle_init_iter(orc_sim *S, int e) { return orc_init_iter(S, e); }
+int oracle_prepare_run(orc_sim *S, int e, double *dt, int *tot) { return orc_prepare_run(S, e, dt, tot); }
+int oracle_step(orc_sim *S, int debug) { return orc_step(S, debug); }
+double oracle_get_dt(orc_sim *S) { return S->dt; }
+double oracle_get_time(orc_sim *S) { return S->time; }
+void oracle_set_dt(orc_sim *S, double dt) { S->dt = dt; S->cfg.dt = dt; }
+void oracle_set_time(orc_sim *S, double t) { S->time = t; }
+double oracle_div_u(orc_sim *S) { return orc_div_u(S); }
+int oracle_band_count(orc_sim *S) { return S->ext.n; }
+void oracle_compute_stress(orc_sim *S) { orc_compute_stress(S); }
+
+/* scalars: 0 dt_reg, 1 eps, 2 K_stiff, 3 vmax, 4 mac cycles, 5 fem cycles,
+ * 6 mac res, 7 fem res, 8 nmask */
+double oracle_scalar(orc_sim *S, int which)
+{
+    switch (which) {
+    case 0: return S->dt_reg;
+    case 1: return S->eps;
+    case 2: return S->K_stiff;
+    case 3: return S->vmax;
+    case 4: return S->mac_cycles;
+    case 5: return S->fem_cycles;
+    case 6: return S->mac_res;
+    case 7: return S->fem_res;
+    case 8: return (double)S->nmask;
+    }
+    return 0;
+}
+
+int oracle_download(orc_sim *S, rmt3d_state *st)
+{
+    int Ng = S->Ng;
+    for (int a = 0; a < Ng; a++) {
+        orc_field *f = S->u + a;
+        orc_rmap *r = S->rm + a;
+        if (st->u) st->u[a] = f->vel[0];
+        if (st->v) st->v[a] = f->vel[1];
+        if (st->w) st->w[a] = f->vel[2];
+        if (st->p) st->p[a] = f->p;
+        if (st->dp) st->dp[a] = f->dp;
+        if (st->q) st->q[a] = f->q;
+        if (st->dvel) for (int c = 0; c < 3; c++) st->dvel[(size_t)c * Ng + a] = f->dvel[c];
+        if (st->fvel) for (int c = 0; c < 18; c++) st->fvel[(size_t)c * Ng + a] = f->fvel[c / 3][c % 3];
+        if (st->sigma) for (int c = 0; c < 9; c++) st->sigma[(size_t)c * Ng + a] = f->sigma[c / 3][c % 3];
+        if (st->rm_c) st->rm_c[a] = r->c;
+        if (st->rm_x) for (int c = 0; c < 3; c++) st->rm_x[(size_t)c * Ng + a] = r->x[c];
+        if (st->rm_xpred) for (int c = 0; c < 3; c++) st->rm_xpred[(size_t)c * Ng + a] = r->xpred[c];
+    }
+    int nb = S->ext.n;
+    if (st->band_cell || st->band_c || st->band_x || st->band_xpred) {
+        if (st->nband < nb) { st->nband = nb; return -1; }
+        for (int a = 0; a < nb; a++) {
+            orc_bentry *e = &S->ext.e[a];
+            if (st->band_cell) st->band_cell[a] = e->cell;
+            if (st->band_c) st->band_c[a] = e->r.c;
+            if (st->band_x) for (int c = 0; c < 3; c++) st->band_x[(size_t)c * nb + a] = e->r.x[c];
+            if (st->band_xpred) for (int c = 0; c < 3; c++) st->band_xpred[(size_t)c * nb + a] = e->r.xpred[c];
+        }
+    }
+    st->nband = nb;
+    return 0;
+}
+
+int oracle_upload(orc_sim *S, const rmt3d_state *st)
+{
+    int Ng = S->Ng;
+    for (int a = 0; a < Ng; a++) {
+        orc_field *f = S->u + a;
+        orc_rmap *r = S->rm + a;
+        if (st->u) f->vel[0] = st->u[a];
+        if (st->v) f->vel[1] = st->v[a];
+        if (st->w) f->vel[2] = st->w[a];
+        if (st->p) f->p = st->p[a];
+        if (st->dp) f->dp = st->dp[a];
+        if (st->q) f->q = st->q[a];
+        if (st->dvel) for (int c = 0; c < 3; c++) f->dvel[c] = st->dvel[(size_t)c * Ng + a];
+        if (st->fvel) for (int c = 0; c < 18; c++) f->fvel[c / 3][c % 3] = st->fvel[(size_t)c * Ng + a];
+        if (st->sigma) for (int c = 0; c < 9; c++) f->sigma[c / 3][c % 3] = st->sigma[(size_t)c * Ng + a];
+        if (st->rm_c) r->c = st->rm_c[a];
+        if (st->rm_x) for (int c = 0; c < 3; c++) r->x[c] = st->rm_x[(size_t)c * Ng + a];
+        if (st->rm_xpred) for (int c = 0; c < 3; c++) r->xpred[c] = st->rm_xpred[(size_t)c * Ng + a];
+    }
+    return 0;
+}
+
+/* Multigrid solve on a stand-alone problem: which 0 MAC, 1 FEM.  Uses the
+ * simulation's solver objects (initial guess = x on entry). */
+int oracle_mg_solve(orc_sim *S, int which, const double *rhs, double *x, double tol, int *ncyc, double *res)
+{
+    orc_mg *G = which ? &S->fem : &S->mac;
+    orc_mglev *L = &G->lev[0];
+    memcpy(L->b, rhs, sizeof(double) * (size_t)L->nn);
+    memcpy(L->x, x, sizeof(double) * (size_t)L->nn);
+    double save = G->max_norm_tol;
+    if (tol > 0) G->max_norm_tol = tol;
+    double r = orc_mg_run(G);
+    G->max_norm_tol = save;
+    memcpy(x, L->x, sizeof(double) * (size_t)L->nn);
+    if (ncyc) *ncyc = G->last_cycles;
+    if (res) *res = r;
+    return 0;
+}
+
+/* y = A x on the finest level of a problem */
+int oracle_mg_apply(orc_sim *S, int which, const double *x, double *y)
+{
+    orc_mg *G = which ? &S->fem : &S->mac;
+    orc_mg_apply(G, 0, x, y);
+    return 0;
+}
+
+int oracle_mg_levels(orc_sim *S, int which, int *dims)
+{
+    orc_mg *G = which ? &S->fem : &S->mac;
+    for (int l = 0; l < G->nlev; l++)
+        for (int a = 0; a < 3; a++) dims[3 * l + a] = G->lev[l].d[a];
+    return G->nlev;
+}
+
+/* unit-level entry points for known-answer tests */
+double oracle_mono_diff(double u0, double u1, double u2, double u3, double u4)
+{
+    return orc_mono_diff(u0, u1, u2, u3, u4);
+}
+void oracle_godunov_vel(double *v6) { orc_godunov_set_velocity(v6, v6 + 1, v6 + 2, v6 + 3, v6 + 4, v6 + 5); }
+void oracle_godunov_refmap(double nl, double nr, double *m6)
+{
+    orc_godunov_set_refmap(nl, nr, m6, m6 + 1, m6 + 2, m6 + 3, m6 + 4, m6 + 5);
+}
+void oracle_mat_inverse(const double *v, double *o) { orc_mat_inverse(v, o); }
+void oracle_mat_aat(const double *v, double *o) { orc_mat_aat(v, o); }
+void oracle_mat_ata(const double *v, double *o) { orc_mat_ata(v, o); }
+double oracle_heaviside(orc_sim *S, double phi) { return orc_heaviside(S, phi); }
+int oracle_lsq(int npt, const double *pos, const double *val, const double *w, double wsum,
+               double *davg, double *ravg, double *coeff, double *det)
+{
+    return orc_least_square_fit(npt, pos, val, w, wsum, davg, ravg, coeff, det);
+}

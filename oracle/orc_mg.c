@@ -1,0 +1,356 @@
+/*
+ * ORACLE -- test infrastructure only (see orc.h).
+ *
+ * Geometric multigrid for the two projection problems of the step.
+ *
+ * Operators (reference, closed form):
+ *  MAC  simp_mac::setup (Utilities/mg_probs.hh:505-527): 7-point stencil,
+ *       centre -6, faces +1, cell-centred unknowns m x n x o; at a wall the
+ *       missing face entry is added to the centre (stencil_set::set_partials
+ *       with macfix, mg_probs.hh:281-343) = homogeneous Neumann.
+ *  FEM  simp_fem::setup (mg_probs.hh:466-492): 27-point nodal Q1 stencil,
+ *       centre 32, edges -2, corners -1, faces 0; at a wall node every entry
+ *       whose offset is 0 in that wall's direction is halved and outward
+ *       entries are absent (set_partials(false)).
+ *  Periodic dimensions wrap.
+ *
+ * Solve loop: mg_prob::run (Utilities/mg_probs.cc:3-83) with the mg_pred
+ * iteration predictor (mg_probs.hh:37-78), max-norm stopping rule, hard cap
+ * of 20 extra batches and two insurance cycles; v_up = v_down = 2
+ * (mg_opt defaults, mg_probs.hh:92-93).
+ *
+ * V-cycle (this project's design; PGMG's is absent, so its iterates are
+ * parity unpinned): MAC -- red-black Gauss-Seidel, restriction = sum of the
+ * 8 children / 2, cell-centred trilinear prolongation; FEM -- 8-colour
+ * (2x2x2 parity) Gauss-Seidel, full-weighting restriction (= P^T) with the
+ * Galerkin coarse operator, which for nested Q1 spaces equals twice the fine
+ * stencil (coarse rhs = R r / 2), vertex trilinear prolongation.  Coarsest
+ * level: 64 Gauss-Seidel sweeps.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include "orc.h"
+
+#define NCOARSE_SWEEPS 64
+
+static int can_coarsen(int fem, int d, int prd)
+{
+    if (!fem) return (d % 2 == 0) && d / 2 >= 2;
+    if (prd) return (d % 2 == 0) && d / 2 >= 2;
+    return (d % 2 == 1) && (d - 1) / 2 + 1 >= 2 && d >= 3;
+}
+
+static int coarse_dim(int fem, int d, int prd)
+{
+    if (!fem || prd) return d / 2;
+    return (d - 1) / 2 + 1;
+}
+
+void orc_mg_setup(orc_mg *G, int fem, const int d[3], const int prd[3])
+{
+    memset(G, 0, sizeof *G);
+    G->fem = fem;
+    for (int a = 0; a < 3; a++) G->prd[a] = prd[a];
+    int cur[3] = {d[0], d[1], d[2]};
+    int l = 0;
+    for (;;) {
+        orc_mglev *L = &G->lev[l];
+        for (int a = 0; a < 3; a++) L->d[a] = cur[a];
+        L->nn = cur[0] * cur[1] * cur[2];
+        L->x = (double *)calloc((size_t)L->nn, sizeof(double));
+        L->b = (double *)calloc((size_t)L->nn, sizeof(double));
+        L->r = (double *)calloc((size_t)L->nn, sizeof(double));
+        l++;
+        int ok = l < 16;
+        for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
+        if (!ok) break;
+        for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
+    }
+    G->nlev = l;
+    double eps = DBL_EPSILON;
+    double c = fem ? 32.0 : 6.0;
+    G->tol = c * c * eps * eps * 1e4;
+    G->max_norm_tol = c * eps * 1e2;
+    G->log_tol = log10(G->tol);
+    G->v_up = 2;
+    G->v_down = 2;
+    G->iP = 16;
+    G->iV = 2 * G->iP;
+    G->V_avg = G->iV;
+    G->V_var = 1e6;
+    G->alpha = 1 / (3 * G->iP); /* integer division in the reference: 0 */
+}
+
+void orc_mg_free(orc_mg *G)
+{
+    for (int l = 0; l < G->nlev; l++) {
+        free(G->lev[l].x); free(G->lev[l].b); free(G->lev[l].r);
+    }
+    G->nlev = 0;
+}
+
+/* neighbour index along one dimension: -1 if outside a wall */
+static inline int nb(int i, int di, int d, int prd)
+{
+    int j = i + di;
+    if (j < 0) return prd ? j + d : -1;
+    if (j >= d) return prd ? j - d : -1;
+    return j;
+}
+
+/* ---- MAC operator ---- */
+static inline double mac_Ax(const orc_mglev *L, const int *prd, const double *x, int i, int j, int k, double *diag, double *offsum)
+{
+    int D0 = L->d[0], D1 = L->d[1], D2 = L->d[2];
+    int c = i + D0 * (j + D1 * k);
+    double s = 0;
+    int miss = 0;
+    int ii;
+    ii = nb(i, -1, D0, prd[0]); if (ii < 0) miss++; else s += x[ii + D0 * (j + D1 * k)];
+    ii = nb(i, 1, D0, prd[0]);  if (ii < 0) miss++; else s += x[ii + D0 * (j + D1 * k)];
+    ii = nb(j, -1, D1, prd[1]); if (ii < 0) miss++; else s += x[i + D0 * (ii + D1 * k)];
+    ii = nb(j, 1, D1, prd[1]);  if (ii < 0) miss++; else s += x[i + D0 * (ii + D1 * k)];
+    ii = nb(k, -1, D2, prd[2]); if (ii < 0) miss++; else s += x[i + D0 * (j + D1 * ii)];
+    ii = nb(k, 1, D2, prd[2]);  if (ii < 0) miss++; else s += x[i + D0 * (j + D1 * ii)];
+    double dg = -6.0 + miss;
+    if (diag) *diag = dg;
+    if (offsum) *offsum = s;
+    return s + dg * x[c];
+}
+
+/* ---- FEM operator ---- */
+static inline double fem_w(int oi, int oj, int ok)
+{
+    int nz = (oi != 0) + (oj != 0) + (ok != 0);
+    if (nz == 0) return 32.0;
+    if (nz == 1) return 0.0;
+    if (nz == 2) return -2.0;
+    return -1.0;
+}
+
+static inline double fem_Ax(const orc_mglev *L, const int *prd, const double *x, int i, int j, int k, double *diag, double *offsum)
+{
+    int D0 = L->d[0], D1 = L->d[1], D2 = L->d[2];
+    int c[3] = {i, j, k}, D[3] = {D0, D1, D2};
+    int wall[3];
+    for (int a = 0; a < 3; a++) wall[a] = !prd[a] && (c[a] == 0 || c[a] == D[a] - 1);
+    double s = 0, dg = 0;
+    for (int ok = -1; ok <= 1; ok++) {
+        int kk = nb(k, ok, D2, prd[2]);
+        if (kk < 0) continue;
+        for (int oj = -1; oj <= 1; oj++) {
+            int jj = nb(j, oj, D1, prd[1]);
+            if (jj < 0) continue;
+            for (int oi = -1; oi <= 1; oi++) {
+                int ii = nb(i, oi, D0, prd[0]);
+                if (ii < 0) continue;
+                double w = fem_w(oi, oj, ok);
+                if (w == 0.0) continue;
+                double f = 1;
+                if (wall[0] && oi == 0) f *= 2;
+                if (wall[1] && oj == 0) f *= 2;
+                if (wall[2] && ok == 0) f *= 2;
+                w /= f;
+                if (oi == 0 && oj == 0 && ok == 0) dg = w;
+                else s += w * x[ii + D0 * (jj + D1 * kk)];
+            }
+        }
+    }
+    if (diag) *diag = dg;
+    if (offsum) *offsum = s;
+    return s + dg * x[i + D0 * (j + D1 * k)];
+}
+
+void orc_mg_apply(const orc_mg *G, int lev, const double *x, double *y)
+{
+    const orc_mglev *L = &G->lev[lev];
+    for (int k = 0; k < L->d[2]; k++)
+        for (int j = 0; j < L->d[1]; j++)
+            for (int i = 0; i < L->d[0]; i++)
+                y[i + L->d[0] * (j + L->d[1] * k)] = G->fem ? fem_Ax(L, G->prd, x, i, j, k, NULL, NULL)
+                                                            : mac_Ax(L, G->prd, x, i, j, k, NULL, NULL);
+}
+
+static void residual(orc_mg *G, int lev)
+{
+    orc_mglev *L = &G->lev[lev];
+    for (int k = 0; k < L->d[2]; k++)
+        for (int j = 0; j < L->d[1]; j++)
+            for (int i = 0; i < L->d[0]; i++) {
+                int c = i + L->d[0] * (j + L->d[1] * k);
+                double ax = G->fem ? fem_Ax(L, G->prd, L->x, i, j, k, NULL, NULL)
+                                   : mac_Ax(L, G->prd, L->x, i, j, k, NULL, NULL);
+                L->r[c] = L->b[c] - ax;
+            }
+}
+
+double orc_mg_resid_max(orc_mg *G, int lev)
+{
+    residual(G, lev);
+    orc_mglev *L = &G->lev[lev];
+    double mx = 0;
+    for (int c = 0; c < L->nn; c++) {
+        double a = fabs(L->r[c]);
+        if (a > mx || isnan(a)) mx = isnan(a) ? NAN : a;
+        if (isnan(mx)) return mx;
+    }
+    return mx;
+}
+
+/* one Gauss-Seidel sweep: MAC red-black, FEM 8-colour */
+static void smooth(orc_mg *G, int lev)
+{
+    orc_mglev *L = &G->lev[lev];
+    int ncol = G->fem ? 8 : 2;
+    for (int col = 0; col < ncol; col++)
+        for (int k = 0; k < L->d[2]; k++)
+            for (int j = 0; j < L->d[1]; j++)
+                for (int i = 0; i < L->d[0]; i++) {
+                    int mycol = G->fem ? ((i & 1) + 2 * (j & 1) + 4 * (k & 1)) : ((i + j + k) & 1);
+                    if (mycol != col) continue;
+                    int c = i + L->d[0] * (j + L->d[1] * k);
+                    double dg, off;
+                    if (G->fem) fem_Ax(L, G->prd, L->x, i, j, k, &dg, &off);
+                    else mac_Ax(L, G->prd, L->x, i, j, k, &dg, &off);
+                    L->x[c] = (L->b[c] - off) / dg;
+                }
+}
+
+/* restriction of level lev's residual into level lev+1's rhs */
+static void restrict_res(orc_mg *G, int lev)
+{
+    orc_mglev *F = &G->lev[lev], *C = &G->lev[lev + 1];
+    for (int K = 0; K < C->d[2]; K++)
+        for (int J = 0; J < C->d[1]; J++)
+            for (int I = 0; I < C->d[0]; I++) {
+                double s = 0;
+                if (!G->fem) {
+                    for (int c = 0; c < 2; c++)
+                        for (int b = 0; b < 2; b++)
+                            for (int a = 0; a < 2; a++)
+                                s += F->r[(2 * I + a) + F->d[0] * ((2 * J + b) + F->d[1] * (2 * K + c))];
+                    s *= 0.5;
+                } else {
+                    for (int dc = -1; dc <= 1; dc++) {
+                        int kk = nb(2 * K, dc, F->d[2], G->prd[2]);
+                        if (kk < 0) continue;
+                        for (int db = -1; db <= 1; db++) {
+                            int jj = nb(2 * J, db, F->d[1], G->prd[1]);
+                            if (jj < 0) continue;
+                            for (int da = -1; da <= 1; da++) {
+                                int ii = nb(2 * I, da, F->d[0], G->prd[0]);
+                                if (ii < 0) continue;
+                                double w = (da ? 0.5 : 1.0) * (db ? 0.5 : 1.0) * (dc ? 0.5 : 1.0);
+                                s += w * F->r[ii + F->d[0] * (jj + F->d[1] * kk)];
+                            }
+                        }
+                    }
+                    s *= 0.5;
+                }
+                int ci = I + C->d[0] * (J + C->d[1] * K);
+                C->b[ci] = s;
+                C->x[ci] = 0;
+            }
+}
+
+/* cell-centred 1D prolongation weights: fine index f = 2I + a */
+static inline void cc_w(int f, int D, int prd, int *i0, int *i1, double *w0, double *w1)
+{
+    int I = f >> 1, a = f & 1;
+    int o = a ? I + 1 : I - 1;
+    int oo = nb(I, o - I, D, prd);
+    *i0 = I; *w0 = 0.75;
+    if (oo < 0) { *i1 = I; *w1 = 0.25; } else { *i1 = oo; *w1 = 0.25; }
+}
+
+/* vertex 1D prolongation: fine index f */
+static inline void vx_w(int f, int D, int prd, int *i0, int *i1, double *w0, double *w1)
+{
+    int I = f >> 1;
+    if ((f & 1) == 0) { *i0 = I; *i1 = I; *w0 = 1.0; *w1 = 0.0; return; }
+    int J = I + 1;
+    if (J >= D) J = prd ? J - D : D - 1;
+    *i0 = I; *i1 = J; *w0 = 0.5; *w1 = 0.5;
+}
+
+static void prolong_add(orc_mg *G, int lev)
+{
+    orc_mglev *F = &G->lev[lev], *C = &G->lev[lev + 1];
+    for (int k = 0; k < F->d[2]; k++) {
+        int k0, k1; double wk0, wk1;
+        if (G->fem) vx_w(k, C->d[2], G->prd[2], &k0, &k1, &wk0, &wk1);
+        else cc_w(k, C->d[2], G->prd[2], &k0, &k1, &wk0, &wk1);
+        for (int j = 0; j < F->d[1]; j++) {
+            int j0, j1; double wj0, wj1;
+            if (G->fem) vx_w(j, C->d[1], G->prd[1], &j0, &j1, &wj0, &wj1);
+            else cc_w(j, C->d[1], G->prd[1], &j0, &j1, &wj0, &wj1);
+            for (int i = 0; i < F->d[0]; i++) {
+                int i0, i1; double wi0, wi1;
+                if (G->fem) vx_w(i, C->d[0], G->prd[0], &i0, &i1, &wi0, &wi1);
+                else cc_w(i, C->d[0], G->prd[0], &i0, &i1, &wi0, &wi1);
+                const double *e = C->x;
+                int D0 = C->d[0], D1 = C->d[1];
+#define E(a, b, c) e[(a) + D0 * ((b) + D1 * (c))]
+                double v = wk0 * (wj0 * (wi0 * E(i0, j0, k0) + wi1 * E(i1, j0, k0)) +
+                                  wj1 * (wi0 * E(i0, j1, k0) + wi1 * E(i1, j1, k0))) +
+                           wk1 * (wj0 * (wi0 * E(i0, j0, k1) + wi1 * E(i1, j0, k1)) +
+                                  wj1 * (wi0 * E(i0, j1, k1) + wi1 * E(i1, j1, k1)));
+#undef E
+                F->x[i + F->d[0] * (j + F->d[1] * k)] += v;
+            }
+        }
+    }
+}
+
+static void vcycle_rec(orc_mg *G, int lev)
+{
+    if (lev == G->nlev - 1) {
+        for (int s = 0; s < NCOARSE_SWEEPS; s++) smooth(G, lev);
+        return;
+    }
+    for (int s = 0; s < G->v_down; s++) smooth(G, lev);
+    residual(G, lev);
+    restrict_res(G, lev);
+    vcycle_rec(G, lev + 1);
+    prolong_add(G, lev);
+    for (int s = 0; s < G->v_up; s++) smooth(G, lev);
+}
+
+void orc_mg_vcycle(orc_mg *G) { vcycle_rec(G, 0); }
+
+/* mg_pred::update(int) (mg_probs.hh:51-66) */
+static void pred_update(orc_mg *G, int extras)
+{
+    double dV = G->iV - G->V_avg;
+    int exreduce = (int)(dV / sqrt(G->V_var));
+    exreduce = exreduce > 0 ? exreduce : 0;
+    G->iV += G->iP * extras * (extras + 1) / 2 - 1 - exreduce;
+    if (G->iV < G->iP) G->iV = 2 * G->iP - 1;
+    G->V_avg = (1 - G->alpha) * G->V_avg + G->alpha * G->iV;
+    G->V_var = (1 - G->alpha) * (G->V_var + G->alpha * dV * dV);
+}
+
+/* mg_prob::run (mg_probs.cc:3-83), profile = false.  Returns the max-norm
+ * residual measured before the two insurance cycles, as the reference. */
+double orc_mg_run(orc_mg *G)
+{
+    const int hard_cap = 20;
+    int cycles = 0;
+    double res = orc_mg_resid_max(G, 0);
+    int pre = G->iV / G->iP;
+    for (int i = 0; i < pre; i++) { orc_mg_vcycle(G); cycles++; }
+    res = orc_mg_resid_max(G, 0);
+    int extra = 0;
+    while (res > G->max_norm_tol && extra < hard_cap) {
+        extra++;
+        for (int n = 0; n < extra; n++) { orc_mg_vcycle(G); cycles++; }
+        res = orc_mg_resid_max(G, 0);
+    }
+    for (int i = 0; i < 2; i++) { orc_mg_vcycle(G); cycles++; }
+    pred_update(G, extra);
+    G->last_cycles = cycles;
+    G->last_res = res;
+    return res;
+}
