@@ -185,6 +185,15 @@ int rmt3d_stage_times(rmt3d_ctx *ctx, double *ms, int n);
 /* Stream the context's kernels run on (a hipStream_t). */
 void *rmt3d_stream(rmt3d_ctx *ctx);
 
+/* Event timing of kernel classes on the context stream (measurement only;
+ * no counterpart in the reference, whose timer buckets are wall clocks,
+ * Utilities/timer.hh).  which: 0 finest-level FEM smoother colour pass,
+ * 1 finest-level MAC smoother colour pass, 2 compute_stress, 3 advection. */
+int rmt3d_profile(rmt3d_ctx *ctx, int enable);
+int rmt3d_profile_read(rmt3d_ctx *ctx, int which, double *ms, int *launches);
+/* hipDeviceSynchronize on the context's device. */
+int rmt3d_device_synchronize(rmt3d_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

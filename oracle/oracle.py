@@ -23,7 +23,7 @@ def build(force=False):
     import subprocess
     srcs = [os.path.join(HERE, s) for s in SOURCES]
     if not force and os.path.exists(LIB_PATH):
-        newest = max(os.path.getmtime(s) for s in srcs + [os.path.join(HERE, "orc.h")])
+        newest = max(os.path.getmtime(s) for s in srcs + [os.path.join(HERE, "orc.h"), os.path.join(HERE, "orc_detmath.h")])
         if os.path.getmtime(LIB_PATH) >= newest:
             return LIB_PATH
     cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-D_GNU_SOURCE",

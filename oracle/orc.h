@@ -163,6 +163,7 @@ int orc_init_iter(orc_sim *S, int init_err);
 int orc_prepare_run(orc_sim *S, int init_err, double *dt, int *total);
 int orc_step(orc_sim *S, int debug);
 double orc_div_u(orc_sim *S);
+double orc_blocked_sum(const double *a, int n, double scale);
 double orc_phi(const orc_sim *S, int obj_index, const double xi[3]);
 double orc_heaviside(const orc_sim *S, double phi);
 void orc_compute_stress(orc_sim *S);

@@ -11,10 +11,36 @@
 #include <stdlib.h>
 #include <string.h>
 #include "orc.h"
+#include "orc_detmath.h"
 
 #define OMASK 1023u
 #define LDIG 10
 #define SMALL_NUMBER (100. * DBL_EPSILON)   /* defs.hh:45 */
+
+/* Sum of a[e]*scale in the fixed blocked order of the HIP reduction: 256
+ * chunks of ceil(n/256) elements, 256 strided partial sums per chunk folded
+ * by a binary tree, chunk sums added in order.  The reference sums
+ * sequentially (fluid_3d.cc:2698-2699); the blocked order differs only in
+ * rounding and makes the HIP path and this oracle bit-identical. */
+double orc_blocked_sum(const double *a, int n, double scale)
+{
+    const int NB = 256, NT = 256;
+    int per = (n + NB - 1) / NB;
+    double tot = 0;
+    for (int b = 0; b < NB; b++) {
+        double sh[256];
+        int s0 = b * per, e0 = s0 + per < n ? s0 + per : n;
+        for (int t = 0; t < NT; t++) {
+            double v = 0;
+            for (int e = s0 + t; e < e0; e += NT) v += a[e] * scale;
+            sh[t] = v;
+        }
+        for (int off = NT / 2; off > 0; off >>= 1)
+            for (int t = 0; t < off; t++) sh[t] += sh[t + off];
+        tot += sh[0];
+    }
+    return tot;
+}
 
 static inline int IX(const orc_sim *S, int i, int j, int k) { return i + S->m4 * (j + S->n4 * k); }
 static inline double dmin(double a, double b) { return (a < b) ? a : b; }
@@ -46,7 +72,7 @@ double orc_heaviside(const orc_sim *S, double phi)
 {
     if (phi >= S->eps) return 0;
     if (phi <= -S->eps) return 1;
-    double tmp = 0.5 - phi * S->tf1 - sin(S->tf3 * phi) * S->tf2;
+    double tmp = 0.5 - phi * S->tf1 - det_sin(S->tf3 * phi) * S->tf2;
     if (tmp < 0) tmp = 0.;
     else if (tmp > 1) tmp = 1;
     return tmp;
@@ -56,7 +82,7 @@ double orc_heaviside(const orc_sim *S, double phi)
 static inline double tderiv_func_in(const orc_sim *S, double x)
 {
     double tmp = 0;
-    if (x > -S->eps && x < S->eps) tmp = 0.5 + 0.5 * cos(x * S->tf3);
+    if (x > -S->eps && x < S->eps) tmp = 0.5 + 0.5 * det_cos(x * S->tf3);
     return tmp;
 }
 
@@ -83,9 +109,9 @@ static void obj_rm0(const orc_obj *O, const double x[3], double xi[3])
 /* sim_ftest::velocity (sim_manager.cc:139-146) */
 static void ftest_vel(double x, double y, double z, double *u, double *v, double *w)
 {
-    *u = -sin(2 * M_PI * x) * cos(2 * M_PI * y) * cos(2 * M_PI * z);
-    *v = 0.5 * cos(2 * M_PI * x) * sin(2 * M_PI * y) * cos(2 * M_PI * z);
-    *w = 0.5 * cos(2 * M_PI * x) * cos(2 * M_PI * y) * sin(2 * M_PI * z);
+    *u = -det_sin(2 * M_PI * x) * det_cos(2 * M_PI * y) * det_cos(2 * M_PI * z);
+    *v = 0.5 * det_cos(2 * M_PI * x) * det_sin(2 * M_PI * y) * det_cos(2 * M_PI * z);
+    *w = 0.5 * det_cos(2 * M_PI * x) * det_cos(2 * M_PI * y) * det_sin(2 * M_PI * z);
 }
 
 /* sim_manager::velocity / sim_ftest::velocity (sim_manager.cc:102-146) */
@@ -127,7 +153,7 @@ static void mgmt_velocity(orc_sim *S, double x, double y, double z, double *u, d
 static double mgmt_pressure(const orc_sim *S, double x, double y, double z)
 {
     if (S->cfg.sim_type == RMT3D_SIM_FTEST || S->cfg.sim_type == RMT3D_SIM_MILLS)
-        return 3 * M_PI * S->cfg.fmu * cos(2 * M_PI * x) * cos(2 * M_PI * y) * cos(2 * M_PI * z);
+        return 3 * M_PI * S->cfg.fmu * det_cos(2 * M_PI * x) * det_cos(2 * M_PI * y) * det_cos(2 * M_PI * z);
     return 0;
 }
 
@@ -137,9 +163,9 @@ static void mgmt_exact(const orc_sim *S, double x, double y, double z, double t,
 {
     if (S->cfg.sim_type == RMT3D_SIM_FTEST || S->cfg.sim_type == RMT3D_SIM_MILLS) {
         ftest_vel(x, y, z, u, v, w);
-        *u *= cos(2 * M_PI * t); *v *= cos(2 * M_PI * t); *w *= cos(2 * M_PI * t);
+        *u *= det_cos(2 * M_PI * t); *v *= det_cos(2 * M_PI * t); *w *= det_cos(2 * M_PI * t);
         x -= 0.5 * h; y -= 0.5 * h; z -= 0.5 * h;
-        *p = mgmt_pressure(S, x, y, z) * (cos(2 * M_PI * t) - sin(2 * M_PI * t) / (6 * M_PI * S->cfg.fmu));
+        *p = mgmt_pressure(S, x, y, z) * (det_cos(2 * M_PI * t) - det_sin(2 * M_PI * t) / (6 * M_PI * S->cfg.fmu));
         return;
     }
     *u = *v = *w = *p = 0;
@@ -152,10 +178,10 @@ static void mgmt_fluid_acc(const orc_sim *S, double x, double y, double z, doubl
     int st = S->cfg.sim_type;
     if (st == RMT3D_SIM_FTEST || st == RMT3D_SIM_MILLS) {
         double mu = S->cfg.fmu, P = M_PI;
-        f[0] = (12 * P * cos(2 * P * y) * cos(2 * P * z) * (-6 * P * mu * cos(2 * P * t) + sin(2 * P * t)) * sin(2 * P * x) +
-                P * cos(2 * P * t) * cos(2 * P * t) * (2 + cos(4 * P * y) + cos(4 * P * z)) * sin(4 * P * x)) / 4;
-        f[1] = -(P * cos(2 * P * t) * cos(2 * P * t) * (-1 + cos(4 * P * x) - 2 * cos(4 * P * z)) * sin(4 * P * y)) / 8;
-        f[2] = -(P * cos(2 * P * t) * cos(2 * P * t) * (-1 + cos(4 * P * x) - 2 * cos(4 * P * y)) * sin(4 * P * z)) / 8;
+        f[0] = (12 * P * det_cos(2 * P * y) * det_cos(2 * P * z) * (-6 * P * mu * det_cos(2 * P * t) + det_sin(2 * P * t)) * det_sin(2 * P * x) +
+                P * det_cos(2 * P * t) * det_cos(2 * P * t) * (2 + det_cos(4 * P * y) + det_cos(4 * P * z)) * det_sin(4 * P * x)) / 4;
+        f[1] = -(P * det_cos(2 * P * t) * det_cos(2 * P * t) * (-1 + det_cos(4 * P * x) - 2 * det_cos(4 * P * z)) * det_sin(4 * P * y)) / 8;
+        f[2] = -(P * det_cos(2 * P * t) * det_cos(2 * P * t) * (-1 + det_cos(4 * P * x) - 2 * det_cos(4 * P * y)) * det_sin(4 * P * z)) / 8;
         return;
     }
     f[0] = f[1] = f[2] = 0;
@@ -871,10 +897,7 @@ static int pp_solve(orc_sim *S)
     if (isnan(res)) return 1;
     int fm = S->fm, fn = S->fn, fo = S->fo;
     double mnoxsp = 1 / (double)(fm * fn * fo);
-    double lm = 0;
-    for (int k = 0; k < fo; k++)
-        for (int j = 0; j < fn; j++)
-            for (int i = 0; i < fm; i++) lm += L->x[i + fm * (j + fn * k)] * mnoxsp;
+    double lm = orc_blocked_sum(L->x, fm * fn * fo, mnoxsp);
     for (int k = 0; k < fo; k++)
         for (int j = 0; j < fn; j++)
             for (int i = 0; i < fm; i++) {
@@ -1308,7 +1331,6 @@ double orc_div_u(orc_sim *S)
     int nn = S->fm * S->fn * S->fo;
     double *r = (double *)malloc(sizeof(double) * (size_t)nn);
     fill_pp_rhs(S, r);
-    double l2 = 0;
     int id = 0;
     for (int k = 0; k < S->fo; k++) {
         double nk = (k == 0 || k == S->fo - 1) ? 0.5 : 1.0;
@@ -1320,11 +1342,12 @@ double orc_div_u(orc_sim *S)
                 double ni = (i == 0 || i == S->fm - 1) ? 0.5 : 1.0;
                 if (S->prd[0]) ni = 1.0;
                 double wt = ni * nj * nk;
-                l2 += wt * r[id] * r[id];
+                r[id] = wt * r[id] * r[id];
                 id++;
             }
         }
     }
+    double l2 = orc_blocked_sum(r, nn, 1.0);
     l2 /= nn;
     free(r);
     return sqrt(l2);

@@ -1,0 +1,856 @@
+// Host orchestration of the MI355X FSI step (see fluid3d.h).
+#include "fluid3d.h"
+#include <float.h>
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <cstdio>
+
+namespace rmt {
+
+#define HCHK(x)                                                                              \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) {                                                              \
+            err = std::string(#x) + ": " + hipGetErrorString(e_);                            \
+            return RMT3D_ERR_HIP;                                                            \
+        }                                                                                    \
+    } while (0)
+
+static const int ORC_VEL = 1, ORC_PRES = 2, ORC_FVEL = 4, ORC_C = 8, ORC_X = 16, ORC_XPRED = 32, ORC_FX = 64,
+                 ORC_DVEL = 2048;
+
+// ---------------------------------------------------------------------------
+// multigrid
+// ---------------------------------------------------------------------------
+static bool can_coarsen(int fem, int d, int prd) {
+    if (!fem || prd) return (d % 2 == 0) && d / 2 >= 2;
+    return (d % 2 == 1) && d >= 3 && (d - 1) / 2 + 1 >= 2;
+}
+static int coarse_dim(int fem, int d, int prd) { return (!fem || prd) ? d / 2 : (d - 1) / 2 + 1; }
+
+void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
+    fem = fem_;
+    mixed = mixed_;
+    for (int a = 0; a < 3; a++) prd[a] = prd_[a];
+    int cur[3] = {d[0], d[1], d[2]};
+    nlev = 0;
+    for (;;) {
+        MGLevel<double> &l = L[nlev];
+        for (int a = 0; a < 3; a++) l.d[a] = cur[a];
+        l.nn = cur[0] * cur[1] * cur[2];
+        hipMalloc(&l.x, sizeof(double) * l.nn);
+        hipMalloc(&l.b, sizeof(double) * l.nn);
+        hipMalloc(&l.r, sizeof(double) * l.nn);
+        hipMemset(l.x, 0, sizeof(double) * l.nn);
+        hipMemset(l.b, 0, sizeof(double) * l.nn);
+        hipMemset(l.r, 0, sizeof(double) * l.nn);
+        if (mixed) {
+            MGLevel<float> &f = Lf[nlev];
+            for (int a = 0; a < 3; a++) f.d[a] = cur[a];
+            f.nn = l.nn;
+            hipMalloc(&f.x, sizeof(float) * f.nn);
+            hipMalloc(&f.b, sizeof(float) * f.nn);
+            hipMalloc(&f.r, sizeof(float) * f.nn);
+            hipMemset(f.x, 0, sizeof(float) * f.nn);
+            hipMemset(f.b, 0, sizeof(float) * f.nn);
+            hipMemset(f.r, 0, sizeof(float) * f.nn);
+        }
+        nlev++;
+        bool ok = nlev < 16;
+        for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
+        if (!ok) break;
+        for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
+    }
+    double eps = DBL_EPSILON, c = fem ? 32.0 : 6.0;
+    tol = c * c * eps * eps * 1e4;
+    max_norm_tol = c * eps * 1e2;
+    log_tol = log10(tol);
+    hipMalloc(&d_max, sizeof(unsigned long long));
+    hipHostMalloc((void **)&h_max, sizeof(unsigned long long), hipHostMallocDefault);
+}
+
+void MGSolver::release() {
+    for (int l = 0; l < nlev; l++) {
+        hipFree(L[l].x); hipFree(L[l].b); hipFree(L[l].r);
+        if (mixed) { hipFree(Lf[l].x); hipFree(Lf[l].b); hipFree(Lf[l].r); }
+    }
+    nlev = 0;
+    if (d_max) hipFree(d_max);
+    if (h_max) hipHostFree(h_max);
+    d_max = nullptr; h_max = nullptr;
+}
+
+double MGSolver::resid_max(hipStream_t s) {
+    launch_zero_u64(s, d_max, 1);
+    launch_mg_resid64_max(s, fem, L[0], prd, d_max);
+    hipMemcpyAsync(h_max, d_max, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    double v;
+    memcpy(&v, h_max, sizeof v);
+    return v;
+}
+
+void MGSolver::vcycle64(hipStream_t s, int lev) {
+    int ncol = fem ? 8 : 2;
+    if (lev == nlev - 1) {
+        launch_mg_coarse(s, fem, L[lev], prd, coarse_sweeps);
+        return;
+    }
+    KProf *kp = (lev == 0) ? prof : nullptr;
+    for (int sw = 0; sw < v_down; sw++)
+        for (int c = 0; c < ncol; c++) {
+            if (kp) kp->begin(s);
+            launch_mg_smooth(s, fem, L[lev], prd, c);
+            if (kp) kp->end(s);
+        }
+    launch_mg_resid(s, fem, L[lev], prd);
+    launch_mg_restrict(s, fem, L[lev], L[lev + 1], prd);
+    vcycle64(s, lev + 1);
+    launch_mg_prolong(s, fem, L[lev], L[lev + 1], prd);
+    for (int sw = 0; sw < v_up; sw++)
+        for (int c = 0; c < ncol; c++) {
+            if (kp) kp->begin(s);
+            launch_mg_smooth(s, fem, L[lev], prd, c);
+            if (kp) kp->end(s);
+        }
+}
+
+void MGSolver::vcycle32(hipStream_t s, int lev) {
+    int ncol = fem ? 8 : 2;
+    if (lev == nlev - 1) {
+        launch_mg_coarse(s, fem, Lf[lev], prd, coarse_sweeps);
+        return;
+    }
+    KProf *kp = (lev == 0) ? prof : nullptr;
+    for (int sw = 0; sw < v_down; sw++)
+        for (int c = 0; c < ncol; c++) {
+            if (kp) kp->begin(s);
+            launch_mg_smooth(s, fem, Lf[lev], prd, c);
+            if (kp) kp->end(s);
+        }
+    launch_mg_resid(s, fem, Lf[lev], prd);
+    launch_mg_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
+    vcycle32(s, lev + 1);
+    launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
+    for (int sw = 0; sw < v_up; sw++)
+        for (int c = 0; c < ncol; c++) {
+            if (kp) kp->begin(s);
+            launch_mg_smooth(s, fem, Lf[lev], prd, c);
+            if (kp) kp->end(s);
+        }
+}
+
+// one iteration: an FP64 V-cycle, or (mixed) one FP64 defect-correction step
+// preconditioned by an FP32 V-cycle: r = b - A x (FP64), A e ~= r (FP32), x += e
+void MGSolver::cycle(hipStream_t s) {
+    if (!mixed) {
+        vcycle64(s, 0);
+        return;
+    }
+    launch_mg_resid(s, fem, L[0], prd);
+    launch_cast_d2f(s, L[0].r, Lf[0].b, L[0].nn);
+    launch_zero_f(s, Lf[0].x, Lf[0].nn);
+    vcycle32(s, 0);
+    launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
+}
+
+// mg_prob::run (Utilities/mg_probs.cc:3-83), profile = false
+double MGSolver::run(hipStream_t s) {
+    const int hard_cap = 20;
+    double thr = tol_override > 0 ? tol_override : max_norm_tol;
+    int cycles = 0;
+    double res = resid_max(s);
+    int pre = iV / iP;
+    for (int i = 0; i < pre; i++) { cycle(s); cycles++; }
+    res = resid_max(s);
+    int extra = 0;
+    while (res > thr && extra < hard_cap) {
+        extra++;
+        for (int n = 0; n < extra; n++) { cycle(s); cycles++; }
+        res = resid_max(s);
+    }
+    for (int i = 0; i < 2; i++) { cycle(s); cycles++; }
+    // mg_pred::update(int) (mg_probs.hh:51-66)
+    double dV = iV - V_avg;
+    int exreduce = (int)(dV / sqrt(V_var));
+    exreduce = exreduce > 0 ? exreduce : 0;
+    iV += iP * extra * (extra + 1) / 2 - 1 - exreduce;
+    if (iV < iP) iV = 2 * iP - 1;
+    V_avg = (1 - alpha) * V_avg + alpha * iV;
+    V_var = (1 - alpha) * (V_var + alpha * dV * dV);
+    last_cycles = cycles;
+    last_res = res;
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// construction (fluid_3d.cc:12-77, sim_manager.cc:19-81, sim_manager.hh:363-444)
+// ---------------------------------------------------------------------------
+void *Sim::dalloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    hipMemset(p, 0, bytes);
+    allocs.push_back(p);
+    return p;
+}
+
+void Sim::alloc_band(Band &B, int cap) {
+    int Ng = g.Ng;
+    B.cap = cap;
+    B.start = (int *)dalloc(sizeof(int) * Ng);
+    B.cnt = (int *)dalloc(sizeof(int) * Ng);
+    B.cell = (int *)dalloc(sizeof(int) * cap);
+    B.c = (uint32_t *)dalloc(sizeof(uint32_t) * cap);
+    for (int a = 0; a < 3; a++) {
+        B.x[a] = (double *)dalloc(sizeof(double) * cap);
+        B.xpred[a] = (double *)dalloc(sizeof(double) * cap);
+        B.detF[a] = (double *)dalloc(sizeof(double) * cap);
+        B.en[a] = (double *)dalloc(sizeof(double) * cap);
+    }
+    for (int a = 0; a < 18; a++) B.fx[a] = (double *)dalloc(sizeof(double) * cap);
+    for (int a = 0; a < 9; a++) B.gphi[a] = (double *)dalloc(sizeof(double) * cap);
+    B.n = (int *)dalloc(sizeof(int));
+}
+
+int Sim::profile(int enable) {
+    for (int k = 0; k < 4; k++) {
+        KProf &P = kprof[k];
+        if (!P.ev) {
+            P.ev = new hipEvent_t[2 * KProf::CAP];
+            for (int a = 0; a < 2 * KProf::CAP; a++) hipEventCreate(&P.ev[a]);
+        }
+        P.n = 0;
+        P.enabled = enable;
+    }
+    fem.prof = &kprof[0];
+    mac.prof = &kprof[1];
+    return 0;
+}
+
+int Sim::profile_read(int which, double *ms, int *launches) {
+    if (which < 0 || which > 3) return RMT3D_ERR_ARG;
+    HCHK(hipStreamSynchronize(stream));
+    KProf &P = kprof[which];
+    double tot = 0;
+    for (int a = 0; a < P.n; a++) {
+        float t = 0;
+        hipEventElapsedTime(&t, P.ev[2 * a], P.ev[2 * a + 1]);
+        tot += t;
+    }
+    *ms = tot;
+    *launches = P.n;
+    return 0;
+}
+
+Sim::~Sim() {
+    for (int k = 0; k < 4; k++)
+        if (kprof[k].ev) {
+            for (int a = 0; a < 2 * KProf::CAP; a++) hipEventDestroy(kprof[k].ev[a]);
+            delete[] kprof[k].ev;
+        }
+    mac.release();
+    fem.release();
+    for (void *p : allocs) hipFree(p);
+    if (h_status) hipHostFree(h_status);
+    if (h_int) hipHostFree(h_int);
+    if (h_dbl) hipHostFree(h_dbl);
+    if (timing)
+        for (int a = 0; a < 8; a++) hipEventDestroy(tev[a]);
+    if (stream) hipStreamDestroy(stream);
+}
+
+int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
+    cfg = *c;
+    precision = prec;
+    int st = cfg.sim_type;
+    n_obj = (st == RMT3D_SIM_FTEST) ? 0 : cfg.n_obj;
+    if (n_obj > 64) { err = "more than 64 objects"; return RMT3D_ERR_ARG; }
+    if (st == RMT3D_SIM_STEST) { err = "sim_type stest is not supported"; return RMT3D_ERR_ARG; }
+    objs_in.assign(o, o + n_obj);
+    g.m = cfg.sys_size[0]; g.n = cfg.sys_size[1]; g.o = cfg.sys_size[2];
+    if (g.m < 4 || g.n < 4 || g.o < 4) { err = "grid too small"; return RMT3D_ERR_ARG; }
+    g.m4 = g.m + 4; g.n4 = g.n + 4; g.o4 = g.o + 4;
+    g.mn4 = g.m4 * g.n4;
+    g.Ng = g.mn4 * g.o4;
+    g.G0 = 2 * (1 + g.m4 * (1 + g.n4));
+    for (int d = 0; d < 3; d++) g.prd[d] = cfg.prd[d];
+    g.fm = g.prd[0] ? g.m : g.m + 1;
+    g.fn = g.prd[1] ? g.n : g.n + 1;
+    g.fo = g.prd[2] ? g.o : g.o + 1;
+    const double *b = cfg.xyz_bounds;
+    double Lx = b[1] - b[0], Ly = b[3] - b[2], Lz = b[5] - b[4];
+    g.dx = Lx / g.m; g.dy = Ly / g.n; g.dz = Lz / g.o;
+    g.dxsp = 1.0 / g.dx; g.dysp = 1.0 / g.dy; g.dzsp = 1.0 / g.dz;
+    if (g.dx != g.dy || g.dx != g.dz) { err = "Spatial discretization must be homogenous."; return RMT3D_ERR_ARG; }
+    hx.resize(g.m4); hy.resize(g.n4); hz.resize(g.o4);
+    for (int i = 0; i < g.m4; i++) hx[i] = b[0] + (i - 2 + 0.5) * g.dx;
+    for (int j = 0; j < g.n4; j++) hy[j] = b[2] + (j - 2 + 0.5) * g.dy;
+    for (int k = 0; k < g.o4; k++) hz[k] = b[4] + (k - 2 + 0.5) * g.dz;
+
+    HCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    double *dlx = (double *)dalloc(sizeof(double) * g.m4);
+    double *dly = (double *)dalloc(sizeof(double) * g.n4);
+    double *dlz = (double *)dalloc(sizeof(double) * g.o4);
+    if (!dlx || !dly || !dlz) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    HCHK(hipMemcpy(dlx, hx.data(), sizeof(double) * g.m4, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(dly, hy.data(), sizeof(double) * g.n4, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(dlz, hz.data(), sizeof(double) * g.o4, hipMemcpyHostToDevice));
+    g.lx = dlx; g.ly = dly; g.lz = dlz;
+
+    // sim_manager: materials, objects, setup_const
+    double dh = std::min(std::min(g.dx, g.dy), g.dz);
+    vmax = 1;
+    h_obj.resize(std::max(1, n_obj));
+    for (int i = 0; i < n_obj; i++) {
+        const rmt3d_object &oo = objs_in[i];
+        if (oo.type != RMT3D_BASIC_SPHERE && oo.type != RMT3D_SPHERE) { err = "object type not supported"; return RMT3D_ERR_ARG; }
+        ObjC &q = h_obj[i];
+        for (int a = 0; a < 3; a++) q.c[a] = oo.basic[a];
+        q.R = oo.basic[3];
+        q.G = oo.shear_mod;
+        q.ex_mu = cfg.ex_visc_mult * cfg.fmu;   // sim_manager.cc:72
+        q.ev_trans_mult = cfg.ev_trans_mult;
+        q.avg_vel[0] = q.avg_vel[1] = q.avg_vel[2] = 0;
+        init.type[i] = oo.type;
+        for (int a = 0; a < 3; a++) { init.eig[i][a] = 1; init.omega[i][a] = 0; }
+        if (oo.type == RMT3D_SPHERE) {
+            int sd = (int)oo.extra[0];
+            double sl = oo.extra[1];
+            for (int a = 0; a < 3; a++) init.eig[i][a] = sl;
+            init.eig[i][sd] = 1. / (sl * sl);
+            for (int a = 0; a < 3; a++) init.omega[i][a] = oo.extra[2 + a];
+        }
+    }
+    {
+        double xxsp3 = 3.0 / (dh * dh), dt_s = 100000;
+        for (int i = 0; i < n_obj; i++) {
+            double rho = objs_in[i].srho, G = objs_in[i].shear_mod, cc = sqrt(G / rho);
+            double exmu = cfg.ex_visc_mult * cfg.fmu;
+            double x3 = 3 / (dh * dh);
+            double dt1 = 0.5 * rho / (exmu * (cfg.ev_trans_mult + 1) * x3) * cfg.dt_ex_pad;
+            double dt0 = dh / cc * cfg.sdt_pad;
+            double odt = (dt0 < dt1) ? dt0 : dt1;
+            if (odt < dt_s) dt_s = odt;
+        }
+        double dt_f = 0.5 * cfg.frho / (cfg.fmu * xxsp3) * cfg.fdt_pad;
+        dt_reg = (dt_s < dt_f) ? dt_s : dt_f;
+    }
+    eps = cfg.wt_n * dh;
+    P.eps = eps;
+    P.tf1 = 0.5 / eps;
+    P.tf2 = 0.5 / M_PI;
+    P.tf3 = M_PI / eps;
+    if (cfg.dt < dt_reg) dt_reg = cfg.dt;
+    K_stiff = 0.0025 / (dt_reg * dt_reg);
+    P.fmu = cfg.fmu;
+    P.sim_type = st;
+    P.n_obj = n_obj;
+    P.gravity = 0;
+    P.min_wall_acc = 0; P.wall_acc_mult = 0; P.wall_dist = 1; P.winv = 1;
+    for (int f = 0; f < 6; f++) { P.walls[f] = 0; P.wall_pos[f] = 0; }
+    if (st == RMT3D_SIM_OBJECTS || st == RMT3D_SIM_MILLS) {
+        for (int f = 0; f < 6; f++) { P.walls[f] = cfg.walls[f]; P.wall_pos[f] = cfg.wall_pos[f]; }
+        P.wall_dist = cfg.wall_dist;
+        P.winv = 1 / cfg.wall_dist;
+        P.gravity = cfg.gravity;
+        P.min_wall_acc = 20 * P.winv;
+        double mvsq = -1, mvol = -1, rho_f = cfg.frho;
+        for (int i = 0; i < n_obj; i++) {
+            double rho_s = objs_in[i].srho, R = objs_in[i].basic[3];
+            double vsq = 0.5 * fabs((rho_s - rho_f) * P.gravity) / rho_f;
+            double volume = M_PI * 4 / 3 * pow(R, 3);
+            double mvf = volume / R / R;
+            vsq *= mvf;
+            if (vsq > mvsq) mvsq = vsq;
+            if (mvf > mvol) mvol = mvf;
+            double sa = fabs((rho_s - rho_f) * P.gravity * P.winv * mvf / rho_s);
+            if (sa > P.min_wall_acc) P.min_wall_acc = sa;
+        }
+        P.wall_acc_mult = 0.5 * mvol * pow(P.winv, 2);
+        vmax = sqrt(mvsq);
+    }
+    // setup_faces (fluid_3d.cc:2951-2974)
+    for (int f = 0; f < 6; f++) {
+        for (int cc = 0; cc < 3; cc++)
+            if (cfg.bct[cc][f] == 1 && cfg.bcv[cc][f] > vmax) vmax = cfg.bcv[cc][f];
+        if (!g.prd[f / 2]) {
+            FaceSpec F;
+            F.f = f;
+            for (int cc = 0; cc < 3; cc++) { F.bct[cc] = cfg.bct[cc][f]; F.bcv[cc] = cfg.bcv[cc][f]; }
+            wall_faces.push_back(F);
+        }
+    }
+    P.K_stiff = K_stiff;
+    init.has_exact = cfg.has_exact;
+    init.vel_prof_num = cfg.vel_prof_num;
+    for (int a = 0; a < 16; a++) init.vel_profile[a] = cfg.vel_profile[a];
+
+    dt = cfg.dt;
+    time = 0;
+    d_obj = (ObjC *)dalloc(sizeof(ObjC) * h_obj.size());
+    HCHK(hipMemcpy(d_obj, h_obj.data(), sizeof(ObjC) * h_obj.size(), hipMemcpyHostToDevice));
+
+    // device state
+    int Ng = g.Ng;
+    for (int a = 0; a < 3; a++) {
+        U.vel[a] = (double *)dalloc(sizeof(double) * Ng);
+        U.dvel[a] = (double *)dalloc(sizeof(double) * Ng);
+    }
+    U.p = (double *)dalloc(sizeof(double) * Ng);
+    U.dp = (double *)dalloc(sizeof(double) * Ng);
+    U.q = (double *)dalloc(sizeof(double) * Ng);
+    for (int a = 0; a < 18; a++) U.fvel[a] = (double *)dalloc(sizeof(double) * Ng);
+    for (int a = 0; a < 9; a++) U.sigma[a] = (double *)dalloc(sizeof(double) * Ng);
+    R.c = (uint32_t *)dalloc(sizeof(uint32_t) * Ng);
+    for (int a = 0; a < 3; a++) {
+        R.x[a] = (double *)dalloc(sizeof(double) * Ng);
+        R.xpred[a] = (double *)dalloc(sizeof(double) * Ng);
+        R.detF[a] = (double *)dalloc(sizeof(double) * Ng);
+        R.en[a] = (double *)dalloc(sizeof(double) * Ng);
+    }
+    for (int a = 0; a < 18; a++) R.fx[a] = (double *)dalloc(sizeof(double) * Ng);
+    for (int a = 0; a < 9; a++) R.gphi[a] = (double *)dalloc(sizeof(double) * Ng);
+    if (!R.gphi[8]) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+
+    // band capacity: twice the padded shells of all objects, at least 64k
+    double cap_est = 65536;
+    for (int i = 0; i < n_obj; i++) {
+        double R0 = objs_in[i].basic[3] + (cfg.nlayers + 4) * dh;
+        cap_est += 2.0 * (4.0 / 3.0) * M_PI * R0 * R0 * R0 / (dh * dh * dh);
+    }
+    int cap = (int)std::min<double>(cap_est, (double)Ng * std::max(1, n_obj));
+    alloc_band(ext, cap);
+    alloc_band(tmp, cap);
+    W.mask = (uint32_t *)dalloc(sizeof(uint32_t) * Ng);
+    W.flag = (unsigned char *)dalloc(Ng);
+    W.lid = (signed char *)dalloc(Ng);
+    for (int a = 0; a < 3; a++) W.val[a] = (double *)dalloc(sizeof(double) * Ng);
+    W.list[0] = (int *)dalloc(sizeof(int) * Ng);
+    W.list[1] = (int *)dalloc(sizeof(int) * Ng);
+    W.cnt = (int *)dalloc(sizeof(int) * 2);
+    W.st_cap = cap;
+    W.st_cell = (int *)dalloc(sizeof(int) * cap);
+    W.st_c = (uint32_t *)dalloc(sizeof(uint32_t) * cap);
+    for (int a = 0; a < 3; a++) {
+        W.st_x[a] = (double *)dalloc(sizeof(double) * cap);
+        W.st_xpred[a] = (double *)dalloc(sizeof(double) * cap);
+    }
+    W.st_n = (int *)dalloc(sizeof(int));
+    W.perm = (int *)dalloc(sizeof(int) * cap);
+    W.fill = (int *)dalloc(sizeof(int) * Ng);
+    W.scan_tmp = (int *)dalloc(sizeof(int) * ((Ng + 1023) / 1024 + 1));
+    W.err = (int *)dalloc(sizeof(int));
+    d_status = (int *)dalloc(sizeof(int));
+    d_u64 = (unsigned long long *)dalloc(sizeof(unsigned long long) * 4);
+    d_partial = (double *)dalloc(sizeof(double) * 1024);
+    d_mean = (double *)dalloc(sizeof(double) * 4);
+    d_faces = dalloc(face_struct_size());
+    d_rhs_tmp = (double *)dalloc(sizeof(double) * (size_t)g.fm * g.fn * g.fo * 2);
+    {
+        double wp[64];
+        for (int d = 0; d < 64; d++) wp[d] = pow(cfg.weight_fac, d);
+        d_wpow = (double *)dalloc(sizeof(wp));
+        HCHK(hipMemcpy(d_wpow, wp, sizeof(wp), hipMemcpyHostToDevice));
+    }
+    if (!W.err || !d_rhs_tmp || !d_wpow) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    HCHK(hipHostMalloc((void **)&h_status, sizeof(int) * 4, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void **)&h_int, sizeof(int) * 16, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void **)&h_dbl, sizeof(double) * 16, hipHostMallocDefault));
+    // initial gphi = -1e200 for all primary maps (ref_map ctor, fields.hh:324-352)
+    {
+        std::vector<double> neg((size_t)Ng, -1.e200);
+        for (int a = 0; a < 9; a++) HCHK(hipMemcpy(R.gphi[a], neg.data(), sizeof(double) * Ng, hipMemcpyHostToDevice));
+    }
+    int dm[3] = {g.m, g.n, g.o}, df[3] = {g.fm, g.fn, g.fo};
+    bool mixed = (prec == RMT3D_PREC_MIXED);
+    mac.setup(0, dm, g.prd, mixed);
+    fem.setup(1, df, g.prd, mixed);
+    if (mixed) {
+        // defect correction stops on the FP64 residual relative to the rhs
+        // scale; the rule itself is mg_prob::run's
+        mac.tol_override = -1;
+        fem.tol_override = -1;
+    }
+    HCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+Maps Sim::maps() const {
+    Maps M;
+    M.r = R;
+    M.b = ext;
+    M.Ng = g.Ng;
+    return M;
+}
+
+int Sim::check_status(const char *where) {
+    HCHK(hipMemcpyAsync(h_status, d_status, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHK(hipStreamSynchronize(stream));
+    int s = *h_status;
+    if (!s) return 0;
+    err = std::string(where) + ": ";
+    if (s & ST_FATAL_NVD) err += "normal_derivatives_refmap: missing necessary cells to carry out finite differences. ";
+    if (s & ST_FATAL_UPWIND) err += "godunov_upwinding_set: missing neighbour map. ";
+    if (s & ST_FATAL_EXTRAP) err += "Extrapolation failed. ";
+    if (s & ST_FATAL_REDRAW) err += "redraw_boundary(): Two deep maps try to be inside a cell at the same time! ";
+    if (s & ST_BAND_OVERFLOW) err += "band capacity exceeded. ";
+    if (s & ST_FATAL_INIT) err += "init_refmap: overlapping primary points! ";
+    return RMT3D_ERR_FATAL;
+}
+
+// ---------------------------------------------------------------------------
+// ghost fills
+// ---------------------------------------------------------------------------
+void Sim::comm_fluid(int flags) {
+    if (!(g.prd[0] || g.prd[1] || g.prd[2])) return;
+    double *d[40];
+    int nd = 0;
+    if (flags & ORC_VEL) for (int a = 0; a < 3; a++) d[nd++] = U.vel[a];
+    if (flags & ORC_PRES) { d[nd++] = U.p; d[nd++] = U.dp; d[nd++] = U.q; }
+    if (flags & ORC_FVEL) for (int a = 0; a < 18; a++) d[nd++] = U.fvel[a];
+    if (flags & ORC_DVEL) for (int a = 0; a < 3; a++) d[nd++] = U.dvel[a];
+    if (flags & ORC_FX) for (int a = 0; a < 18; a++) d[nd++] = R.fx[a];
+    uint32_t *u[2];
+    int nu = 0;
+    launch_periodic(stream, g, d, nd, u, nu, flags == ORC_PRES);
+}
+
+void Sim::comm_rm(int flags) {
+    if (!(g.prd[0] || g.prd[1] || g.prd[2])) return;
+    double *d[40];
+    int nd = 0;
+    if (flags & ORC_X) for (int a = 0; a < 3; a++) d[nd++] = R.x[a];
+    if (flags & ORC_XPRED) for (int a = 0; a < 3; a++) d[nd++] = R.xpred[a];
+    uint32_t *u[2];
+    int nu = 0;
+    if (flags & ORC_C) u[nu++] = R.c;
+    launch_periodic(stream, g, d, nd, u, nu, 0);
+}
+
+// fluid_3d::set_bc_cc / set_bc_cf (fluid_3d.cc:145-165)
+void Sim::fill_bc_cc() {
+    for (auto &f : wall_faces) launch_bc_cc(stream, g, f, U);
+    comm_fluid(ORC_VEL);
+}
+void Sim::fill_bc_cf() {
+    for (auto &f : wall_faces) launch_bc_cf(stream, g, f, U);
+    comm_fluid(ORC_FVEL);
+}
+
+// ---------------------------------------------------------------------------
+// initialisation (fluid_3d.cc:192-400)
+// ---------------------------------------------------------------------------
+void Sim::cfl_recheck() {
+    double dh = std::min(std::min(g.dx, g.dy), g.dz);
+    double dt_min = cfg.fdt_pad * dh / vmax;
+    if (dt_reg > dt_min) {
+        dt_reg = dt_min;
+        if (cfg.dt < dt_reg) dt_reg = cfg.dt;
+    }
+    K_stiff = 0.0025 / (dt_reg * dt_reg);
+    P.K_stiff = K_stiff;
+}
+
+void Sim::init_fluid() {
+    P.time = time;
+    launch_zero_u64(stream, d_u64, 1);
+    launch_init_fluid(stream, g, P, d_obj, init, U, d_u64);
+    comm_fluid(ORC_VEL);
+    unsigned long long bits;
+    hipMemcpyAsync(h_dbl, d_u64, sizeof(bits), hipMemcpyDeviceToHost, stream);
+    hipStreamSynchronize(stream);
+    memcpy(&bits, h_dbl, sizeof bits);
+    double m;
+    memcpy(&m, &bits, sizeof m);
+    if (m > vmax) vmax = m;   // sim_manager::velocity (sim_manager.cc:129-130)
+    cfl_recheck();
+    dt = dt_reg;
+}
+
+void Sim::init_pres() {
+    launch_init_pres(stream, g, P, U);
+    comm_fluid(ORC_PRES);
+}
+
+int Sim::init_refmap() {
+    launch_init_refmap(stream, g, P, d_obj, init, R, d_status);
+    comm_rm(ORC_C);
+    comm_rm(ORC_X | ORC_XPRED);
+    return 0;
+}
+
+void Sim::extrapolate() {
+    W.tband = tmp;
+    band_extrapolate(stream, g, P, d_obj, R, ext, W, cfg.nlayers, cfg.max_extrap_rs, d_wpow, nmask, d_status, n_obj);
+    band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status);
+}
+
+void Sim::init_extrapolate() {
+    extrapolate();
+    launch_band_x_eq_xpred(stream, ext);
+}
+
+void Sim::compute_grad_basis() {
+    double fac = 3 * g.dx / dt;
+    for (int i = 0; i < 8; i++) {
+        gradphi[i][0] = (2 * (i % 2) - 1) * fac;
+        gradphi[i][1] = (2 * ((i / 2) % 2) - 1) * fac;
+        gradphi[i][2] = (2 * (i / 4) - 1) * fac;
+    }
+}
+
+int Sim::initialize() {
+    time = 0;
+    launch_zero_i32(stream, d_status, 1);
+    init_fluid();
+    init_pres();
+    init_refmap();
+    launch_zero_i32(stream, ext.cnt, g.Ng);
+    launch_zero_i32(stream, ext.n, 1);
+    init_extrapolate();
+    fill_bc_cc();
+    compute_grad_basis();
+    int e = check_status("initialize");
+    if (e) return e;
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// step stages
+// ---------------------------------------------------------------------------
+void Sim::mark(int k) {
+    if (timing) hipEventRecord(tev[2 * k], stream);
+}
+void Sim::mark_end(int k) {
+    (void)k;
+}
+
+int Sim::mac_solve() {
+    launch_mac_rhs(stream, g, U, mac.L[0].b, g.dx / (2 * dt));
+    double res = mac.run(stream);
+    if (std::isnan(res)) {
+        err = "fluid_3d::mac_solve(): NaN detected in multigrid solve";
+        return RMT3D_ERR_FATAL;
+    }
+    launch_mac_extract(stream, g, U, mac.L[0].x);
+    comm_fluid(ORC_PRES);
+    launch_sub_qgrad(stream, g, U, 2 * dt * g.dxsp);
+    return 0;
+}
+
+int Sim::pp_solve() {
+    launch_pp_rhs(stream, g, U, gradphi, wall_faces.data(), (int)wall_faces.size(), d_faces, fem.L[0].b);
+    double res = fem.run(stream);
+    if (std::isnan(res)) return 1;
+    int nn = g.fm * g.fn * g.fo;
+    launch_sum_blocks(stream, fem.L[0].x, nn, 1 / (double)nn, d_partial);
+    launch_finish_sum(stream, d_partial, d_mean);
+    launch_pp_extract(stream, g, U, fem.L[0].x, d_mean);
+    comm_fluid(ORC_PRES);
+    launch_sub_pgrad(stream, g, U, dt);
+    return 0;
+}
+
+int Sim::update_reference_map() {
+    launch_xpred(stream, g, P, U, R);
+    comm_rm(ORC_C);
+    comm_rm(ORC_X | ORC_XPRED);
+    std::swap(ext, tmp);
+    extrapolate();
+    launch_zero_i32(stream, W.err, 1);
+    launch_corrector(stream, g, P, d_obj, R, ext, tmp, W.err);
+    hipMemcpyAsync(h_int, W.err, sizeof(int), hipMemcpyDeviceToHost, stream);
+    hipStreamSynchronize(stream);
+    return *h_int;
+}
+
+// fluid_3d::step_forward (fluid_3d.cc:429-577), non-TANG, no tracers
+int Sim::step(int debug) {
+    (void)debug;
+    nt++;
+    P.dt = dt;
+    P.time = time;
+    launch_zero_i32(stream, d_status, 1);
+    Maps M = maps();
+    kprof[2].begin(stream);
+    launch_stress(stream, g, P, d_obj, U, M);
+    kprof[2].end(stream);
+    kprof[3].begin(stream);
+    launch_advect(stream, g, P, d_obj, U, M, d_status);
+    kprof[3].end(stream);
+    time += 0.5 * dt;
+    P.time = time;
+    fill_bc_cf();
+    comm_fluid(ORC_FVEL | ORC_FX);
+    launch_periodic_band_fx(stream, g, ext);
+    launch_upwind(stream, g, U, M, d_status);
+    fill_bc_cf();
+    int e = check_status("step(godunov)");
+    if (e) return e;
+    e = mac_solve();
+    if (e) return e;
+    fill_bc_cf();
+    int prob = update_reference_map();
+    M = maps();
+    if (prob) {
+        launch_revert(stream, g, R, ext, tmp);
+        hipStreamSynchronize(stream);
+        return RMT3D_REFMAP_UPDATE_ERR;
+    }
+    launch_stress(stream, g, P, d_obj, U, M);
+    launch_ustar(stream, g, P, d_obj, U, M);
+    fill_bc_cc();
+    comm_fluid(ORC_DVEL);
+    time += 0.5 * dt;
+    P.time = time;
+    if (pp_solve()) return RMT3D_PROJECTION_ERR;
+    launch_full_update(stream, g, R, ext);
+    launch_redraw(stream, g, P, d_obj, R, ext, W, cfg.nlayers, d_status);
+    band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status);
+    fill_bc_cc();
+    comm_fluid(ORC_DVEL);
+    e = check_status("step");
+    if (e) return e;
+    return RMT3D_NORMAL_EXIT;
+}
+
+// fluid_3d::div_u, L2 part (fluid_3d_utils.cc:507-544)
+double Sim::div_u() {
+    int nn = g.fm * g.fn * g.fo;
+    double *r = d_rhs_tmp, *t = d_rhs_tmp + nn;
+    launch_pp_rhs(stream, g, U, gradphi, wall_faces.data(), (int)wall_faces.size(), d_faces, r);
+    launch_divu_terms(stream, g, r, t);
+    launch_sum_blocks(stream, t, nn, 1.0, d_partial);
+    launch_finish_sum(stream, d_partial, d_mean);
+    hipMemcpyAsync(h_dbl, d_mean, sizeof(double), hipMemcpyDeviceToHost, stream);
+    hipStreamSynchronize(stream);
+    double l2 = h_dbl[0];
+    l2 /= nn;
+    return sqrt(l2);
+}
+
+// fluid_3d::init_iter (fluid_3d.cc:235-287)
+int Sim::init_iter(int init_err) {
+    if (init_err > 0) {
+        double tol = 1e-4;
+        for (int it = 0; it < cfg.num_iters; it++) {
+            time = 0;
+            fill_bc_cc();
+            int e = step(cfg.debug_flag);
+            if (e < 0) return e;
+            double d = div_u();
+            init_fluid();
+            init_refmap();
+            launch_zero_i32(stream, ext.cnt, g.Ng);
+            launch_zero_i32(stream, ext.n, 1);
+            init_extrapolate();
+            if (d < tol) break;
+        }
+        nt = 0;
+        time = 0.0;
+    }
+    P.dt = dt;
+    P.time = time;
+    launch_stress(stream, g, P, d_obj, U, maps());
+    return check_status("init_iter");
+}
+
+// run_sim.cc:98-106 + stepper ctor (stepper.hh:211-225) + stepper::run set_dt
+int Sim::prepare_run(int init_err, double *dt_out, int *total) {
+    double sdt = cfg.dt;
+    if (dt_reg < sdt) sdt = dt_reg;
+    cfg.dt = sdt;
+    double df = cfg.T / (cfg.frames - 1);
+    int steps = (int)(df / sdt);
+    double rem = df - steps * sdt;
+    if (rem > 0) sdt -= (sdt - rem) / (++steps);
+    int tot = (cfg.frames - 1) * steps;
+    dt = sdt;
+    cfg.dt = sdt;
+    int e = init_iter(init_err);
+    if (e) return e;
+    dt = sdt;
+    if (dt_out) *dt_out = sdt;
+    if (total) *total = (cfg.stop_short > 0 && cfg.stop_short < tot) ? cfg.stop_short : tot;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// state transfer
+// ---------------------------------------------------------------------------
+int Sim::upload(const rmt3d_state *st) {
+    size_t n = (size_t)g.Ng;
+    auto up = [&](double *dst, const double *src) {
+        if (src) hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyHostToDevice);
+    };
+    up(U.vel[0], st->u); up(U.vel[1], st->v); up(U.vel[2], st->w);
+    up(U.p, st->p); up(U.dp, st->dp); up(U.q, st->q);
+    if (st->dvel) for (int a = 0; a < 3; a++) up(U.dvel[a], st->dvel + a * n);
+    if (st->fvel) for (int a = 0; a < 18; a++) up(U.fvel[a], st->fvel + a * n);
+    if (st->sigma) for (int a = 0; a < 9; a++) up(U.sigma[a], st->sigma + a * n);
+    if (st->rm_c) hipMemcpy(R.c, st->rm_c, sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+    if (st->rm_x) for (int a = 0; a < 3; a++) up(R.x[a], st->rm_x + a * n);
+    if (st->rm_xpred) for (int a = 0; a < 3; a++) up(R.xpred[a], st->rm_xpred + a * n);
+    HCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+int Sim::band_count() {
+    int nb = 0;
+    hipStreamSynchronize(stream);
+    hipMemcpy(&nb, ext.n, sizeof(int), hipMemcpyDeviceToHost);
+    return nb;
+}
+
+int Sim::download(rmt3d_state *st) {
+    HCHK(hipStreamSynchronize(stream));
+    size_t n = (size_t)g.Ng;
+    auto dn = [&](double *dst, const double *src) {
+        if (dst) hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost);
+    };
+    dn(st->u, U.vel[0]); dn(st->v, U.vel[1]); dn(st->w, U.vel[2]);
+    dn(st->p, U.p); dn(st->dp, U.dp); dn(st->q, U.q);
+    if (st->dvel) for (int a = 0; a < 3; a++) dn(st->dvel + a * n, U.dvel[a]);
+    if (st->fvel) for (int a = 0; a < 18; a++) dn(st->fvel + a * n, U.fvel[a]);
+    if (st->sigma) for (int a = 0; a < 9; a++) dn(st->sigma + a * n, U.sigma[a]);
+    if (st->rm_c) hipMemcpy(st->rm_c, R.c, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    if (st->rm_x) for (int a = 0; a < 3; a++) dn(st->rm_x + a * n, R.x[a]);
+    if (st->rm_xpred) for (int a = 0; a < 3; a++) dn(st->rm_xpred + a * n, R.xpred[a]);
+    int nb = band_count();
+    if (st->band_cell || st->band_c || st->band_x || st->band_xpred) {
+        if (st->nband < nb) {
+            st->nband = nb;
+            err = "band arrays too small";
+            return RMT3D_ERR_ARG;
+        }
+        if (st->band_cell) hipMemcpy(st->band_cell, ext.cell, sizeof(int) * nb, hipMemcpyDeviceToHost);
+        if (st->band_c) hipMemcpy(st->band_c, ext.c, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost);
+        for (int a = 0; a < 3; a++) {
+            if (st->band_x) hipMemcpy(st->band_x + (size_t)a * nb, ext.x[a], sizeof(double) * nb, hipMemcpyDeviceToHost);
+            if (st->band_xpred)
+                hipMemcpy(st->band_xpred + (size_t)a * nb, ext.xpred[a], sizeof(double) * nb, hipMemcpyDeviceToHost);
+        }
+    }
+    st->nband = nb;
+    HCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+int Sim::mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res) {
+    MGSolver &G = which ? fem : mac;
+    size_t nn = (size_t)G.L[0].nn;
+    HCHK(hipMemcpy(G.L[0].b, rhs, sizeof(double) * nn, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(G.L[0].x, x, sizeof(double) * nn, hipMemcpyHostToDevice));
+    double save = G.tol_override;
+    if (tol > 0) G.tol_override = tol;
+    double r = G.run(stream);
+    G.tol_override = save;
+    HCHK(hipMemcpy(x, G.L[0].x, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (ncyc) *ncyc = G.last_cycles;
+    if (res) *res = r;
+    return 0;
+}
+
+}  // namespace rmt

@@ -1,0 +1,144 @@
+// Host side of the MI355X FSI step: the fluid_3d / sim_manager pair of the
+// reference (F3D_Source/fluid_3d.hh:40-469, Sim/sim_manager.hh:101-320)
+// re-expressed around device-resident structure-of-arrays state.  Every stage
+// of step_forward is a kernel launch on the context's stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "../../include/rmt3d.h"
+#include "launch.h"
+
+namespace rmt {
+
+// Event timing of one kernel class on the context stream: every launch of
+// the class is bracketed by two events; durations are summed on collect.
+struct KProf {
+    static const int CAP = 16384;
+    hipEvent_t *ev = nullptr;
+    int n = 0;           // events recorded (pairs)
+    int enabled = 0;
+    void begin(hipStream_t s) {
+        if (enabled && n < CAP) hipEventRecord(ev[2 * n], s);
+    }
+    void end(hipStream_t s) {
+        if (enabled && n < CAP) { hipEventRecord(ev[2 * n + 1], s); n++; }
+    }
+};
+
+// One projection problem: geometric multigrid behind mg_prob::run
+// (Utilities/mg_probs.cc:3-83).
+struct MGSolver {
+    int fem = 0;
+    int prd[3] = {0, 0, 0};
+    int nlev = 0;
+    MGLevel<double> L[16];
+    MGLevel<float> Lf[16];      // FP32 hierarchy (mixed precision)
+    bool mixed = false;
+    double max_norm_tol = 0;
+    double tol = 0, log_tol = 0;
+    int v_up = 2, v_down = 2;
+    int coarse_sweeps = 64;
+    // mg_pred (mg_probs.hh:37-78)
+    int iP = 16, iV = 32;
+    double V_avg = 32, V_var = 1e6, alpha = 0;
+    int last_cycles = 0;
+    double last_res = 0;
+    KProf *prof = nullptr;     // times the finest-level smoother launches
+    unsigned long long *d_max = nullptr;
+    unsigned long long *h_max = nullptr;   // pinned
+    double tol_override = 0;               // > 0 replaces max_norm_tol
+
+    void setup(int fem_, const int d[3], const int prd_[3], bool mixed_);
+    void release();
+    double resid_max(hipStream_t s);
+    void vcycle64(hipStream_t s, int lev);
+    void vcycle32(hipStream_t s, int lev);
+    void cycle(hipStream_t s);              // one V-cycle (or one defect-correction step)
+    double run(hipStream_t s);
+};
+
+
+
+class Sim {
+  public:
+    rmt3d_config cfg;
+    std::vector<rmt3d_object> objs_in;
+    int precision = RMT3D_PREC_FP64;
+    std::string err;
+
+    // geometry and constants
+    Geo g{};
+    Phys P{};
+    int n_obj = 0;
+    double dt = 0, time = 0;
+    int nt = 0;
+    double dt_reg = 0, K_stiff = 0, vmax = 1, eps = 0;
+    double gradphi[8][3];
+    std::vector<FaceSpec> wall_faces;
+    std::vector<double> hx, hy, hz;
+    InitSpec init{};
+    uint32_t nmask = 1;
+
+    // device state
+    hipStream_t stream = nullptr;
+    ObjC *d_obj = nullptr;
+    std::vector<ObjC> h_obj;
+    Fluid U{};
+    RMap R{};
+    Band ext{}, tmp{};
+    Scratch W{};
+    double *d_wpow = nullptr;
+    int *d_status = nullptr;
+    int *h_status = nullptr;           // pinned
+    int *h_int = nullptr;              // pinned scratch
+    double *h_dbl = nullptr;           // pinned scratch
+    unsigned long long *d_u64 = nullptr;
+    double *d_partial = nullptr, *d_mean = nullptr;
+    void *d_faces = nullptr;
+    double *d_rhs_tmp = nullptr;       // FEM rhs for div_u
+    MGSolver mac, fem;
+    std::vector<void *> allocs;
+    double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+    KProf kprof[4];             // 0 FEM smoother (finest), 1 MAC smoother (finest), 2 stress, 3 advect
+    int profile(int enable);
+    int profile_read(int which, double *ms, int *launches);
+    bool timing = false;
+    hipEvent_t tev[8];
+
+    ~Sim();
+    int create(const rmt3d_config *c, const rmt3d_object *o, int prec);
+    int initialize();
+    int init_iter(int init_err);
+    int prepare_run(int init_err, double *dt_out, int *total);
+    int step(int debug);
+    double div_u();
+    int upload(const rmt3d_state *st);
+    int download(rmt3d_state *st);
+    int band_count();
+    int mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res);
+
+  private:
+    void *dalloc(size_t bytes);
+    Maps maps() const;
+    void fill_bc_cc();
+    void fill_bc_cf();
+    void comm_fluid(int flags);
+    void comm_rm(int flags);
+    void init_fluid();
+    void init_pres();
+    int init_refmap();
+    void init_extrapolate();
+    void extrapolate();
+    void cfl_recheck();
+    void compute_grad_basis();
+    int check_status(const char *where);
+    int mac_solve();
+    int pp_solve();
+    int update_reference_map();
+    void alloc_band(Band &B, int cap);
+    void mark(int k);
+    void mark_end(int k);
+};
+
+}  // namespace rmt

@@ -1,0 +1,728 @@
+// Band ("extraps") kernels: layered least-squares extrapolation of the
+// reference map, the cell-sorted band build, the predictor/corrector updates
+// and redraw_boundary.
+//
+// Extrapolation follows fluid_3d::extrapolate (fluid_3d_ex.cc:31-328) with
+// extrap_helper's masks (Extrapolation/extrap_helper.{hh,cc}):
+//  - one object at a time; the object's maps live in a dense per-cell
+//    scratch (lid: -1 none, 0 primary, L layer; val = xpred), which is exactly
+//    what get_refmap(ind, oid) sees during that object's extrapolation;
+//  - the BFS keeps the reference's u32 mask array and nmask counter; the
+//    immediate mask writes of add_to_next_list become a candidate flag that is
+//    applied after the pass (same set: the write only de-duplicates);
+//  - the layer list lives on the device; kernels loop over its device count,
+//    so no host round trip is needed per layer;
+//  - every list cell fits its weighted affine model in FP64 with two passes
+//    over the search cube (means, then centred moments) in the reference's
+//    summation order.
+#include <limits.h>
+#include "dev.h"
+#include "launch.h"
+
+namespace rmt {
+
+__device__ __forceinline__ int IXb(const Geo &g, int i, int j, int k) { return i + g.m4 * (j + g.n4 * k); }
+__device__ __forceinline__ void unpack(const Geo &g, int ind, int &i, int &j, int &k) {
+    k = ind / g.mn4; j = (ind % g.mn4) / g.m4; i = ind % g.m4;
+}
+
+// periodic source of a ghost cell (cell table)
+__device__ __forceinline__ int psrc(const Geo &g, int i, int j, int k) {
+    int c[3] = {i, j, k};
+    const int s[3] = {g.m, g.n, g.o};
+    bool any = false;
+    for (int d = 0; d < 3; d++) {
+        int t = (c[d] < 2) ? 0 : ((c[d] >= s[d] + 2) ? 2 : 1);
+        if (t != 1) {
+            if (!g.prd[d]) return -1;
+            any = true;
+            c[d] += (t == 0) ? s[d] : -s[d];
+        }
+    }
+    if (!any) return -1;
+    return IXb(g, c[0], c[1], c[2]);
+}
+
+// extrap_helper::within_extrapolation_domain on one rank (extrap_helper.hh:295-404)
+__device__ __forceinline__ bool within_extrap(const Geo &g, int ind) {
+    int c[3];
+    unpack(g, ind, c[0], c[1], c[2]);
+    const int s[3] = {g.m, g.n, g.o};
+    int t[3];
+    bool any = false, has = true;
+    for (int d = 0; d < 3; d++) {
+        t[d] = (c[d] < 2) ? 0 : ((c[d] > s[d] + 1) ? 2 : 1);
+        if (t[d] != 1) { any = true; if (!g.prd[d]) has = false; }
+    }
+    if (!any) return true;
+    if (t[0] == 2 && t[1] == 2 && t[2] == 2) return true;
+    return !has;
+}
+__device__ __forceinline__ bool within_inner_ghost(const Geo &g, int ind) {
+    int i, j, k;
+    unpack(g, ind, i, j, k);
+    return i >= 1 && i <= g.m4 - 2 && j >= 1 && j <= g.n4 - 2 && k >= 1 && k <= g.o4 - 2;
+}
+
+// ---- object scratch and masks ----
+__global__ void k_obj_init(Geo g, RMap R, Scratch W, uint32_t oid, uint32_t nmask) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    if (R.c[t] == oid) {
+        W.lid[t] = 0;
+        for (int a = 0; a < 3; a++) W.val[a][t] = R.xpred[a][t];
+        W.mask[t] = nmask;
+    } else {
+        W.lid[t] = -1;
+    }
+    W.flag[t] = 0;
+}
+
+__global__ void k_mask_reset(Geo g, Scratch W) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < g.Ng) W.mask[t] = 0;
+}
+
+__device__ __forceinline__ void push6(const Geo &g, Scratch &W, int ind, uint32_t cur_mask) {
+    const int st[6] = {-1, 1, -g.m4, g.m4, -g.mn4, g.mn4};
+    for (int a = 0; a < 6; a++) {
+        int cand = ind + st[a];
+        if (W.mask[cand] < cur_mask) W.flag[cand] = 1;
+    }
+}
+
+// add_first_layer (extrap_helper.cc:207-219): primary cells in the inner box
+__global__ void k_first_push(Geo g, Scratch W, uint32_t cur_mask) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.m4 - 2) return;
+    int ind = IXb(g, i, j, k);
+    if (W.mask[ind] == cur_mask) push6(g, W, ind, cur_mask);
+}
+
+// next-layer candidates from the list (fluid_3d_ex.cc:277-284)
+__global__ void k_list_push(Geo g, Scratch W, const int *list, const int *cnt, uint32_t cur_mask) {
+    int n = *cnt;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int ind = list[t];
+        if (within_inner_ghost(g, ind)) push6(g, W, ind, cur_mask);
+    }
+}
+
+__global__ void k_apply_flags(Geo g, Scratch W, uint32_t nmask) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    if (W.flag[t]) W.mask[t] = nmask;
+}
+
+// extrap_helper::comm_mask on one rank (periodic copy) followed by
+// scan_ghost_region (extrap_helper.cc:261-327)
+__global__ void k_comm_mask(Geo g, Scratch W) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    int i, j, k;
+    unpack(g, t, i, j, k);
+    int s = psrc(g, i, j, k);
+    if (s >= 0) W.mask[t] = W.mask[s];
+}
+__global__ void k_scan_ghost(Geo g, Scratch W, uint32_t nmask) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    int i, j, k;
+    unpack(g, t, i, j, k);
+    bool ghost = (i < 2 || i > g.m + 1 || j < 2 || j > g.n + 1 || k < 2 || k > g.o + 1);
+    if (ghost && W.mask[t] == nmask) W.flag[t] = 1;
+}
+
+// update_list: the flagged cells become the next list (set semantics)
+__global__ void k_compact(Geo g, Scratch W, int *list, int *cnt) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    if (W.flag[t]) {
+        int p = atomicAdd(cnt, 1);
+        list[p] = t;
+        W.flag[t] = 0;
+    }
+}
+
+// ---- per-cell weighted least squares (fluid_3d_ex.cc:94-271, extrap_helper.cc:34-169) ----
+__device__ void prev_normal(const Geo &g, const RMap &R, const Band &T, int ind, unsigned oid, double gp[3],
+                            double &gnorm) {
+    int ii, jj, kk;
+    unpack(g, ind, ii, jj, kk);
+    gp[0] = gp[1] = gp[2] = 0;
+    gnorm = 0;
+    if (!(ii >= 2 && ii <= g.m4 - 3 && jj >= 2 && jj <= g.n4 - 3 && kk >= 2 && kk <= g.o4 - 3)) return;
+    int cells[4] = {ind, ind + 1, ind + g.m4, ind + g.mn4};
+    int nf = 0;
+    for (int pp = 0; pp < 4; pp++) {
+        int c = cells[pp];
+        // get_refmap_prev (fluid_3d_utils.cc:44-55): primary or temp band
+        const double *row[3];
+        bool found = false;
+        int ref = -1;
+        if (R.c[c] == oid) { found = true; ref = c; }
+        else {
+            int s = T.start[c], e = s + T.cnt[c];
+            for (int a = s; a < e; a++)
+                if ((T.c[a] & OMASK) == oid) { found = true; ref = -2 - a; break; }
+        }
+        if (!found) continue;
+        int r0 = (pp == 0) ? 0 : pp - 1, r1 = (pp == 0) ? 3 : pp;
+        for (int dd = r0; dd < r1; dd++) {
+            double gv[3];
+            for (int ff = 0; ff < 3; ff++)
+                gv[ff] = (ref >= 0) ? R.gphi[3 * dd + ff][ref] : T.gphi[3 * dd + ff][-2 - ref];
+            if (gv[dd] != -1.e200) {
+                bool add = false;
+                for (int ff = 0; ff < 3; ff++) {
+                    if (gv[ff] != 0.0) add = true;
+                    gp[ff] += gv[ff];
+                }
+                if (add) nf++;
+            }
+        }
+        (void)row;
+    }
+    if (nf == 6) {
+        double nrm = 0;
+        for (int a = 0; a < 3; a++) { gp[a] /= nf; nrm += gp[a] * gp[a]; }
+        if (nrm != 0) {
+            nrm = sqrt(nrm);
+            for (int a = 0; a < 3; a++) gp[a] /= nrm;
+            gnorm = nrm;
+        } else {
+            gp[0] = gp[1] = gp[2] = 0;
+        }
+    } else {
+        gp[0] = gp[1] = gp[2] = 0;
+    }
+}
+
+struct LsqArgs {
+    int L;
+    int max_rs;
+    uint32_t oid;
+    const double *wpow;
+};
+
+// weight of neighbour (p,q,r) or a negative value if it is not a data point
+__device__ __forceinline__ double nb_weight(const Geo &g, const Scratch &W, int ind, int ii, int jj, int kk, int p,
+                                            int q, int r, int L, bool cosw, const double gp[3], const double *wpow) {
+    if (ii + p > g.m4 - 1 || ii + p < 0) return -1;
+    if (jj + q > g.n4 - 1 || jj + q < 0) return -1;
+    if (kk + r > g.o4 - 1 || kk + r < 0) return -1;
+    int nb = ind + p + g.m4 * q + g.mn4 * r;
+    int lid = W.lid[nb];
+    if (lid < 0 || lid >= L) return -1;
+    int dist = abs(p) + abs(q) + abs(r);
+    if (cosw) {
+        double ev0 = g.lx[ii] - g.lx[ii + p], ev1 = g.ly[jj] - g.ly[jj + q], ev2 = g.lz[kk] - g.lz[kk + r];
+        double dw = 0., en = 0.;
+        dw += ev0 * gp[0]; en += ev0 * ev0;
+        dw += ev1 * gp[1]; en += ev1 * ev1;
+        dw += ev2 * gp[2]; en += ev2 * ev2;
+        en = sqrt(en);
+        dw /= en;
+        dw *= wpow[dist];
+        if (!(dw > 0)) return -1;
+        return dw;
+    }
+    return wpow[dist];
+}
+
+__global__ void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt, LsqArgs A,
+                             int *status) {
+    int n = *cnt;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int ind = list[t];
+        if (!within_extrap(g, ind)) continue;
+        int ii, jj, kk;
+        unpack(g, ind, ii, jj, kk);
+        double gp[3], gnorm;
+        prev_normal(g, R, T, ind, A.oid, gp, gnorm);
+        bool cosw = (gnorm != 0. && A.L <= 2);
+        bool ok = false;
+        double davg[3], ravg[3], coeff[9];
+        for (int sr = 2; !ok && sr <= A.max_rs; sr++) {
+            // pass 1: weighted sums (extrap_helper.cc:49-62)
+            int npt = 0;
+            double wsum = 0;
+            for (int a = 0; a < 3; a++) { ravg[a] = 0; davg[a] = 0; }
+            for (int r = -sr; r <= sr; r++)
+                for (int q = -sr; q <= sr; q++)
+                    for (int p = -sr; p <= sr; p++) {
+                        double w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
+                        if (w < 0) continue;
+                        int nb = ind + p + g.m4 * q + g.mn4 * r;
+                        ravg[0] += W.val[0][nb] * w; ravg[1] += W.val[1][nb] * w; ravg[2] += W.val[2][nb] * w;
+                        davg[0] += g.lx[ii + p] * w; davg[1] += g.ly[jj + q] * w; davg[2] += g.lz[kk + r] * w;
+                        wsum += w;
+                        npt++;
+                    }
+            if (npt == 0) continue;
+            for (int a = 0; a < 3; a++) ravg[a] /= wsum;
+            for (int a = 0; a < 3; a++) davg[a] /= wsum;
+            // pass 2: centred moments (extrap_helper.cc:68-75, 129-146)
+            double am[6] = {0, 0, 0, 0, 0, 0}, rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int r = -sr; r <= sr; r++)
+                for (int q = -sr; q <= sr; q++)
+                    for (int p = -sr; p <= sr; p++) {
+                        double Q = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
+                        if (Q < 0) continue;
+                        int nb = ind + p + g.m4 * q + g.mn4 * r;
+                        double dp0 = g.lx[ii + p], dp1 = g.ly[jj + q], dp2 = g.lz[kk + r];
+                        am[0] += Q * (dp0 - davg[0]) * (dp0 - davg[0]);
+                        am[1] += Q * (dp1 - davg[1]) * (dp1 - davg[1]);
+                        am[2] += Q * (dp2 - davg[2]) * (dp2 - davg[2]);
+                        am[3] += Q * (dp0 - davg[0]) * (dp1 - davg[1]);
+                        am[4] += Q * (dp0 - davg[0]) * (dp2 - davg[2]);
+                        am[5] += Q * (dp1 - davg[1]) * (dp2 - davg[2]);
+                        double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
+                        double dd[3] = {dp0 - davg[0], dp1 - davg[1], dp2 - davg[2]};
+                        for (int i = 0; i < 3; i++) {
+                            double lr = Q * (rv[i] - ravg[i]);
+                            for (int j = 0; j < 3; j++) rhs[3 * i + j] += dd[j] * lr;
+                        }
+                    }
+            // mat3::inverse (mat3.hh:61-67) and extrap_helper::dot
+            double a11 = am[0], a12 = am[3], a13 = am[4], a21 = am[3], a22 = am[1], a23 = am[5], a31 = am[4],
+                   a32 = am[5], a33 = am[2];
+            double det = a11 * (a22 * a33 - a23 * a32) + a21 * (a13 * a32 - a12 * a33) + a31 * (a12 * a23 - a13 * a22);
+            double id = 1 / det;
+            double Ai[9] = {(a22 * a33 - a23 * a32) * id, (a13 * a32 - a12 * a33) * id, (a12 * a23 - a13 * a22) * id,
+                            (a23 * a31 - a21 * a33) * id, (a11 * a33 - a13 * a31) * id, (a13 * a21 - a11 * a23) * id,
+                            (a21 * a32 - a22 * a31) * id, (a12 * a31 - a11 * a32) * id, (a11 * a22 - a12 * a21) * id};
+            if (fabs(det) < 1e-17) continue;
+            for (int i = 0; i < 3; i++) {
+                int x = 3 * i, y = x + 1, z = x + 2;
+                coeff[x] = Ai[0] * rhs[x] + Ai[1] * rhs[y] + Ai[2] * rhs[z];
+                coeff[y] = Ai[3] * rhs[x] + Ai[4] * rhs[y] + Ai[5] * rhs[z];
+                coeff[z] = Ai[6] * rhs[x] + Ai[7] * rhs[y] + Ai[8] * rhs[z];
+            }
+            ok = true;
+        }
+        if (!ok) { atomicOr(status, ST_FATAL_EXTRAP); continue; }
+        // fluid_3d::transfer (fluid_3d_ex.cc:293-313)
+        double ps[3] = {g.lx[ii] - davg[0], g.ly[jj] - davg[1], g.lz[kk] - davg[2]};
+        double out[3];
+        for (int a = 0; a < 3; a++) {
+            out[a] = ravg[a];
+            for (int b = 0; b < 3; b++) out[a] += coeff[3 * a + b] * ps[b];
+        }
+        if (isnan(out[0]) || isnan(out[1]) || isnan(out[2])) continue;
+        for (int a = 0; a < 3; a++) W.val[a][ind] = out[a];
+        W.lid[ind] = (signed char)A.L;
+    }
+}
+
+// communicate_a_layer on one rank: a periodic ghost takes its source's map of
+// this layer (fluid_3d_comm.cc:203-280)
+__global__ void k_wrap_layer(Geo g, Scratch W, int L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    int i, j, k;
+    unpack(g, t, i, j, k);
+    int s = psrc(g, i, j, k);
+    if (s < 0 || W.lid[s] != L) return;
+    W.lid[t] = (signed char)L;
+    for (int a = 0; a < 3; a++) W.val[a][t] = W.val[a][s];
+}
+
+// the object's layers become staged band entries
+__global__ void k_stage_obj(Geo g, Scratch W, uint32_t oid, int *status) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    int L = W.lid[t];
+    if (L < 1) return;
+    int p = atomicAdd(W.st_n, 1);
+    if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); return; }
+    W.st_cell[p] = t;
+    W.st_c[p] = oid | ((uint32_t)L << LDIG);
+    for (int a = 0; a < 3; a++) { W.st_x[a][p] = W.val[a][t]; W.st_xpred[a][p] = W.val[a][t]; }
+}
+
+// ---- band build: count, scan, place, per-cell order, gather ----
+__global__ void k_count_cells(Geo g, Band B, Scratch W) {
+    int n = *W.st_n;
+    if (n > W.st_cap) n = W.st_cap;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        atomicAdd(&B.cnt[W.st_cell[t]], 1);
+}
+
+// exclusive scan of B.cnt into B.start, total into *B.n (three phases)
+constexpr int SCAN_BS = 1024;
+__global__ void k_scan_blocks(const int *in, int *out, int *bsum, int n) {
+    __shared__ int sh[SCAN_BS];
+    int t = blockIdx.x * SCAN_BS + threadIdx.x;
+    int v = (t < n) ? in[t] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BS; off <<= 1) {
+        int add = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    if (t < n) out[t] = sh[threadIdx.x] - v;
+    if (threadIdx.x == SCAN_BS - 1) bsum[blockIdx.x] = sh[SCAN_BS - 1];
+}
+__global__ void k_scan_sums(int *bsum, int nb, int *total) {
+    if (threadIdx.x != 0) return;
+    int run = 0;
+    for (int b = 0; b < nb; b++) {
+        int v = bsum[b];
+        bsum[b] = run;
+        run += v;
+    }
+    *total = run;
+}
+__global__ void k_scan_add(int *out, const int *bsum, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] += bsum[t / SCAN_BS];
+}
+
+__global__ void k_place(Geo g, Band B, Scratch W) {
+    int n = *W.st_n;
+    if (n > W.st_cap) n = W.st_cap;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int c = W.st_cell[t];
+        int slot = B.start[c] + atomicAdd(&W.fill[c], 1);
+        if (slot < B.cap) W.perm[slot] = t;
+    }
+}
+
+// multimap order inside a cell: object order, a redrawn primary last
+__global__ void k_sort_cells(Geo g, Band B, Scratch W, int nlayers) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    int n = B.cnt[t];
+    if (n < 2) return;
+    int s = B.start[t];
+    if (s + n > B.cap) return;
+    uint32_t moved = (uint32_t)(nlayers + 1);
+    for (int a = 1; a < n; a++) {
+        int v = W.perm[s + a];
+        uint32_t cv = W.st_c[v];
+        uint32_t kv = ((cv >> LDIG) == moved ? 4096u : 0u) + (cv & OMASK);
+        int b = a - 1;
+        while (b >= 0) {
+            uint32_t cb = W.st_c[W.perm[s + b]];
+            uint32_t kb = ((cb >> LDIG) == moved ? 4096u : 0u) + (cb & OMASK);
+            if (kb <= kv) break;
+            W.perm[s + b + 1] = W.perm[s + b];
+            b--;
+        }
+        W.perm[s + b + 1] = v;
+    }
+}
+
+__global__ void k_gather_entries(Band B, Scratch W) {
+    int n = *B.n;
+    if (n > B.cap) n = B.cap;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        int v = W.perm[e];
+        B.cell[e] = W.st_cell[v];
+        B.c[e] = W.st_c[v];
+        for (int a = 0; a < 3; a++) { B.x[a][e] = W.st_x[a][v]; B.xpred[a][e] = W.st_xpred[a][v]; }
+        for (int a = 0; a < 9; a++) B.gphi[a][e] = -1.e200;
+        for (int a = 0; a < 3; a++) { B.detF[a][e] = -1.e200; B.en[a][e] = -1.e200; }
+        for (int a = 0; a < 18; a++) B.fx[a][e] = 0;
+    }
+}
+
+__global__ void k_zero_i32(int *p, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+__global__ void k_check_cap(Band B, int *status) {
+    if (*B.n > B.cap) atomicOr(status, ST_BAND_OVERFLOW);
+}
+
+void launch_zero_i32(hipStream_t s, int *p, int n) {
+    hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+void launch_zero_u64(hipStream_t s, unsigned long long *p, int n) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+static inline dim3 g1(int n) { return dim3((n + 255) / 256); }
+static const int GRID_STRIDE_BLOCKS = 2048;
+
+void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status) {
+    launch_zero_i32(s, B.cnt, g.Ng);
+    launch_zero_i32(s, W.fill, g.Ng);
+    hipLaunchKernelGGL(k_count_cells, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
+    int nb = (g.Ng + SCAN_BS - 1) / SCAN_BS;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BS), 0, s, B.cnt, B.start, W.scan_tmp, g.Ng);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, W.scan_tmp, nb, B.n);
+    hipLaunchKernelGGL(k_scan_add, g1(g.Ng), dim3(256), 0, s, B.start, W.scan_tmp, g.Ng);
+    hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(1), 0, s, B, status);
+    hipLaunchKernelGGL(k_place, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
+    hipLaunchKernelGGL(k_sort_cells, g1(g.Ng), dim3(256), 0, s, g, B, W, nlayers);
+    hipLaunchKernelGGL(k_gather_entries, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B, W);
+}
+
+// fluid_3d::extrapolate (fluid_3d_ex.cc:31-47); the caller has swapped the
+// previous band into the temp slot.  nmask is the host copy of the mask
+// counter (extrap_helper::nmask), advanced exactly as the reference does.
+void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
+                      int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj) {
+    (void)P; (void)O;
+    launch_zero_i32(s, W.st_n, 1);
+    bool per = g.prd[0] || g.prd[1] || g.prd[2];
+    // temp band needed by prev_normal is passed in B's place by the caller
+    // through W; see fluid3d.hip
+    (void)B;
+    for (int oid = 1; oid <= n_obj; oid++) {
+        if (nmask > UINT_MAX - 50) {
+            nmask = 1;
+            hipLaunchKernelGGL(k_mask_reset, g1(g.Ng), dim3(256), 0, s, g, W);
+        }
+        uint32_t cur_mask = nmask;
+        hipLaunchKernelGGL(k_obj_init, g1(g.Ng), dim3(256), 0, s, g, R, W, (uint32_t)oid, nmask);
+        nmask++;
+        hipLaunchKernelGGL(k_first_push, dim3((g.m4 - 2 + 63) / 64, g.n4 - 2, g.o4 - 2), dim3(64), 0, s, g, W,
+                           cur_mask);
+        hipLaunchKernelGGL(k_apply_flags, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
+        if (per) hipLaunchKernelGGL(k_comm_mask, g1(g.Ng), dim3(256), 0, s, g, W);
+        hipLaunchKernelGGL(k_scan_ghost, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
+        int cur = 0;
+        launch_zero_i32(s, W.cnt + cur, 1);
+        hipLaunchKernelGGL(k_compact, g1(g.Ng), dim3(256), 0, s, g, W, W.list[cur], W.cnt + cur);
+        for (int L = 1; L <= nlayers; L++) {
+            nmask++;
+            LsqArgs A;
+            A.L = L; A.max_rs = max_rs; A.oid = (uint32_t)oid; A.wpow = wpow_dev;
+            hipLaunchKernelGGL(k_gather_lsq, dim3(GRID_STRIDE_BLOCKS), dim3(64), 0, s, g, R, W.tband, W,
+                               W.list[cur], W.cnt + cur, A, status);
+            int nxt = cur ^ 1;
+            launch_zero_i32(s, W.cnt + nxt, 1);
+            if (L < nlayers) {
+                hipLaunchKernelGGL(k_list_push, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, W, W.list[cur],
+                                   W.cnt + cur, cur_mask);
+                hipLaunchKernelGGL(k_apply_flags, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
+                if (per) hipLaunchKernelGGL(k_comm_mask, g1(g.Ng), dim3(256), 0, s, g, W);
+                hipLaunchKernelGGL(k_scan_ghost, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
+                hipLaunchKernelGGL(k_compact, g1(g.Ng), dim3(256), 0, s, g, W, W.list[nxt], W.cnt + nxt);
+            }
+            cur = nxt;
+            if (per) hipLaunchKernelGGL(k_wrap_layer, g1(g.Ng), dim3(256), 0, s, g, W, L);
+        }
+        hipLaunchKernelGGL(k_stage_obj, g1(g.Ng), dim3(256), 0, s, g, W, (uint32_t)oid, status);
+    }
+}
+
+// ---- predictor, corrector, full update ----
+// update_reference_map predictor (fluid_3d.cc:1153-1192)
+__global__ void k_xpred(Geo g, Phys P, Fluid U, RMap R) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= g.m) return;
+    int c = IXb(g, i + 2, j + 2, k + 2);
+    if ((R.c[c] & OMASK) == 0) return;
+    double cdt = P.dt;
+    double av[3];
+    av[0] = -0.5 * cdt * (U.fvel[0][c] + U.fvel[3][c]) * g.dxsp;
+    av[1] = -0.5 * cdt * (U.fvel[7][c] + U.fvel[10][c]) * g.dysp;
+    av[2] = -0.5 * cdt * (U.fvel[14][c] + U.fvel[17][c]) * g.dzsp;
+    for (int d = 0; d < 3; d++) {
+        double dxt = av[0] * (R.fx[3 + d][c] - R.fx[d][c]) + av[1] * (R.fx[9 + d][c] - R.fx[6 + d][c]) +
+                     av[2] * (R.fx[15 + d][c] - R.fx[12 + d][c]);
+        R.xpred[d][c] = R.x[d][c] + dxt;
+    }
+}
+void launch_xpred(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U, RMap R) {
+    hipLaunchKernelGGL(k_xpred, dim3((g.m + 63) / 64, g.n, g.o), dim3(64), 0, s, g, P, U, R);
+}
+
+// corrector (fluid_3d.cc:1205-1248) over every padded cell
+__global__ void k_corrector(Geo g, Phys P, const ObjC *O, RMap R, Band B, Band T, int *err) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    if (R.c[t] & OMASK)
+        for (int a = 0; a < 3; a++) R.x[a][t] = 0.5 * (R.x[a][t] + R.xpred[a][t]);
+    int s = B.start[t], n = B.cnt[t];
+    for (int e = s; e < s + n; e++) {
+        uint32_t oid = B.c[e] & OMASK;
+        int old = -1;
+        int ts = T.start[t], tn = T.cnt[t];
+        for (int b = ts; b < ts + tn; b++)
+            if ((T.c[b] & OMASK) == oid) { old = b; break; }
+        if (old >= 0) {
+            for (int q = 0; q < 3; q++) B.x[q][e] = 0.5 * (T.x[q][old] + B.xpred[q][e]);
+        } else {
+            double dist = obj_phi(O, P.n_obj, (int)oid - 1, B.xpred[0][e], B.xpred[1][e], B.xpred[2][e]);
+            if (dist < P.eps && (B.c[e] >> LDIG) <= 3) atomicAdd(err, 1);
+            else
+                for (int q = 0; q < 3; q++) B.x[q][e] = B.xpred[q][e];
+        }
+    }
+}
+void launch_corrector(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Band T, int *err) {
+    hipLaunchKernelGGL(k_corrector, g1(g.Ng), dim3(256), 0, s, g, P, O, R, B, T, err);
+}
+
+// revert_reference_map_update (fluid_3d.cc:1257-1282)
+__global__ void k_revert(Geo g, RMap R, Band B, Band T) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    if (R.c[t] & OMASK)
+        for (int a = 0; a < 3; a++) R.x[a][t] = 2 * R.x[a][t] - R.xpred[a][t];
+    int s = B.start[t], n = B.cnt[t];
+    for (int e = s; e < s + n; e++) {
+        uint32_t oid = B.c[e] & OMASK;
+        int ts = T.start[t], tn = T.cnt[t];
+        for (int b = ts; b < ts + tn; b++)
+            if ((T.c[b] & OMASK) == oid) {
+                for (int q = 0; q < 3; q++) B.x[q][e] = T.x[q][b];
+                break;
+            }
+    }
+}
+void launch_revert(hipStream_t s, const Geo &g, RMap R, Band B, Band T) {
+    hipLaunchKernelGGL(k_revert, g1(g.Ng), dim3(256), 0, s, g, R, B, T);
+}
+
+// update_reference_map_full (fluid_3d.cc:1285-1306)
+__global__ void k_full_prim(Geo g, RMap R) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    if ((R.c[t] & OMASK) > 0)
+        for (int a = 0; a < 3; a++) R.x[a][t] = R.xpred[a][t];
+}
+__global__ void k_band_x_eq_xpred(Band B) {
+    int n = *B.n;
+    if (n > B.cap) n = B.cap;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        for (int a = 0; a < 3; a++) B.x[a][e] = B.xpred[a][e];
+}
+void launch_full_update(hipStream_t s, const Geo &g, RMap R, Band B) {
+    hipLaunchKernelGGL(k_full_prim, g1(g.Ng), dim3(256), 0, s, g, R);
+    hipLaunchKernelGGL(k_band_x_eq_xpred, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B);
+}
+void launch_band_x_eq_xpred(hipStream_t s, Band B) {
+    hipLaunchKernelGGL(k_band_x_eq_xpred, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B);
+}
+
+// ---- redraw_boundary (fluid_3d_ex.cc:336-394) ----
+constexpr int RD_MAX = 32;
+__global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    uint32_t lc[RD_MAX];
+    double lx[RD_MAX][3], lp[RD_MAX][3];
+    int nl = 0;
+    int s = B.start[t], n = B.cnt[t];
+    for (int e = s; e < s + n && nl < RD_MAX - 1; e++, nl++) {
+        lc[nl] = B.c[e];
+        for (int a = 0; a < 3; a++) { lx[nl][a] = B.x[a][e]; lp[nl][a] = B.xpred[a][e]; }
+    }
+    uint32_t lflag = (uint32_t)(nlayers + 1) << LDIG;
+    uint32_t pc = R.c[t];
+    double px[3] = {R.x[0][t], R.x[1][t], R.x[2][t]};
+    double pp[3] = {R.xpred[0][t], R.xpred[1][t], R.xpred[2][t]};
+    bool occupied = false, changed = false, cleared = false;
+    int oidp = (int)(pc & OMASK);
+    if (oidp > 0) {
+        double pphi = obj_phi(O, P.n_obj, oidp - 1, px[0], px[1], px[2]);
+        if (pphi > 0) {
+            lc[nl] = lflag | pc;
+            for (int a = 0; a < 3; a++) { lx[nl][a] = px[a]; lp[nl][a] = pp[a]; }
+            nl++;
+            pc = 0;
+            for (int a = 0; a < 3; a++) { px[a] = 0; pp[a] = 0; }
+            changed = true;
+            cleared = true;
+        } else {
+            occupied = true;
+        }
+    }
+    int N = nl;
+    for (int l = 0; l < N; l++) {
+        if (l >= nl) continue;
+        uint32_t oid = lc[l] & OMASK;
+        double sphi = obj_phi(O, P.n_obj, (int)oid - 1, lx[l][0], lx[l][1], lx[l][2]);
+        bool inside = (sphi < 0 && (lc[l] >> LDIG) == 1);
+        if (!inside) continue;
+        int po = (int)(pc & OMASK);
+        double pphi = po > 0 ? obj_phi(O, P.n_obj, po - 1, px[0], px[1], px[2]) : __builtin_nan("");
+        if ((occupied && sphi < pphi) || !occupied) {
+            if (pphi < -g.dx) { atomicOr(status, ST_FATAL_REDRAW); return; }
+            pc = oid;
+            for (int a = 0; a < 3; a++) { px[a] = lx[l][a]; pp[a] = lp[l][a]; }
+            occupied = true;
+            changed = true;
+            cleared = false;
+            int j0 = -1;
+            for (int b = 0; b < nl; b++)
+                if ((lc[b] & OMASK) == oid && j0 == -1) j0 = b;
+            if (j0 >= 0) {
+                for (int b = j0; b < nl - 1; b++) {
+                    lc[b] = lc[b + 1];
+                    for (int a = 0; a < 3; a++) { lx[b][a] = lx[b + 1][a]; lp[b][a] = lp[b + 1][a]; }
+                }
+                nl--;
+            }
+        }
+    }
+    if (changed) {
+        R.c[t] = pc;
+        for (int a = 0; a < 3; a++) { R.x[a][t] = px[a]; R.xpred[a][t] = pp[a]; }
+        if (cleared) {
+            for (int a = 0; a < 18; a++) R.fx[a][t] = 0;
+            for (int a = 0; a < 9; a++) R.gphi[a][t] = 0;
+            for (int a = 0; a < 3; a++) { R.detF[a][t] = 0; R.en[a][t] = 0; }
+        }
+    }
+    if (nl == 0) return;
+    int base = atomicAdd(W.st_n, nl);
+    for (int e = 0; e < nl; e++) {
+        int p = base + e;
+        if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); return; }
+        W.st_cell[p] = t;
+        W.st_c[p] = lc[e];
+        for (int a = 0; a < 3; a++) { W.st_x[a][p] = lx[e][a]; W.st_xpred[a][p] = lp[e][a]; }
+    }
+}
+void launch_redraw(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W, int nlayers,
+                   int *status) {
+    launch_zero_i32(s, W.st_n, 1);
+    hipLaunchKernelGGL(k_redraw, g1(g.Ng), dim3(256), 0, s, g, P, O, R, B, W, nlayers, status);
+}
+
+// ---- reductions ----
+// fixed blocked order (256 chunks, 256 strided partial sums, binary tree,
+// chunk sums added in order): deterministic and reproduced by the oracle
+__global__ void k_sum_blocks(const double *a, int n, double scale, double *partial) {
+    __shared__ double sh[256];
+    double v = 0;
+    int per = (n + gridDim.x - 1) / gridDim.x;
+    int s = blockIdx.x * per, e = s + per < n ? s + per : n;
+    for (int t = s + threadIdx.x; t < e; t += blockDim.x) v += a[t] * scale;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+__global__ void k_finish_sum(const double *partial, int nb, double *out) {
+    if (threadIdx.x != 0) return;
+    double v = 0;
+    for (int b = 0; b < nb; b++) v += partial[b];
+    *out = v;
+}
+void launch_sum_blocks(hipStream_t s, const double *a, int n, double scale, double *partial) {
+    hipLaunchKernelGGL(k_sum_blocks, dim3(256), dim3(256), 0, s, a, n, scale, partial);
+}
+void launch_finish_sum(hipStream_t s, const double *partial, double *out) {
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, partial, 256, out);
+}
+
+}  // namespace rmt

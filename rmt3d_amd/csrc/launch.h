@@ -1,0 +1,113 @@
+// Host-side declarations of the kernel launchers (one translation unit per
+// stage group: k_fluid.hip, k_mg.hip, k_band.hip).
+#pragma once
+#include "dev.h"
+
+namespace rmt {
+
+struct FaceSpec {
+    int f;
+    int bct[3];
+    double bcv[3];
+};
+
+struct InitSpec {
+    int has_exact;
+    int vel_prof_num;
+    double vel_profile[16];
+    int type[64];
+    double eig[64][3];
+    double omega[64][3];
+};
+
+// ---- k_fluid.hip ----
+void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M);
+void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                   int *status);
+void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status);
+void launch_bc_cc(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U);
+void launch_bc_cf(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U);
+void launch_periodic(hipStream_t s, const Geo &g, double *const *d, int nd, uint32_t *const *u, int nu, int corner);
+void launch_periodic_band_fx(hipStream_t s, const Geo &g, const Band &B);
+void launch_mac_rhs(hipStream_t s, const Geo &g, const Fluid &U, double *b, double fac);
+void launch_mac_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x);
+void launch_sub_qgrad(hipStream_t s, const Geo &g, const Fluid &U, double f2);
+void launch_ustar(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M);
+void launch_pp_rhs(hipStream_t s, const Geo &g, const Fluid &U, const double gradphi[8][3], const FaceSpec *faces,
+                   int nwall, void *faces_dev, double *r);
+size_t face_struct_size();
+void launch_pp_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x, const double *mean);
+void launch_sub_pgrad(hipStream_t s, const Geo &g, const Fluid &U, double dt);
+void launch_divu_terms(hipStream_t s, const Geo &g, const double *r, double *out);
+void launch_init_fluid(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const InitSpec &is, const Fluid &U,
+                       unsigned long long *vmax_bits);
+void launch_init_pres(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U);
+void launch_init_refmap(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const InitSpec &is, const RMap &R,
+                        int *status);
+
+// ---- k_mg.hip ----
+template <typename T>
+struct MGLevel {
+    int d[3];
+    int nn;
+    T *x, *b, *r;
+};
+
+// smooth one colour (MAC: 2 colours red-black, FEM: 8 colours 2x2x2)
+template <typename T>
+void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color);
+template <typename T>
+void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]);
+template <typename T>
+void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
+template <typename T>
+void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
+template <typename T>
+void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int sweeps);
+// FP64 residual r64 = b - A x64 with max|r| accumulated into *maxbits
+void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
+                           unsigned long long *maxbits);
+// mixed precision helpers
+void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n);
+void launch_axpy_f2d(hipStream_t s, double *x, const float *e, int n);
+void launch_zero_f(hipStream_t s, float *a, int n);
+
+// ---- reductions / scans (k_band.hip) ----
+void launch_sum_blocks(hipStream_t s, const double *a, int n, double scale, double *partial);
+void launch_finish_sum(hipStream_t s, const double *partial, double *out);
+void launch_zero_u64(hipStream_t s, unsigned long long *p, int n);
+void launch_zero_i32(hipStream_t s, int *p, int n);
+
+// ---- band / extrapolation (k_band.hip) ----
+struct Scratch {
+    uint32_t *mask;
+    unsigned char *flag;
+    signed char *lid;
+    double *val[3];
+    int *list[2];
+    int *cnt;            // 2 counters (list sizes)
+    // staging
+    int *st_cell;
+    uint32_t *st_c;
+    double *st_x[3], *st_xpred[3];
+    int *st_n;
+    int st_cap;
+    int *perm;
+    int *fill;
+    int *scan_tmp;       // block sums
+    int *err;            // corrector problem count
+    Band tband;          // previous band (get_refmap_prev)
+};
+
+void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
+                      int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj);
+void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status);
+void launch_corrector(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Band T, int *err);
+void launch_revert(hipStream_t s, const Geo &g, RMap R, Band B, Band T);
+void launch_full_update(hipStream_t s, const Geo &g, RMap R, Band B);
+void launch_redraw(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W, int nlayers,
+                   int *status);
+void launch_xpred(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U, RMap R);
+void launch_band_x_eq_xpred(hipStream_t s, Band B);
+
+}  // namespace rmt

@@ -1,0 +1,221 @@
+"""ctypes binding of librmt3d.so (the C ABI of include/rmt3d.h).
+
+There is no fallback: if the HIP library is missing or fails to load, every
+entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librmt3d.so")
+
+_lib = None
+
+EXPORTS = [
+    "rmt3d_parse_cfg", "rmt3d_default_config", "rmt3d_create", "rmt3d_destroy", "rmt3d_last_error",
+    "rmt3d_initialize", "rmt3d_prepare_run", "rmt3d_init_iter", "rmt3d_step", "rmt3d_set_dt",
+    "rmt3d_get_dt", "rmt3d_get_time", "rmt3d_set_time", "rmt3d_upload_state", "rmt3d_download_state",
+    "rmt3d_band_count", "rmt3d_mg_solve", "rmt3d_mg_dims", "rmt3d_diag", "rmt3d_stage_times",
+    "rmt3d_stream", "rmt3d_profile", "rmt3d_profile_read", "rmt3d_device_synchronize",
+]
+
+
+class Rmt3dError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise Rmt3dError("librmt3d.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P, D = C.c_void_p, C.c_double
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.rmt3d_parse_cfg.argtypes = [C.c_char_p, C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_int]
+        L.rmt3d_default_config.argtypes = [C.POINTER(_abi.Config)]
+        L.rmt3d_default_config.restype = None
+        L.rmt3d_create.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_int, C.POINTER(P)]
+        L.rmt3d_destroy.argtypes = [P]
+        L.rmt3d_destroy.restype = None
+        L.rmt3d_last_error.argtypes = [P]
+        L.rmt3d_last_error.restype = C.c_char_p
+        for f in ("rmt3d_initialize", "rmt3d_band_count"):
+            getattr(L, f).argtypes = [P]
+        L.rmt3d_prepare_run.argtypes = [P, C.c_int, dp, ip]
+        L.rmt3d_init_iter.argtypes = [P, C.c_int]
+        L.rmt3d_step.argtypes = [P, C.c_int]
+        L.rmt3d_set_dt.argtypes = [P, D]
+        L.rmt3d_set_time.argtypes = [P, D]
+        L.rmt3d_get_dt.argtypes = [P]
+        L.rmt3d_get_dt.restype = D
+        L.rmt3d_get_time.argtypes = [P]
+        L.rmt3d_get_time.restype = D
+        L.rmt3d_upload_state.argtypes = [P, C.POINTER(_abi.State)]
+        L.rmt3d_download_state.argtypes = [P, C.POINTER(_abi.State)]
+        L.rmt3d_mg_solve.argtypes = [P, C.c_int, dp, dp, D, ip, dp]
+        L.rmt3d_mg_dims.argtypes = [P, C.c_int, ip]
+        L.rmt3d_diag.argtypes = [P, C.c_int, dp]
+        L.rmt3d_stage_times.argtypes = [P, dp, C.c_int]
+        L.rmt3d_stream.argtypes = [P]
+        L.rmt3d_stream.restype = P
+        L.rmt3d_profile.argtypes = [P, C.c_int]
+        L.rmt3d_profile_read.argtypes = [P, C.c_int, dp, ip]
+        L.rmt3d_device_synchronize.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def parse_cfg(path, max_obj=256):
+    cfg = _abi.Config()
+    objs = (_abi.Object * max_obj)()
+    rc = lib().rmt3d_parse_cfg(path.encode(), C.byref(cfg), objs, max_obj)
+    if rc:
+        raise Rmt3dError("rmt3d_parse_cfg(%s) failed with %d" % (path, rc))
+    return cfg, [objs[i] for i in range(cfg.n_obj)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Simulation:
+    """One device-resident simulation (fluid_3d on the GPU)."""
+
+    def __init__(self, cfg, objs, precision=_abi.PREC_FP64):
+        n = max(1, len(objs))
+        self._objs = (_abi.Object * n)(*objs) if objs else (_abi.Object * 1)()
+        self.cfg = cfg
+        h = C.c_void_p()
+        rc = lib().rmt3d_create(C.byref(cfg), self._objs, precision, C.byref(h))
+        if rc:
+            raise Rmt3dError("rmt3d_create failed (%d): %s" % (rc, lib().rmt3d_last_error(None).decode()))
+        self.h = h
+        m, nn, o = cfg.sys_size
+        self.Ng = (m + 4) * (nn + 4) * (o + 4)
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise Rmt3dError("%s failed (%d): %s" % (what, rc, lib().rmt3d_last_error(self.h).decode()))
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rmt3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self):
+        return self._check(lib().rmt3d_initialize(self.h), "initialize")
+
+    def init_iter(self, e):
+        return self._check(lib().rmt3d_init_iter(self.h, e), "init_iter")
+
+    def prepare_run(self, init_err):
+        dt = C.c_double()
+        tot = C.c_int()
+        self._check(lib().rmt3d_prepare_run(self.h, init_err, C.byref(dt), C.byref(tot)), "prepare_run")
+        return 0, dt.value, tot.value
+
+    def step(self, debug=0):
+        return self._check(lib().rmt3d_step(self.h, debug), "step")
+
+    @property
+    def dt(self):
+        return lib().rmt3d_get_dt(self.h)
+
+    @property
+    def time(self):
+        return lib().rmt3d_get_time(self.h)
+
+    def set_dt(self, dt):
+        lib().rmt3d_set_dt(self.h, dt)
+
+    def set_time(self, t):
+        lib().rmt3d_set_time(self.h, t)
+
+    def diag(self, which):
+        out = C.c_double()
+        self._check(lib().rmt3d_diag(self.h, which, C.byref(out)), "diag")
+        return out.value
+
+    def div_u(self):
+        return self.diag(0)
+
+    def band_count(self):
+        return self._check(lib().rmt3d_band_count(self.h), "band_count")
+
+    def download(self, band=True):
+        Ng = self.Ng
+        f = {k: np.zeros(Ng) for k in ("u", "v", "w", "p", "dp", "q")}
+        f["dvel"] = np.zeros(3 * Ng)
+        f["fvel"] = np.zeros(18 * Ng)
+        f["sigma"] = np.zeros(9 * Ng)
+        f["rm_c"] = np.zeros(Ng, dtype=np.uint32)
+        f["rm_x"] = np.zeros(3 * Ng)
+        f["rm_xpred"] = np.zeros(3 * Ng)
+        st = _abi.State()
+        for k in ("u", "v", "w", "p", "dp", "q", "dvel", "fvel", "sigma", "rm_x", "rm_xpred"):
+            setattr(st, k, _dp(f[k]))
+        st.rm_c = f["rm_c"].ctypes.data_as(C.POINTER(C.c_uint32))
+        if band:
+            nb = self.band_count()
+            f["band_cell"] = np.zeros(nb, dtype=np.int32)
+            f["band_c"] = np.zeros(nb, dtype=np.uint32)
+            f["band_x"] = np.zeros(3 * nb)
+            f["band_xpred"] = np.zeros(3 * nb)
+            st.nband = nb
+            st.band_cell = f["band_cell"].ctypes.data_as(C.POINTER(C.c_int32))
+            st.band_c = f["band_c"].ctypes.data_as(C.POINTER(C.c_uint32))
+            st.band_x = _dp(f["band_x"])
+            st.band_xpred = _dp(f["band_xpred"])
+        self._check(lib().rmt3d_download_state(self.h, C.byref(st)), "download")
+        return f
+
+    def upload(self, **fields):
+        st = _abi.State()
+        keep = []
+        for k, a in fields.items():
+            if k == "rm_c":
+                a = np.ascontiguousarray(a, dtype=np.uint32)
+                st.rm_c = a.ctypes.data_as(C.POINTER(C.c_uint32))
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                setattr(st, k, _dp(a))
+            keep.append(a)
+        self._check(lib().rmt3d_upload_state(self.h, C.byref(st)), "upload")
+
+    def mg_dims(self, which):
+        d = (C.c_int * 3)()
+        self._check(lib().rmt3d_mg_dims(self.h, which, d), "mg_dims")
+        return tuple(d)
+
+    def mg_solve(self, which, rhs, x0, tol=0.0):
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        r = np.ascontiguousarray(rhs, dtype=np.float64)
+        nc = C.c_int()
+        res = C.c_double()
+        self._check(lib().rmt3d_mg_solve(self.h, which, _dp(r), _dp(x), tol, C.byref(nc), C.byref(res)),
+                    "mg_solve")
+        return x, nc.value, res.value
+
+    def profile(self, enable=True):
+        self._check(lib().rmt3d_profile(self.h, 1 if enable else 0), "profile")
+
+    def profile_read(self, which):
+        ms = C.c_double()
+        n = C.c_int()
+        self._check(lib().rmt3d_profile_read(self.h, which, C.byref(ms), C.byref(n)), "profile_read")
+        return ms.value, n.value
+
+    def synchronize(self):
+        self._check(lib().rmt3d_device_synchronize(self.h), "synchronize")
