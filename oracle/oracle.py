@@ -91,6 +91,8 @@ class OracleSim:
         arr = (_abi.Object * n)(*objs) if objs else (_abi.Object * 1)()
         self._objs = arr
         self.h = lib().oracle_create(C.byref(cfg), arr)
+        if not self.h:
+            raise ValueError("oracle_create rejected the configuration")
         m, nn, o = cfg.sys_size
         self.dims = (m, nn, o)
         self.Ng = (m + 4) * (nn + 4) * (o + 4)

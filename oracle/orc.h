@@ -94,6 +94,7 @@ typedef struct {
     double max_norm_tol;
     double tol, log_tol;
     int v_up, v_down;
+    int coarse_sweeps;
     /* mg_pred (mg_probs.hh:37-78) */
     int iP, iV;
     double V_avg, V_var, alpha;

@@ -157,3 +157,5 @@ int oracle_lsq(int npt, const double *pos, const double *val, const double *w, d
 {
     return orc_least_square_fit(npt, pos, val, w, wsum, davg, ravg, coeff, det);
 }
+
+void oracle_mg_set_coarse_sweeps(orc_sim *S, int n) { S->mac.coarse_sweeps = n; S->fem.coarse_sweeps = n; }

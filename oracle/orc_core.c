@@ -224,6 +224,9 @@ static void solid_acc(const orc_sim *S, int oi, const double rval[3], double x, 
 /* ------------------------------------------------------------------ */
 orc_sim *orc_create(const rmt3d_config *cfg, const rmt3d_object *objs)
 {
+    /* the multigrid colourings need an even cell count across a wrap */
+    for (int d = 0; d < 3; d++)
+        if (cfg->prd[d] && (cfg->sys_size[d] % 2)) return NULL;
     orc_sim *S = (orc_sim *)calloc(1, sizeof(orc_sim));
     S->cfg = *cfg;
     int st = cfg->sim_type;

@@ -21,11 +21,12 @@
  *
  * V-cycle (this project's design; PGMG's is absent, so its iterates are
  * parity unpinned): MAC -- red-black Gauss-Seidel, restriction = sum of the
- * 8 children / 2, cell-centred trilinear prolongation; FEM -- 8-colour
- * (2x2x2 parity) Gauss-Seidel, full-weighting restriction (= P^T) with the
+ * 8 children / 2, cell-centred trilinear prolongation; FEM -- four-colour
+ * Gauss-Seidel by (j,k) parity (whole x-rows per colour, valid because the
+ * Q1 face weights vanish), full-weighting restriction (= P^T) with the
  * Galerkin coarse operator, which for nested Q1 spaces equals twice the fine
  * stencil (coarse rhs = R r / 2), vertex trilinear prolongation.  Coarsest
- * level: 64 Gauss-Seidel sweeps.
+ * level: 4 Gauss-Seidel sweeps (more change no cycle count).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -33,12 +34,14 @@
 #include <float.h>
 #include "orc.h"
 
-#define NCOARSE_SWEEPS 64
+#define NCOARSE_SWEEPS 4
 
+/* a periodic dimension is halved only while the half stays even, so that
+ * the parity colourings of the smoothers stay valid across the wrap */
 static int can_coarsen(int fem, int d, int prd)
 {
+    if (prd) return (d % 4 == 0) && d / 2 >= 2;
     if (!fem) return (d % 2 == 0) && d / 2 >= 2;
-    if (prd) return (d % 2 == 0) && d / 2 >= 2;
     return (d % 2 == 1) && (d - 1) / 2 + 1 >= 2 && d >= 3;
 }
 
@@ -75,6 +78,7 @@ void orc_mg_setup(orc_mg *G, int fem, const int d[3], const int prd[3])
     G->max_norm_tol = c * eps * 1e2;
     G->log_tol = log10(G->tol);
     G->v_up = 2;
+    G->coarse_sweeps = NCOARSE_SWEEPS;
     G->v_down = 2;
     G->iP = 16;
     G->iV = 2 * G->iP;
@@ -199,16 +203,18 @@ double orc_mg_resid_max(orc_mg *G, int lev)
     return mx;
 }
 
-/* one Gauss-Seidel sweep: MAC red-black, FEM 8-colour */
+/* one Gauss-Seidel sweep: MAC red-black by (i+j+k) parity; FEM four colours
+ * by (j,k) parity -- the Q1 stencil's face weights are zero, so nodes that
+ * differ only in i are not coupled and each colour is a set of whole x-rows */
 static void smooth(orc_mg *G, int lev)
 {
     orc_mglev *L = &G->lev[lev];
-    int ncol = G->fem ? 8 : 2;
+    int ncol = G->fem ? 4 : 2;
     for (int col = 0; col < ncol; col++)
         for (int k = 0; k < L->d[2]; k++)
             for (int j = 0; j < L->d[1]; j++)
                 for (int i = 0; i < L->d[0]; i++) {
-                    int mycol = G->fem ? ((i & 1) + 2 * (j & 1) + 4 * (k & 1)) : ((i + j + k) & 1);
+                    int mycol = G->fem ? ((j & 1) + 2 * (k & 1)) : ((i + j + k) & 1);
                     if (mycol != col) continue;
                     int c = i + L->d[0] * (j + L->d[1] * k);
                     double dg, off;
@@ -307,7 +313,7 @@ static void prolong_add(orc_mg *G, int lev)
 static void vcycle_rec(orc_mg *G, int lev)
 {
     if (lev == G->nlev - 1) {
-        for (int s = 0; s < NCOARSE_SWEEPS; s++) smooth(G, lev);
+        for (int s = 0; s < G->coarse_sweeps; s++) smooth(G, lev);
         return;
     }
     for (int s = 0; s < G->v_down; s++) smooth(G, lev);

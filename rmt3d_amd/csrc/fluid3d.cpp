@@ -23,8 +23,11 @@ static const int ORC_VEL = 1, ORC_PRES = 2, ORC_FVEL = 4, ORC_C = 8, ORC_X = 16,
 // ---------------------------------------------------------------------------
 // multigrid
 // ---------------------------------------------------------------------------
+// a periodic dimension is halved only while the half stays even, keeping the
+// smoothers' parity colourings valid across the wrap
 static bool can_coarsen(int fem, int d, int prd) {
-    if (!fem || prd) return (d % 2 == 0) && d / 2 >= 2;
+    if (prd) return (d % 4 == 0) && d / 2 >= 2;
+    if (!fem) return (d % 2 == 0) && d / 2 >= 2;
     return (d % 2 == 1) && d >= 3 && (d - 1) / 2 + 1 >= 2;
 }
 static int coarse_dim(int fem, int d, int prd) { return (!fem || prd) ? d / 2 : (d - 1) / 2 + 1; }
@@ -92,7 +95,7 @@ double MGSolver::resid_max(hipStream_t s) {
 }
 
 void MGSolver::vcycle64(hipStream_t s, int lev) {
-    int ncol = fem ? 8 : 2;
+    int ncol = fem ? 4 : 2;
     if (lev == nlev - 1) {
         launch_mg_coarse(s, fem, L[lev], prd, coarse_sweeps);
         return;
@@ -117,7 +120,7 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
 }
 
 void MGSolver::vcycle32(hipStream_t s, int lev) {
-    int ncol = fem ? 8 : 2;
+    int ncol = fem ? 4 : 2;
     if (lev == nlev - 1) {
         launch_mg_coarse(s, fem, Lf[lev], prd, coarse_sweeps);
         return;
@@ -271,6 +274,11 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     objs_in.assign(o, o + n_obj);
     g.m = cfg.sys_size[0]; g.n = cfg.sys_size[1]; g.o = cfg.sys_size[2];
     if (g.m < 4 || g.n < 4 || g.o < 4) { err = "grid too small"; return RMT3D_ERR_ARG; }
+    for (int d = 0; d < 3; d++)
+        if (cfg.prd[d] && (cfg.sys_size[d] % 2)) {
+            err = "periodic dimensions must have an even number of cells";
+            return RMT3D_ERR_ARG;
+        }
     g.m4 = g.m + 4; g.n4 = g.n + 4; g.o4 = g.o + 4;
     g.mn4 = g.m4 * g.n4;
     g.Ng = g.mn4 * g.o4;

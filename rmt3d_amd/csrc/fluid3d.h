@@ -38,7 +38,7 @@ struct MGSolver {
     double max_norm_tol = 0;
     double tol = 0, log_tol = 0;
     int v_up = 2, v_down = 2;
-    int coarse_sweeps = 64;
+    int coarse_sweeps = 4;
     // mg_pred (mg_probs.hh:37-78)
     int iP = 16, iV = 32;
     double V_avg = 32, V_var = 1e6, alpha = 0;

@@ -367,15 +367,29 @@ __global__ void k_scan_blocks(const int *in, int *out, int *bsum, int n) {
     if (t < n) out[t] = sh[threadIdx.x] - v;
     if (threadIdx.x == SCAN_BS - 1) bsum[blockIdx.x] = sh[SCAN_BS - 1];
 }
+// exclusive scan of the block sums in one workgroup of SCAN_BS threads, each
+// thread owning a contiguous run of ceil(nb / SCAN_BS) sums
 __global__ void k_scan_sums(int *bsum, int nb, int *total) {
-    if (threadIdx.x != 0) return;
-    int run = 0;
-    for (int b = 0; b < nb; b++) {
+    __shared__ int sh[SCAN_BS];
+    int per = (nb + SCAN_BS - 1) / SCAN_BS;
+    int b0 = threadIdx.x * per, b1 = min(b0 + per, nb);
+    int loc = 0;
+    for (int b = b0; b < b1; b++) loc += bsum[b];
+    sh[threadIdx.x] = loc;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BS; off <<= 1) {
+        int add = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - loc;
+    for (int b = b0; b < b1; b++) {
         int v = bsum[b];
         bsum[b] = run;
         run += v;
     }
-    *total = run;
+    if (threadIdx.x == SCAN_BS - 1) *total = sh[SCAN_BS - 1];
 }
 __global__ void k_scan_add(int *out, const int *bsum, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -459,7 +473,7 @@ void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, in
     hipLaunchKernelGGL(k_count_cells, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
     int nb = (g.Ng + SCAN_BS - 1) / SCAN_BS;
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BS), 0, s, B.cnt, B.start, W.scan_tmp, g.Ng);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, W.scan_tmp, nb, B.n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, s, W.scan_tmp, nb, B.n);
     hipLaunchKernelGGL(k_scan_add, g1(g.Ng), dim3(256), 0, s, B.start, W.scan_tmp, g.Ng);
     hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(1), 0, s, B, status);
     hipLaunchKernelGGL(k_place, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);

@@ -6,8 +6,8 @@
 // (Utilities/mg_probs.hh:281-343, 458-530) in closed form: MAC 7-point with
 // the missing wall face folded into the centre; FEM 27-point Q1 with wall
 // entries halved per wall direction whose offset is zero.  Periodic
-// dimensions wrap.  Smoothers: red-black (MAC) and 2x2x2-colour (FEM)
-// Gauss-Seidel, one launch per colour; inside a colour no two unknowns are
+// dimensions wrap.  Smoothers: red-black (MAC) and four-colour by (j,k)
+// parity (FEM: whole x-rows per colour) Gauss-Seidel, one launch per colour; inside a colour no two unknowns are
 // neighbours, so the update order is free and the result is deterministic.
 #include "dev.h"
 #include "launch.h"
@@ -82,16 +82,52 @@ __global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color) {
     L.x[c] = (L.b[c] - s) / dg;
 }
 
+// interior node (no wall, no wrap): the 20 non-zero neighbours in the order
+// of fem_off's (k, j, i) loop, so the sum is bit-identical to the general path
+template <typename T>
+__device__ __forceinline__ T fem_off_interior(const T *x, int c, int D0, int D01) {
+    T s = 0;
+    const T *p = x + c - D01;
+    s += T(-1) * p[-D0 - 1]; s += T(-2) * p[-D0]; s += T(-1) * p[-D0 + 1];
+    s += T(-2) * p[-1];      s += T(-2) * p[1];
+    s += T(-1) * p[D0 - 1];  s += T(-2) * p[D0];  s += T(-1) * p[D0 + 1];
+    p = x + c;
+    s += T(-2) * p[-D0 - 1]; s += T(-2) * p[-D0 + 1];
+    s += T(-2) * p[D0 - 1];  s += T(-2) * p[D0 + 1];
+    p = x + c + D01;
+    s += T(-1) * p[-D0 - 1]; s += T(-2) * p[-D0]; s += T(-1) * p[-D0 + 1];
+    s += T(-2) * p[-1];      s += T(-2) * p[1];
+    s += T(-1) * p[D0 - 1];  s += T(-2) * p[D0];  s += T(-1) * p[D0 + 1];
+    return s;
+}
+
+// four colours by (j,k) parity: one colour is a set of whole x-rows
 template <typename T>
 __global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color) {
-    int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + (color & 1);
-    int j = 2 * blockIdx.y + ((color >> 1) & 1);
-    int k = 2 * blockIdx.z + ((color >> 2) & 1);
-    if (i >= L.d[0] || j >= L.d[1] || k >= L.d[2]) return;
-    int c = i + L.d[0] * (j + L.d[1] * k);
-    T dg;
-    T s = fem_off(L, P, L.x, i, j, k, dg);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = 2 * blockIdx.y + (color & 1);
+    int k = 2 * blockIdx.z + ((color >> 1) & 1);
+    int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
+    if (i >= D0 || j >= D1 || k >= D2) return;
+    int c = i + D0 * (j + D1 * k);
+    T s, dg;
+    if (i >= 1 && i <= D0 - 2 && j >= 1 && j <= D1 - 2 && k >= 1 && k <= D2 - 2) {
+        s = fem_off_interior(L.x, c, D0, D0 * D1);
+        dg = T(32);
+    } else {
+        s = fem_off(L, P, L.x, i, j, k, dg);
+    }
     L.x[c] = (L.b[c] - s) / dg;
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_off(const MGLevel<T> &L, const Prd &P, int fem, int i, int j, int k, int c, T &dg) {
+    int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
+    if (fem && i >= 1 && i <= D0 - 2 && j >= 1 && j <= D1 - 2 && k >= 1 && k <= D2 - 2) {
+        dg = T(32);
+        return fem_off_interior(L.x, c, D0, D0 * D1);
+    }
+    return fem ? fem_off(L, P, L.x, i, j, k, dg) : mac_off(L, P, L.x, i, j, k, dg);
 }
 
 template <typename T>
@@ -101,30 +137,37 @@ __global__ void k_resid(MGLevel<T> L, Prd P, int fem) {
     if (i >= L.d[0]) return;
     int c = i + L.d[0] * (j + L.d[1] * k);
     T dg;
-    T s = fem ? fem_off(L, P, L.x, i, j, k, dg) : mac_off(L, P, L.x, i, j, k, dg);
+    T s = apply_off(L, P, fem, i, j, k, c, dg);
     L.r[c] = L.b[c] - (s + dg * L.x[c]);
 }
 
+// FP64 residual and its max norm: grid-stride blocks of 256, one atomic per
+// block (non-negative doubles order as their bit patterns; a NaN compares
+// above every finite value)
 __global__ void k_resid64_max(MGLevel<double> L, Prd P, int fem, unsigned long long *maxbits) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    double a = 0;
-    if (i < L.d[0]) {
-        int c = i + L.d[0] * (j + L.d[1] * k);
+    __shared__ unsigned long long sh[4];
+    unsigned long long v = 0;
+    int D0 = L.d[0], D1 = L.d[1];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < L.nn; c += gridDim.x * blockDim.x) {
+        int i = c % D0, j = (c / D0) % D1, k = c / (D0 * D1);
         double dg;
-        double s = fem ? fem_off(L, P, L.x, i, j, k, dg) : mac_off(L, P, L.x, i, j, k, dg);
+        double s = apply_off(L, P, fem, i, j, k, c, dg);
         double r = L.b[c] - (s + dg * L.x[c]);
         L.r[c] = r;
-        a = fabs(r);
+        unsigned long long b = (unsigned long long)__double_as_longlong(fabs(r));
+        v = b > v ? b : v;
     }
-    // wavefront max, then one atomic per wavefront (non-negative doubles order
-    // as their bit patterns; a NaN compares above every finite value)
-    unsigned long long v = (unsigned long long)__double_as_longlong(a);
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = sh[w] > m ? sh[w] : m;
+        atomicMax(maxbits, m);
+    }
 }
 
 template <typename T>
@@ -203,24 +246,36 @@ __global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
     F.x[i + F.d[0] * (j + F.d[1] * k)] += v;
 }
 
-// coarsest level: Gauss-Seidel sweeps inside one workgroup
+// coarsest level: coloured Gauss-Seidel sweeps inside one workgroup, on an
+// LDS copy when it fits
 template <typename T>
-__global__ void k_coarse(MGLevel<T> L, Prd P, int fem, int sweeps) {
+__global__ void k_coarse(MGLevel<T> L, Prd P, int fem, int sweeps, int use_lds) {
+    extern __shared__ unsigned char smem[];
     int n = L.nn;
-    int ncol = fem ? 8 : 2;
+    int ncol = fem ? 4 : 2;
+    MGLevel<T> W = L;
+    if (use_lds) {
+        T *xs = (T *)smem, *bs = xs + n;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) { xs[c] = L.x[c]; bs[c] = L.b[c]; }
+        W.x = xs;
+        W.b = bs;
+        __syncthreads();
+    }
     for (int sw = 0; sw < sweeps; sw++) {
         for (int col = 0; col < ncol; col++) {
             for (int c = threadIdx.x; c < n; c += blockDim.x) {
-                int i = c % L.d[0], j = (c / L.d[0]) % L.d[1], k = c / (L.d[0] * L.d[1]);
-                int my = fem ? ((i & 1) + 2 * (j & 1) + 4 * (k & 1)) : ((i + j + k) & 1);
+                int i = c % W.d[0], j = (c / W.d[0]) % W.d[1], k = c / (W.d[0] * W.d[1]);
+                int my = fem ? ((j & 1) + 2 * (k & 1)) : ((i + j + k) & 1);
                 if (my != col) continue;
                 T dg;
-                T s = fem ? fem_off(L, P, L.x, i, j, k, dg) : mac_off(L, P, L.x, i, j, k, dg);
-                L.x[c] = (L.b[c] - s) / dg;
+                T s = fem ? fem_off(W, P, W.x, i, j, k, dg) : mac_off(W, P, W.x, i, j, k, dg);
+                W.x[c] = (W.b[c] - s) / dg;
             }
             __syncthreads();
         }
     }
+    if (use_lds)
+        for (int c = threadIdx.x; c < n; c += blockDim.x) L.x[c] = W.x[c];
 }
 
 __global__ void k_cast_d2f(const double *a, float *b, int n) {
@@ -246,8 +301,8 @@ void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd
         dim3 gr((hx + 63) / 64, L.d[1], L.d[2]);
         hipLaunchKernelGGL(k_mac_smooth<T>, gr, dim3(64), 0, s, L, P, color);
     } else {
-        int hx = (L.d[0] + 1) / 2, hy = (L.d[1] + 1) / 2, hz = (L.d[2] + 1) / 2;
-        dim3 gr((hx + 63) / 64, hy, hz);
+        int hy = (L.d[1] + 1) / 2, hz = (L.d[2] + 1) / 2;
+        dim3 gr((L.d[0] + 63) / 64, hy, hz);
         hipLaunchKernelGGL(k_fem_smooth<T>, gr, dim3(64), 0, s, L, P, color);
     }
 }
@@ -258,8 +313,9 @@ void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[
 }
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits) {
-    dim3 gr((L.d[0] + 63) / 64, L.d[1], L.d[2]);
-    hipLaunchKernelGGL(k_resid64_max, gr, dim3(64), 0, s, L, mkprd(prd), fem, maxbits);
+    int nb = (L.nn + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_resid64_max, dim3(nb), dim3(256), 0, s, L, mkprd(prd), fem, maxbits);
 }
 template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
@@ -273,7 +329,9 @@ void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLeve
 }
 template <typename T>
 void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int sweeps) {
-    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(256), 0, s, L, mkprd(prd), fem, sweeps);
+    size_t lds = 2 * sizeof(T) * (size_t)L.nn;
+    int use = lds <= 48 * 1024;
+    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(256), use ? lds : 0, s, L, mkprd(prd), fem, sweeps, use);
 }
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n) {
     hipLaunchKernelGGL(k_cast_d2f, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);
