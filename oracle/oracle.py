@@ -32,6 +32,45 @@ def build(force=False):
     return LIB_PATH
 
 
+REF_DIR = os.path.join(HERE, "_ref")
+REF_LIB = os.path.join(REF_DIR, "libref_mat.so")
+REFERENCE = "/root/reference"
+_ref = None
+
+
+def build_ref(force=False):
+    """Compile oracle/_ref/libref_mat.so from the reference's own matrix
+    headers (oracle/ref.mk) when the reference tree is present.  Returns the
+    library path, or None when the reference is absent."""
+    import subprocess
+    if not os.path.isdir(REFERENCE):
+        return REF_LIB if os.path.exists(REF_LIB) else None
+    root = os.path.dirname(HERE)
+    cmd = ["make", "-s", "-f", os.path.join("oracle", "ref.mk"), "REF=" + REFERENCE]
+    if force:
+        cmd.insert(1, "-B")
+    subprocess.check_call(cmd, cwd=root)
+    return REF_LIB
+
+
+def ref_lib():
+    """The reference's matrix algebra (None when it was never built)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_LIB):
+            return None
+        L = C.CDLL(REF_LIB)
+        dp = C.POINTER(C.c_double)
+        L.ref_matrix_inverse.argtypes = [dp, dp]
+        L.ref_matrix_det.argtypes = [dp]
+        L.ref_matrix_det.restype = C.c_double
+        L.ref_matrix_aat.argtypes = [dp, dp]
+        L.ref_matrix_ata.argtypes = [dp, dp]
+        L.ref_mat3_inverse.argtypes = [dp, dp, dp]
+        _ref = L
+    return _ref
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -71,6 +110,14 @@ def lib():
         L.oracle_mat_inverse.argtypes = [dp, dp]
         L.oracle_mat_aat.argtypes = [dp, dp]
         L.oracle_mat_ata.argtypes = [dp, dp]
+        L.oracle_mat_det.argtypes = [dp]
+        L.oracle_mat_det.restype = D
+        L.oracle_mat3_inverse.argtypes = [dp, dp, dp]
+        L.oracle_det_sin.argtypes = [D]
+        L.oracle_det_sin.restype = D
+        L.oracle_det_cos.argtypes = [D]
+        L.oracle_det_cos.restype = D
+        L.oracle_mg_set_coarse_sweeps.argtypes = [P, C.c_int]
         L.oracle_heaviside.argtypes = [P, D]
         L.oracle_heaviside.restype = D
         L.oracle_lsq.argtypes = [C.c_int, dp, dp, dp, D, dp, dp, dp, dp]

@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "orc.h"
+#include "orc_detmath.h"
 
 double orc_mono_diff(double u0, double u1, double u2, double u3, double u4);
 void orc_godunov_set_velocity(double *nl, double *vl, double *wl, double *nr, double *vr, double *wr);
@@ -15,6 +16,8 @@ void orc_godunov_set_refmap(double nl, double nr, double *ml, double *mvl, doubl
 void orc_mat_inverse(const double v[9], double m2[9]);
 void orc_mat_aat(const double F[9], double s[6]);
 void orc_mat_ata(const double F[9], double s[6]);
+double orc_mat_det(const double v[9]);
+void orc_mat3_inverse(const double a[9], double out[9], double *dt);
 
 orc_sim *oracle_create(const rmt3d_config *cfg, const rmt3d_object *objs) { return orc_create(cfg, objs); }
 void oracle_destroy(orc_sim *S) { orc_destroy(S); }
@@ -151,6 +154,8 @@ void oracle_godunov_refmap(double nl, double nr, double *m6)
 void oracle_mat_inverse(const double *v, double *o) { orc_mat_inverse(v, o); }
 void oracle_mat_aat(const double *v, double *o) { orc_mat_aat(v, o); }
 void oracle_mat_ata(const double *v, double *o) { orc_mat_ata(v, o); }
+double oracle_mat_det(const double *v) { return orc_mat_det(v); }
+void oracle_mat3_inverse(const double *v, double *o, double *det) { orc_mat3_inverse(v, o, det); }
 double oracle_heaviside(orc_sim *S, double phi) { return orc_heaviside(S, phi); }
 int oracle_lsq(int npt, const double *pos, const double *val, const double *w, double wsum,
                double *davg, double *ravg, double *coeff, double *det)
@@ -159,3 +164,7 @@ int oracle_lsq(int npt, const double *pos, const double *val, const double *w, d
 }
 
 void oracle_mg_set_coarse_sweeps(orc_sim *S, int n) { S->mac.coarse_sweeps = n; S->fem.coarse_sweeps = n; }
+
+/* the deterministic sine/cosine shared with the HIP path */
+double oracle_det_sin(double x) { return det_sin(x); }
+double oracle_det_cos(double x) { return det_cos(x); }

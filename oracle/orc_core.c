@@ -1016,13 +1016,13 @@ static void update_reference_map_full(orc_sim *S)
 /* stresses (b)                                                         */
 /* ------------------------------------------------------------------ */
 /* matrix::det / inverse / aat / ata (Utilities/matlib.hh:227-232,303-329,366-370) */
-static double mat_det(const double v[9])
+double orc_mat_det(const double v[9])
 {
     return v[0] * (v[4] * v[8] - v[5] * v[7]) + v[3] * (v[2] * v[7] - v[1] * v[8]) + v[6] * (v[1] * v[5] - v[2] * v[4]);
 }
 void orc_mat_inverse(const double v[9], double m2[9])
 {
-    double d = mat_det(v);
+    double d = orc_mat_det(v);
     for (int j = 0; j < 3; j++)
         for (int i = 0; i < 3; i++) {
             m2[3 * j + i] = v[3 * ((i + 1) % 3) + (j + 1) % 3] * v[3 * ((i + 2) % 3) + (j + 2) % 3] -
@@ -1137,7 +1137,7 @@ static void solid_stress(orc_sim *S, int eid, int F, double ss[3], double *sfrac
         double gx[9] = {dR[0][0], dR[1][0], dR[2][0], dR[0][1], dR[1][1], dR[2][1], dR[0][2], dR[1][2], dR[2][2]};
         double Fd[9];
         orc_mat_inverse(gx, Fd);
-        rf->detF[F] = mat_det(Fd);
+        rf->detF[F] = orc_mat_det(Fd);
         double sg[6], cg[6];
         orc_mat_aat(Fd, sg);
         orc_mat_ata(Fd, cg);

@@ -196,7 +196,7 @@ static void scan_ghost_region(orc_sim *S)
 }
 
 /* mat3::inverse (mat3.hh:61-67) */
-static void mat3_inverse(const double a[9], double out[9], double *dt)
+void orc_mat3_inverse(const double a[9], double out[9], double *dt)
 {
     double a11 = a[0], a12 = a[1], a13 = a[2], a21 = a[3], a22 = a[4], a23 = a[5],
            a31 = a[6], a32 = a[7], a33 = a[8];
@@ -243,7 +243,7 @@ int orc_least_square_fit(int npt, const double *pos, const double *val,
         }
     }
     double A[9] = {a[0], a[3], a[4], a[3], a[1], a[5], a[4], a[5], a[2]}, Ai[9];
-    mat3_inverse(A, Ai, det);
+    orc_mat3_inverse(A, Ai, det);
     if (fabs(*det) < 1e-17) return 0;
     for (int i = 0; i < 3; i++) {
         int x = 3 * i, y = x + 1, z = x + 2;

@@ -37,8 +37,11 @@ def sphere(center, radius, G=1.0, srho=1.0, stretch_dir=None, sqrt_lam=None):
     for i in range(3):
         o.basic[i] = center[i]
     o.basic[3] = radius
-    o.extra[0] = 0.0 if stretch_dir is None else float(stretch_dir)
-    o.extra[1] = 1.0 if sqrt_lam is None else sqrt_lam
+    if stretch_dir is None:
+        o.extra[0] = 1.0              # sim_params' default extra_specs row
+    else:
+        o.extra[0] = float(stretch_dir)
+        o.extra[1] = 1.0 if sqrt_lam is None else sqrt_lam
     o.srho = srho
     o.shear_mod = G
     return o

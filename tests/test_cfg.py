@@ -1,0 +1,60 @@
+"""The .cfg reader of the C ABI (rmt3d_parse_cfg, restating
+Sim/sim_params.cc:1-420) on the shipped workload files and on the
+reference's sample configurations (CPU only: parsing touches no GPU)."""
+import os
+
+import pytest
+
+from rmt3d_amd import _abi, configs, lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CFG_DIR = os.path.join(os.path.dirname(HERE), "rmt3d_amd", "sim_configs")
+REF_CFG = "/root/reference/sim_configs"
+
+
+def _fields(s):
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        if hasattr(v, "__len__"):
+            v = [list(x) if hasattr(x, "__len__") else x for x in v]
+        out[name] = v
+    return out
+
+
+@pytest.mark.parametrize("name,builder", [
+    ("taylor_green_32", lambda: configs.taylor_green(32)),
+    ("cavity_sphere_64", lambda: configs.cavity_sphere(64)),
+    ("stretched_sphere_128", lambda: configs.stretched_sphere(128)),
+])
+def test_workload_cfg_matches_builder(name, builder):
+    cfg, objs = lib.parse_cfg(os.path.join(CFG_DIR, name + ".cfg"))
+    cb, ob = builder()
+    assert _fields(cfg) == _fields(cb)
+    assert len(objs) == len(ob)
+    for a, b in zip(objs, ob):
+        assert _fields(a) == _fields(b)
+
+
+def test_parse_rejects_missing_file():
+    with pytest.raises(lib.Rmt3dError):
+        lib.parse_cfg(os.path.join(CFG_DIR, "does_not_exist.cfg"))
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CFG), reason="reference sample configurations absent")
+def test_reference_sample_configs_parse():
+    cfg, objs = lib.parse_cfg(os.path.join(REF_CFG, "pre-stretched_sphere.cfg"))
+    assert list(cfg.sys_size) == [64, 64, 64]
+    assert list(cfg.prd) == [1, 1, 1]
+    assert cfg.fmu == 0.005 and cfg.T == 8 and cfg.nlayers == 4 and cfg.wt_n == 1.0
+    assert cfg.sim_type == _abi.SIM_OBJECTS and cfg.n_obj == 1
+    o = objs[0]
+    assert o.type == _abi.SPHERE
+    assert list(o.basic) == [0.5, 0.5, 0.5, 0.2]
+    assert o.extra[0] == 0 and o.extra[1] == 1.1
+    assert o.shear_mod == 0.25 and o.srho == 1.0
+    cfg, objs = lib.parse_cfg(os.path.join(REF_CFG, "lid_driven_with_sphere.cfg"))
+    assert cfg.n_obj == 1 and objs[0].type == _abi.SPHERE
+    assert list(objs[0].basic) == [0.6, 0.5, 0.5, 0.2] and objs[0].extra[1] == 1.0
+    for f in ("ftest.cfg", "stest.cfg", "lid_driven_cavity.cfg"):
+        lib.parse_cfg(os.path.join(REF_CFG, f))
