@@ -9,6 +9,12 @@ G = 1 (rmt3d_amd/configs.py: stretched_sphere).  Inputs are resident in HBM
 when the timed region starts; the timed region is exactly --steps steps,
 bracketed by device synchronisation.
 
+By default the FP64 path and the mixed path (FP32 multigrid preconditioner
+inside FP64 defect correction) are timed one after the other from the same
+initial state; value is the mixed path, the FP64 rate, the speedup and the
+mixed path's relative L2 distance from the FP64 fields after the same steps
+sit in config.
+
 The JSON line carries:
   value        whole-job cell-steps/s of the measured precision path
   roofline     the dominant kernel's algorithmic HBM bytes per launch over its
@@ -36,24 +42,31 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="both: time the FP64 path, then the mixed path; value is the mixed path")
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
-def fem_colour_bytes(nodes, word):
-    """Algorithmic HBM bytes of one colour pass of the nodal 27-point
-    Gauss-Seidel smoother: the other seven colours' unknowns are read once
-    (7/8 N words), the pass's own rhs read and unknowns written (1/8 N each)."""
-    return word * nodes * (7.0 / 8 + 1.0 / 8 + 1.0 / 8)
+# Nodal 27-point smoother, four colours by (j,k) parity: a colour pass reads
+# the unknowns of the three other colours (3/4 N words), its own rhs (1/4 N)
+# and writes its own unknowns (1/4 N).
+FEM_COLOUR_WORDS = 3.0 / 4 + 1.0 / 4 + 1.0 / 4
+# One V-cycle, per finest unknown, at 2 pre + 2 post sweeps: the sum over the
+# levels (geometric factor 8/7) of the smoothing passes, residual, restriction,
+# prolongation and coarse-correction zeroing.
+VCYCLE_WORDS = 21.0
+# One FP64 defect-correction step of the mixed path adds the FP64 residual
+# (read x, b, write r) per finest unknown.
+DEFECT_WORDS64 = 3.0
 
 
-def run_gpu(args):
+def run_gpu(args, precision):
     from rmt3d_amd import _abi, configs
     from rmt3d_amd.lib import Simulation
     cfg, objs = configs.stretched_sphere(args.n, steps=args.steps + args.warmup + 10)
-    prec = _abi.PREC_MIXED if args.precision == "mixed" else _abi.PREC_FP64
+    prec = _abi.PREC_MIXED if precision == "mixed" else _abi.PREC_FP64
     sim = Simulation(cfg, objs, prec)
     e = sim.initialize()
     sim.prepare_run(e)
@@ -70,25 +83,36 @@ def run_gpu(args):
             raise RuntimeError("step returned %d" % rc)
     sim.synchronize()
     t1 = time.perf_counter()
-    ms_fem, n_fem = sim.profile_read(0)
-    ms_mac, n_mac = sim.profile_read(1)
+    prof = {k: sim.profile_read(i) for i, k in enumerate(
+        ("fem_smooth", "mac_smooth", "stress", "advect", "fem_vcycle", "mac_vcycle"))}
     sim.profile(False)
     elapsed = t1 - t0
     m, n, o = cfg.sys_size
     cells = m * n * o
-    word = 8
     nodes = (m + 1) * (n + 1) * (o + 1)
-    avg_s = (ms_fem / max(n_fem, 1)) * 1e-3
-    bytes_launch = fem_colour_bytes(nodes, word)
+    word = 4 if precision == "mixed" else 8
+    ms, launches = prof["fem_smooth"]
+    avg_s = ms / max(launches, 1) * 1e-3
+    bytes_launch = word * nodes * FEM_COLOUR_WORDS
     achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "fem smoother colour pass (finest level, %d^3 nodes)" % (m + 1),
+            "kernel": "k_fem_smooth<%s> colour pass, finest level (%d^3 nodes)"
+                      % ("float" if word == 4 else "double", m + 1),
             "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 3),
-            "launches": n_fem}
-    res = {"value": cells * args.steps / elapsed, "elapsed": elapsed, "roofline": roof,
-           "ms_fem_smooth": ms_fem, "ms_mac_smooth": ms_mac, "n_mac": n_mac,
-           "mac_cycles": sim.diag(1), "fem_cycles": sim.diag(2), "dt": sim.dt}
+            "launches": launches}
+    vc = {}
+    for key, N in (("fem", nodes), ("mac", cells)):
+        ms_v, nv = prof[key + "_vcycle"]
+        t_v = ms_v / max(nv, 1) * 1e-3
+        b = VCYCLE_WORDS * N * word + (DEFECT_WORDS64 * N * 8 if precision == "mixed" else 0)
+        vc[key] = {"us": round(t_v * 1e6, 2), "gbs": round(b / t_v / 1e9, 1) if t_v > 0 else 0.0,
+                   "frac": round(b / t_v / 1e9 / HBM_PEAK_GBS, 4) if t_v > 0 else 0.0, "cycles": nv}
+    fields = sim.download(band=False)
+    res = {"value": cells * args.steps / elapsed, "elapsed": elapsed, "roofline": roof, "vcycle": vc,
+           "mac_cycles": sim.diag(1), "fem_cycles": sim.diag(2), "dt": sim.dt,
+           "stress_ms_per_step": prof["stress"][0] / args.steps,
+           "fields": {k: fields[k] for k in ("u", "v", "w", "p")}}
     sim.close()
     return res
 
@@ -114,6 +138,13 @@ def run_cpu(args):
                       % (args.cpu_steps, args.n, multiprocessing.cpu_count())}
 
 
+def _rel(a, b):
+    import numpy as np
+    d = np.linalg.norm(a - b)
+    n = np.linalg.norm(b)
+    return float(d / n) if n > 0 else float(d)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -121,7 +152,29 @@ def main():
         print("bench.py: this build runs the FSI step on one GPU only; --gpus %d is not supported "
               "(multi-GPU sharding is design only)" % max(args.gpus, world), file=sys.stderr)
         sys.exit(2)
-    g = run_gpu(args)
+    runs = {}
+    for prec in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
+        runs[prec] = run_gpu(args, prec)
+        print("bench: %s path %.1f ms/step" % (prec, 1e3 * runs[prec]["elapsed"] / args.steps),
+              file=sys.stderr, flush=True)
+    head = "mixed" if "mixed" in runs else "fp64"
+    g = runs[head]
+    config = {"workload": "stretched_sphere_%d" % args.n, "grid": [args.n] * 3, "objects": 1,
+              "lambda": 1.3, "G": 1.0, "mu": 0.01, "nlayers": 4, "precision": head,
+              "dt": g["dt"], "mac_vcycles_last_step": g["mac_cycles"],
+              "fem_vcycles_last_step": g["fem_cycles"], "vcycle": g["vcycle"],
+              "stress_ms_per_step": round(g["stress_ms_per_step"], 3)}
+    if len(runs) == 2:
+        f64, fmx = runs["fp64"]["fields"], runs["mixed"]["fields"]
+        import numpy as np
+        vel64 = np.concatenate([f64[k] for k in ("u", "v", "w")])
+        velmx = np.concatenate([fmx[k] for k in ("u", "v", "w")])
+        config["fp64_cell_steps_per_s"] = round(runs["fp64"]["value"], 1)
+        config["fp64_ms_per_step"] = round(1e3 * runs["fp64"]["elapsed"] / args.steps, 4)
+        config["mixed_speedup_vs_fp64"] = round(runs["mixed"]["value"] / runs["fp64"]["value"], 3)
+        config["mixed_rel_l2_vs_fp64"] = {"vel": _rel(velmx, vel64), "p": _rel(fmx["p"], f64["p"]),
+                                          "steps": args.warmup + args.steps}
+        config["fp64_vcycle"] = runs["fp64"]["vcycle"]
     line = {
         "metric": METRIC,
         "value": round(g["value"], 1),
@@ -133,12 +186,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": "stretched_sphere_%d" % args.n, "grid": [args.n] * 3, "objects": 1,
-                   "lambda": 1.3, "G": 1.0, "mu": 0.01, "nlayers": 4, "precision": args.precision,
-                   "dt": g["dt"], "mac_vcycles_last_step": g["mac_cycles"],
-                   "fem_vcycles_last_step": g["fem_cycles"]},
+        "config": config,
         "roofline": g["roofline"],
     }
     if not args.no_cpu:

@@ -174,7 +174,10 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
 /* Diagnostics: which = 0 -> div_u L2 (fluid_3d_utils.cc:507-653),
  * 1 -> number of V-cycles in the last MAC solve, 2 -> last FEM solve,
  * 3 -> last MAC residual, 4 -> last FEM residual, 5 -> dt_reg,
- * 6 -> eps (transition zone width), 7 -> max |det F - 1| on primary faces */
+ * 6 -> eps (transition zone width), 8 -> vmax (sim_manager), 9 -> K_stiff;
+ * multigrid layout (no counterpart in the reference): 10/11 -> MAC/FEM
+ * level count, 12/13 -> first level run by the one-workgroup tail kernel,
+ * 14/15 -> tail levels held in LDS (bit 0 FP64 hierarchy, bit 1 FP32). */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
 /* Per-stage device time accumulated by the step (ms), in the order of the
@@ -188,7 +191,9 @@ void *rmt3d_stream(rmt3d_ctx *ctx);
 /* Event timing of kernel classes on the context stream (measurement only;
  * no counterpart in the reference, whose timer buckets are wall clocks,
  * Utilities/timer.hh).  which: 0 finest-level FEM smoother colour pass,
- * 1 finest-level MAC smoother colour pass, 2 compute_stress, 3 advection. */
+ * 1 finest-level MAC smoother colour pass, 2 compute_stress, 3 advection,
+ * 4 one FEM V-cycle, 5 one MAC V-cycle (with the mixed path, one FP64
+ * defect-correction step around an FP32 V-cycle). */
 int rmt3d_profile(rmt3d_ctx *ctx, int enable);
 int rmt3d_profile_read(rmt3d_ctx *ctx, int which, double *ms, int *launches);
 /* hipDeviceSynchronize on the context's device. */

@@ -102,6 +102,12 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out) {
         case 6: *out = S.eps; return 0;
         case 8: *out = S.vmax; return 0;
         case 9: *out = S.K_stiff; return 0;
+        case 10: *out = S.mac.nlev; return 0;
+        case 11: *out = S.fem.nlev; return 0;
+        case 12: *out = S.mac.tail_lev; return 0;
+        case 13: *out = S.fem.tail_lev; return 0;
+        case 14: *out = S.mac.tail_lds64 + 2 * S.mac.tail_lds32; return 0;
+        case 15: *out = S.fem.tail_lds64 + 2 * S.fem.tail_lds32; return 0;
         }
     } catch (...) {
         return RMT3D_ERR_FATAL;

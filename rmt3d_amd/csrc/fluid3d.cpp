@@ -65,6 +65,14 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         if (!ok) break;
         for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
     }
+    // the tail starts at the first level small enough for one workgroup,
+    // with all its levels in LDS when they fit
+    tail_lev = nlev - 1;
+    while (tail_lev > 0 && L[tail_lev - 1].nn <= MG_TAIL_NN && nlev - (tail_lev - 1) <= MG_TAIL_MAXLEV &&
+           mg_tail_lds_bytes(L + tail_lev - 1, nlev - tail_lev + 1) <= MG_TAIL_LDS_MAX)
+        tail_lev--;
+    tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
+    tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
     double eps = DBL_EPSILON, c = fem ? 32.0 : 6.0;
     tol = c * c * eps * eps * 1e4;
     max_norm_tol = c * eps * 1e2;
@@ -96,8 +104,8 @@ double MGSolver::resid_max(hipStream_t s) {
 
 void MGSolver::vcycle64(hipStream_t s, int lev) {
     int ncol = fem ? 4 : 2;
-    if (lev == nlev - 1) {
-        launch_mg_coarse(s, fem, L[lev], prd, coarse_sweeps);
+    if (lev >= tail_lev) {
+        launch_mg_tail(s, fem, L + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds64);
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
@@ -121,8 +129,8 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
 
 void MGSolver::vcycle32(hipStream_t s, int lev) {
     int ncol = fem ? 4 : 2;
-    if (lev == nlev - 1) {
-        launch_mg_coarse(s, fem, Lf[lev], prd, coarse_sweeps);
+    if (lev >= tail_lev) {
+        launch_mg_tail(s, fem, Lf + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds32);
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
@@ -158,6 +166,12 @@ void MGSolver::cycle(hipStream_t s) {
     launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
 }
 
+void MGSolver::timed_cycle(hipStream_t s) {
+    if (vprof) vprof->begin(s);
+    cycle(s);
+    if (vprof) vprof->end(s);
+}
+
 // mg_prob::run (Utilities/mg_probs.cc:3-83), profile = false
 double MGSolver::run(hipStream_t s) {
     const int hard_cap = 20;
@@ -165,15 +179,15 @@ double MGSolver::run(hipStream_t s) {
     int cycles = 0;
     double res = resid_max(s);
     int pre = iV / iP;
-    for (int i = 0; i < pre; i++) { cycle(s); cycles++; }
+    for (int i = 0; i < pre; i++) { timed_cycle(s); cycles++; }
     res = resid_max(s);
     int extra = 0;
     while (res > thr && extra < hard_cap) {
         extra++;
-        for (int n = 0; n < extra; n++) { cycle(s); cycles++; }
+        for (int n = 0; n < extra; n++) { timed_cycle(s); cycles++; }
         res = resid_max(s);
     }
-    for (int i = 0; i < 2; i++) { cycle(s); cycles++; }
+    for (int i = 0; i < 2; i++) { timed_cycle(s); cycles++; }
     // mg_pred::update(int) (mg_probs.hh:51-66)
     double dV = iV - V_avg;
     int exreduce = (int)(dV / sqrt(V_var));
@@ -218,7 +232,7 @@ void Sim::alloc_band(Band &B, int cap) {
 }
 
 int Sim::profile(int enable) {
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < NPROF; k++) {
         KProf &P = kprof[k];
         if (!P.ev) {
             P.ev = new hipEvent_t[2 * KProf::CAP];
@@ -229,11 +243,13 @@ int Sim::profile(int enable) {
     }
     fem.prof = &kprof[0];
     mac.prof = &kprof[1];
+    fem.vprof = &kprof[4];
+    mac.vprof = &kprof[5];
     return 0;
 }
 
 int Sim::profile_read(int which, double *ms, int *launches) {
-    if (which < 0 || which > 3) return RMT3D_ERR_ARG;
+    if (which < 0 || which >= NPROF) return RMT3D_ERR_ARG;
     HCHK(hipStreamSynchronize(stream));
     KProf &P = kprof[which];
     double tot = 0;
@@ -248,7 +264,7 @@ int Sim::profile_read(int which, double *ms, int *launches) {
 }
 
 Sim::~Sim() {
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < NPROF; k++)
         if (kprof[k].ev) {
             for (int a = 0; a < 2 * KProf::CAP; a++) hipEventDestroy(kprof[k].ev[a]);
             delete[] kprof[k].ev;
@@ -451,6 +467,8 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     W.scan_tmp = (int *)dalloc(sizeof(int) * ((Ng + 1023) / 1024 + 1));
     W.err = (int *)dalloc(sizeof(int));
     d_status = (int *)dalloc(sizeof(int));
+    d_solid_list = (int *)dalloc(sizeof(int) * Ng);
+    d_solid_n = (int *)dalloc(sizeof(int));
     d_u64 = (unsigned long long *)dalloc(sizeof(unsigned long long) * 4);
     d_partial = (double *)dalloc(sizeof(double) * 1024);
     d_mean = (double *)dalloc(sizeof(double) * 4);
@@ -683,7 +701,7 @@ int Sim::step(int debug) {
     launch_zero_i32(stream, d_status, 1);
     Maps M = maps();
     kprof[2].begin(stream);
-    launch_stress(stream, g, P, d_obj, U, M);
+    launch_stress(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n);
     kprof[2].end(stream);
     kprof[3].begin(stream);
     launch_advect(stream, g, P, d_obj, U, M, d_status);
@@ -707,7 +725,7 @@ int Sim::step(int debug) {
         hipStreamSynchronize(stream);
         return RMT3D_REFMAP_UPDATE_ERR;
     }
-    launch_stress(stream, g, P, d_obj, U, M);
+    launch_stress(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n);
     launch_ustar(stream, g, P, d_obj, U, M);
     fill_bc_cc();
     comm_fluid(ORC_DVEL);
@@ -761,7 +779,7 @@ int Sim::init_iter(int init_err) {
     }
     P.dt = dt;
     P.time = time;
-    launch_stress(stream, g, P, d_obj, U, maps());
+    launch_stress(stream, g, P, d_obj, U, maps(), d_solid_list, d_solid_n);
     return check_status("init_iter");
 }
 

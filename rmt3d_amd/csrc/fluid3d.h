@@ -14,7 +14,7 @@ namespace rmt {
 // Event timing of one kernel class on the context stream: every launch of
 // the class is bracketed by two events; durations are summed on collect.
 struct KProf {
-    static const int CAP = 16384;
+    static const int CAP = 4096;
     hipEvent_t *ev = nullptr;
     int n = 0;           // events recorded (pairs)
     int enabled = 0;
@@ -39,12 +39,15 @@ struct MGSolver {
     double tol = 0, log_tol = 0;
     int v_up = 2, v_down = 2;
     int coarse_sweeps = 4;
+    int tail_lev = 0;                      // first level run by the one-workgroup tail kernel
+    int tail_lds64 = 0, tail_lds32 = 0;    // tail levels held in LDS
     // mg_pred (mg_probs.hh:37-78)
     int iP = 16, iV = 32;
     double V_avg = 32, V_var = 1e6, alpha = 0;
     int last_cycles = 0;
     double last_res = 0;
     KProf *prof = nullptr;     // times the finest-level smoother launches
+    KProf *vprof = nullptr;    // times whole V-cycles (one defect-correction step when mixed)
     unsigned long long *d_max = nullptr;
     unsigned long long *h_max = nullptr;   // pinned
     double tol_override = 0;               // > 0 replaces max_norm_tol
@@ -55,6 +58,7 @@ struct MGSolver {
     void vcycle64(hipStream_t s, int lev);
     void vcycle32(hipStream_t s, int lev);
     void cycle(hipStream_t s);              // one V-cycle (or one defect-correction step)
+    void timed_cycle(hipStream_t s);
     double run(hipStream_t s);
 };
 
@@ -90,6 +94,7 @@ class Sim {
     Scratch W{};
     double *d_wpow = nullptr;
     int *d_status = nullptr;
+    int *d_solid_list = nullptr, *d_solid_n = nullptr;   // cells on the solid stress path
     int *h_status = nullptr;           // pinned
     int *h_int = nullptr;              // pinned scratch
     double *h_dbl = nullptr;           // pinned scratch
@@ -100,7 +105,10 @@ class Sim {
     MGSolver mac, fem;
     std::vector<void *> allocs;
     double stage_ms[6] = {0, 0, 0, 0, 0, 0};
-    KProf kprof[4];             // 0 FEM smoother (finest), 1 MAC smoother (finest), 2 stress, 3 advect
+    // 0 FEM smoother colour pass (finest), 1 MAC smoother colour pass
+    // (finest), 2 stress, 3 advection, 4 FEM V-cycle, 5 MAC V-cycle
+    static const int NPROF = 6;
+    KProf kprof[NPROF];
     int profile(int enable);
     int profile_read(int which, double *ms, int *launches);
     bool timing = false;

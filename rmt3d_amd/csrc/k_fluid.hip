@@ -221,29 +221,69 @@ __device__ void collision_stress(const Geo &g, const Phys &P, const ObjC *O, con
 }
 
 // compute_stress over eid in -1..s+1 (padded 1..s+3)
-__global__ void k_stress(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > g.m + 3) return;
-    int ind = IX(g, i, j, k);
+// A cell takes the solid path when it holds a primary map of an object or a
+// band entry; every other cell's lower-face stresses are the fluid's alone,
+// mu du/dn, with the reference's blend (1 - sfrac) fs + ss at sfrac = ss = 0.
+__device__ __forceinline__ bool stress_solid_cell(const Maps &M, int ind) {
+    return (M.r.c[ind] != 0) || (M.b.cnt[ind] > 0);
+}
+
+// lower face F of a solid-path cell: every write is to this cell's sigma row F
+// and to row F of its own maps' face fields, so faces run independently
+__device__ void stress_face(const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M, int ind,
+                            int F) {
     int eid = ind - g.G0;
     const int str[3] = {1, g.m4, g.mn4};
     const double facs[3] = {g.dxsp, g.dysp, g.dzsp};
-    bool solid = (M.r.c[ind] != 0) || (M.b.cnt[ind] > 0);
-    for (int F = 0; F < 3; F++) {
-        double ss[3] = {0, 0, 0}, sfrac = 0;
-        if (solid) solid_stress(g, P, O, U, M, ind, F, ss, sfrac);
-        double sig[3];
-        if (sfrac >= 1) {
-            for (int s = 0; s < 3; s++) sig[s] = ss[s] / sfrac;
-        } else {
-            for (int s = 0; s < 3; s++) {
-                double fs = P.fmu * facs[F] * (U.vel[s][ind] - U.vel[s][ind - str[F]]);
-                sig[s] = (1 - sfrac) * fs + ss[s];
-            }
+    double ss[3] = {0, 0, 0}, sfrac = 0;
+    solid_stress(g, P, O, U, M, ind, F, ss, sfrac);
+    double sig[3];
+    if (sfrac >= 1) {
+        for (int s = 0; s < 3; s++) sig[s] = ss[s] / sfrac;
+    } else {
+        for (int s = 0; s < 3; s++) {
+            double fs = P.fmu * facs[F] * (U.vel[s][ind] - U.vel[s][ind - str[F]]);
+            sig[s] = (1 - sfrac) * fs + ss[s];
         }
-        if (solid) collision_stress(g, P, O, M, ind, eid, F, sig);
-        for (int s = 0; s < 3; s++) U.sigma[3 * F + s][ind] = sig[s];
+    }
+    collision_stress(g, P, O, M, ind, eid, F, sig);
+    for (int s = 0; s < 3; s++) U.sigma[3 * F + s][ind] = sig[s];
+}
+
+// fluid cells of padded 1..s+3 in place; solid cells appended to list
+__global__ void k_stress_fluid(Geo g, Phys P, Fluid U, Maps M, int *list, int *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1, k = blockIdx.z + 1;
+    bool in = i <= g.m + 3;
+    int ind = in ? IX(g, i, j, k) : 0;
+    bool solid = in && stress_solid_cell(M, ind);
+    // one atomic per wavefront for the solid cells it holds
+    unsigned long long bal = __ballot(solid);
+    int lane = threadIdx.x & 63;
+    int base = 0;
+    if (bal) {
+        int first = __ffsll((long long)bal) - 1;
+        if (lane == first) base = atomicAdd(count, __popcll(bal));
+        base = __shfl(base, first, 64);
+    }
+    if (solid) list[base + __popcll(bal & ((1ull << lane) - 1))] = ind;
+    if (!in || solid) return;
+    const int str[3] = {1, g.m4, g.mn4};
+    const double facs[3] = {g.dxsp, g.dysp, g.dzsp};
+    for (int F = 0; F < 3; F++)
+        for (int s = 0; s < 3; s++) {
+            double fs = P.fmu * facs[F] * (U.vel[s][ind] - U.vel[s][ind - str[F]]);
+            U.sigma[3 * F + s][ind] = 1.0 * fs + 0.0;
+        }
+}
+
+// one thread per (solid cell, face); the three faces of a cell sit in
+// neighbouring lanes
+__global__ void k_stress_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
+    int n = 3 * *count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int c = t / 3;
+        stress_face(g, P, O, U, M, list[c], t - 3 * c);
     }
 }
 
@@ -968,8 +1008,15 @@ __global__ void k_init_refmap(Geo g, Phys P, const ObjC *O, InitC I, RMap R, int
 // ---------------------------------------------------------------------------
 static inline dim3 rows(int nx, int ny, int nz, int bs) { return dim3((nx + bs - 1) / bs, ny, nz); }
 
-void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M) {
-    hipLaunchKernelGGL(k_stress, rows(g.m + 3, g.n + 3, g.o + 3, 64), dim3(64), 0, s, g, P, O, U, M);
+__global__ void k_zero1(int *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                   int *list, int *count) {
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, count);
+    hipLaunchKernelGGL(k_stress_fluid, rows(g.m + 3, g.n + 3, g.o + 3, 64), dim3(64), 0, s, g, P, U, M, list, count);
+    hipLaunchKernelGGL(k_stress_solid, dim3(8192), dim3(64), 0, s, g, P, O, U, M, list, count);
 }
 void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    int *status) {

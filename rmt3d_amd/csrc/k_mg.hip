@@ -38,48 +38,85 @@ __device__ __forceinline__ T mac_off(const MGLevel<T> &L, const Prd &P, const T 
     return s;
 }
 
+// The wall halving of a weight is a multiplication by 1/2 per wall dimension
+// with zero offset: exact, so it equals the division by 2, 4 or 8 of the
+// closed form.  rdg returns 1/dg, exact because dg is 32 times a power of two.
 template <typename T>
-__device__ __forceinline__ T fem_off(const MGLevel<T> &L, const Prd &P, const T *x, int i, int j, int k, T &dg) {
+__device__ __forceinline__ T fem_off(const MGLevel<T> &L, const Prd &P, const T *x, int i, int j, int k, T &dg,
+                                     T &rdg) {
     int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
+    int ia[3] = {nbw(i, -1, D0, P.p[0]), i, nbw(i, 1, D0, P.p[0])};
+    int ja[3] = {nbw(j, -1, D1, P.p[1]), j, nbw(j, 1, D1, P.p[1])};
+    int ka[3] = {nbw(k, -1, D2, P.p[2]), k, nbw(k, 1, D2, P.p[2])};
     bool w0 = !P.p[0] && (i == 0 || i == D0 - 1);
     bool w1 = !P.p[1] && (j == 0 || j == D1 - 1);
     bool w2 = !P.p[2] && (k == 0 || k == D2 - 1);
+    T sx = w0 ? T(0.5) : T(1), sy = w1 ? T(0.5) : T(1), sz = w2 ? T(0.5) : T(1);
     T s = 0;
     dg = 0;
-    for (int ok = -1; ok <= 1; ok++) {
-        int kk = nbw(k, ok, D2, P.p[2]);
-        if (kk < 0) continue;
-        for (int oj = -1; oj <= 1; oj++) {
-            int jj = nbw(j, oj, D1, P.p[1]);
-            if (jj < 0) continue;
-            for (int oi = -1; oi <= 1; oi++) {
-                int ii = nbw(i, oi, D0, P.p[0]);
-                if (ii < 0) continue;
-                int nz = (oi != 0) + (oj != 0) + (ok != 0);
+#pragma unroll
+    for (int ok = 0; ok < 3; ok++) {
+        if (ka[ok] < 0) continue;
+#pragma unroll
+        for (int oj = 0; oj < 3; oj++) {
+            if (ja[oj] < 0) continue;
+#pragma unroll
+            for (int oi = 0; oi < 3; oi++) {
+                if (ia[oi] < 0) continue;
+                const int nz = (oi != 1) + (oj != 1) + (ok != 1);
                 if (nz == 1) continue;
                 T w = (nz == 0) ? T(32) : (nz == 2) ? T(-2) : T(-1);
-                T f = 1;
-                if (w0 && oi == 0) f *= 2;
-                if (w1 && oj == 0) f *= 2;
-                if (w2 && ok == 0) f *= 2;
-                w /= f;
+                if (oi == 1) w *= sx;
+                if (oj == 1) w *= sy;
+                if (ok == 1) w *= sz;
                 if (nz == 0) dg = w;
-                else s += w * x[ii + D0 * (jj + D1 * kk)];
+                else s += w * x[ia[oi] + D0 * (ja[oj] + D1 * ka[ok])];
             }
         }
     }
+    rdg = T(0.03125);
+    if (w0) rdg *= T(2);
+    if (w1) rdg *= T(2);
+    if (w2) rdg *= T(2);
     return s;
+}
+template <typename T>
+__device__ __forceinline__ T fem_off(const MGLevel<T> &L, const Prd &P, const T *x, int i, int j, int k, T &dg) {
+    T rdg;
+    return fem_off(L, P, x, i, j, k, dg, rdg);
+}
+
+// quotient of non-negative integers (a < 2^25, d >= 2) from a float
+// reciprocal, corrected by the exact remainder (the estimate is off by at
+// most 2 in that range)
+__device__ __forceinline__ int qdiv(int a, int d, float rd) {
+    int q = (int)((float)a * rd);
+    int r = a - q * d;
+    while (r < 0) { q--; r += d; }
+    while (r >= d) { q++; r -= d; }
+    return q;
 }
 
 template <typename T>
-__global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color) {
-    int j = blockIdx.y, k = blockIdx.z;
-    int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
-    if (i >= L.d[0]) return;
+__device__ __forceinline__ void mac_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
     int c = i + L.d[0] * (j + L.d[1] * k);
     T dg;
     T s = mac_off(L, P, L.x, i, j, k, dg);
     L.x[c] = (L.b[c] - s) / dg;
+}
+
+// one thread per node of the colour: t -> (ih, j, k), i = 2 ih + parity
+template <typename T>
+__global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, float rd1) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int row = qdiv(t, hx, rhx);
+    int ih = t - row * hx;
+    int k = qdiv(row, L.d[1], rd1);
+    int j = row - k * L.d[1];
+    if (k >= L.d[2]) return;
+    int i = 2 * ih + ((j + k + color) & 1);
+    if (i >= L.d[0]) return;
+    mac_smooth_at(L, P, i, j, k);
 }
 
 // interior node (no wall, no wrap): the 20 non-zero neighbours in the order
@@ -101,23 +138,35 @@ __device__ __forceinline__ T fem_off_interior(const T *x, int c, int D0, int D01
     return s;
 }
 
-// four colours by (j,k) parity: one colour is a set of whole x-rows
+// x = (b - s) / dg, evaluated as a product with the exact reciprocal of the
+// power-of-two diagonal
 template <typename T>
-__global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = 2 * blockIdx.y + (color & 1);
-    int k = 2 * blockIdx.z + ((color >> 1) & 1);
+__device__ __forceinline__ void fem_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
     int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
-    if (i >= D0 || j >= D1 || k >= D2) return;
     int c = i + D0 * (j + D1 * k);
-    T s, dg;
+    T s, dg, rdg;
     if (i >= 1 && i <= D0 - 2 && j >= 1 && j <= D1 - 2 && k >= 1 && k <= D2 - 2) {
         s = fem_off_interior(L.x, c, D0, D0 * D1);
-        dg = T(32);
+        rdg = T(0.03125);
     } else {
-        s = fem_off(L, P, L.x, i, j, k, dg);
+        s = fem_off(L, P, L.x, i, j, k, dg, rdg);
     }
-    L.x[c] = (L.b[c] - s) / dg;
+    L.x[c] = (L.b[c] - s) * rdg;
+}
+
+// four colours by (j,k) parity: one colour is a set of whole x-rows; one
+// thread per node of the colour, t -> (i, jj, kk)
+template <typename T>
+__global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, int hy, float rd0, float rhy) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int row = qdiv(t, L.d[0], rd0);
+    int i = t - row * L.d[0];
+    int kk = qdiv(row, hy, rhy);
+    int jj = row - kk * hy;
+    int j = 2 * jj + (color & 1);
+    int k = 2 * kk + ((color >> 1) & 1);
+    if (j >= L.d[1] || k >= L.d[2]) return;
+    fem_smooth_at(L, P, i, j, k);
 }
 
 template <typename T>
@@ -131,14 +180,19 @@ __device__ __forceinline__ T apply_off(const MGLevel<T> &L, const Prd &P, int fe
 }
 
 template <typename T>
-__global__ void k_resid(MGLevel<T> L, Prd P, int fem) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= L.d[0]) return;
+__device__ __forceinline__ void resid_at(const MGLevel<T> &L, const Prd &P, int fem, int i, int j, int k) {
     int c = i + L.d[0] * (j + L.d[1] * k);
     T dg;
     T s = apply_off(L, P, fem, i, j, k, c, dg);
     L.r[c] = L.b[c] - (s + dg * L.x[c]);
+}
+
+template <typename T>
+__global__ void k_resid(MGLevel<T> L, Prd P, int fem) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.d[0]) return;
+    resid_at(L, P, fem, i, j, k);
 }
 
 // FP64 residual and its max norm: grid-stride blocks of 256, one atomic per
@@ -148,8 +202,12 @@ __global__ void k_resid64_max(MGLevel<double> L, Prd P, int fem, unsigned long l
     __shared__ unsigned long long sh[4];
     unsigned long long v = 0;
     int D0 = L.d[0], D1 = L.d[1];
+    float rd0 = 1.0f / D0, rd1 = 1.0f / D1;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < L.nn; c += gridDim.x * blockDim.x) {
-        int i = c % D0, j = (c / D0) % D1, k = c / (D0 * D1);
+        int row = qdiv(c, D0, rd0);
+        int i = c - row * D0;
+        int k = qdiv(row, D1, rd1);
+        int j = row - k * D1;
         double dg;
         double s = apply_off(L, P, fem, i, j, k, c, dg);
         double r = L.b[c] - (s + dg * L.x[c]);
@@ -171,10 +229,8 @@ __global__ void k_resid64_max(MGLevel<double> L, Prd P, int fem, unsigned long l
 }
 
 template <typename T>
-__global__ void k_restrict(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
-    int I = blockIdx.x * blockDim.x + threadIdx.x;
-    int J = blockIdx.y, K = blockIdx.z;
-    if (I >= C.d[0]) return;
+__device__ __forceinline__ void restrict_at(const MGLevel<T> &F, const MGLevel<T> &C, const Prd &P, int fem,
+                                            int I, int J, int K) {
     T s = 0;
     if (!fem) {
         for (int c = 0; c < 2; c++)
@@ -205,6 +261,14 @@ __global__ void k_restrict(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
 }
 
 template <typename T>
+__global__ void k_restrict(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    int J = blockIdx.y, K = blockIdx.z;
+    if (I >= C.d[0]) return;
+    restrict_at(F, C, P, fem, I, J, K);
+}
+
+template <typename T>
 __device__ __forceinline__ void cc_w(int f, int D, int prd, int &i0, int &i1, T &w0, T &w1) {
     int I = f >> 1, a = f & 1;
     int o = a ? I + 1 : I - 1;
@@ -222,10 +286,8 @@ __device__ __forceinline__ void vx_w(int f, int D, int prd, int &i0, int &i1, T 
 }
 
 template <typename T>
-__global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= F.d[0]) return;
+__device__ __forceinline__ void prolong_at(const MGLevel<T> &F, const MGLevel<T> &C, const Prd &P, int fem,
+                                           int i, int j, int k) {
     int i0, i1, j0, j1, k0, k1;
     T wi0, wi1, wj0, wj1, wk0, wk1;
     if (fem) {
@@ -244,6 +306,136 @@ __global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
           wk1 * (wj0 * (wi0 * E(i0, j0, k1) + wi1 * E(i1, j0, k1)) + wj1 * (wi0 * E(i0, j1, k1) + wi1 * E(i1, j1, k1)));
 #undef E
     F.x[i + F.d[0] * (j + F.d[1] * k)] += v;
+}
+
+template <typename T>
+__global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y, k = blockIdx.z;
+    if (i >= F.d[0]) return;
+    prolong_at(F, C, P, fem, i, j, k);
+}
+
+// ---- the lower part of a V-cycle in one workgroup --------------------------
+// Every level from the first one small enough down to the coarsest is
+// processed by a single workgroup: the same per-node operations as the
+// per-level kernels, in the same colour order, with workgroup barriers in
+// place of kernel boundaries, so the result is identical and the launch cost
+// of the small levels is paid once per V-cycle.
+// (i, j, k) of a flat index e of dims (a, b, .) with float reciprocals ra, rb
+__device__ __forceinline__ void unflat(int e, int a, int b, float ra, float rb, int &i, int &j, int &k) {
+    int row = qdiv(e, a, ra);
+    i = e - row * a;
+    k = qdiv(row, b, rb);
+    j = row - k * b;
+}
+
+template <typename T>
+__device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
+    int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
+    if (fem) {
+        int hy = (D1 + 1) / 2, hz = (D2 + 1) / 2, n = D0 * hy * hz;
+        float ra = 1.0f / D0, rb = 1.0f / hy;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            int i, jj, kk;
+            unflat(t, D0, hy, ra, rb, i, jj, kk);
+            int j = 2 * jj + (col & 1), k = 2 * kk + ((col >> 1) & 1);
+            if (j < D1 && k < D2) fem_smooth_at(L, P, i, j, k);
+        }
+    } else {
+        int hx = (D0 + 1) / 2, n = hx * D1 * D2;
+        float ra = 1.0f / hx, rb = 1.0f / D1;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            int ih, j, k;
+            unflat(t, hx, D1, ra, rb, ih, j, k);
+            int i = 2 * ih + ((j + k + col) & 1);
+            if (i < D0) mac_smooth_at(L, P, i, j, k);
+        }
+    }
+    __syncthreads();
+}
+
+// lds != 0: every tail level lives in LDS (x, b, r of each level packed in
+// that order); the finest tail level's x and b are loaded from HBM on entry
+// and its x stored back on exit.  lds == 0: the levels stay in HBM.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int vdown, int vup, int csweeps,
+                                               int lds) {
+    extern __shared__ unsigned char smem[];
+    __shared__ MGLevel<T> lv[MG_TAIL_MAXLEV];
+    int ncol = fem ? 4 : 2;
+    int last = t.n - 1;
+    if (threadIdx.x == 0) {
+        T *p = (T *)smem;
+        for (int l = 0; l < t.n; l++) {
+            lv[l] = t.L[l];
+            if (lds) {
+                int nn = t.L[l].nn;
+                lv[l].x = p; lv[l].b = p + nn; lv[l].r = p + 2 * nn;
+                p += 3 * nn;
+            }
+        }
+    }
+    __syncthreads();
+    if (lds) {
+        for (int e = threadIdx.x; e < lv[0].nn; e += blockDim.x) {
+            lv[0].x[e] = t.L[0].x[e];
+            lv[0].b[e] = t.L[0].b[e];
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < last; l++) {
+        const MGLevel<T> L = lv[l];
+        const MGLevel<T> C = lv[l + 1];
+        for (int sw = 0; sw < vdown; sw++)
+            for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
+        float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
+        for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
+            int i, j, k;
+            unflat(e, L.d[0], L.d[1], ra, rb, i, j, k);
+            resid_at(L, P, fem, i, j, k);
+        }
+        __syncthreads();
+        float rc0 = 1.0f / C.d[0], rc1 = 1.0f / C.d[1];
+        for (int e = threadIdx.x; e < C.nn; e += blockDim.x) {
+            int I, J, K;
+            unflat(e, C.d[0], C.d[1], rc0, rc1, I, J, K);
+            restrict_at(L, C, P, fem, I, J, K);
+        }
+        __syncthreads();
+    }
+    {
+        const MGLevel<T> W = lv[last];
+        float ra = 1.0f / W.d[0], rb = 1.0f / W.d[1];
+        for (int sw = 0; sw < csweeps; sw++)
+            for (int c = 0; c < ncol; c++) {
+                for (int e = threadIdx.x; e < W.nn; e += blockDim.x) {
+                    int i, j, k;
+                    unflat(e, W.d[0], W.d[1], ra, rb, i, j, k);
+                    int my = fem ? ((j & 1) + 2 * (k & 1)) : ((i + j + k) & 1);
+                    if (my != c) continue;
+                    T dg;
+                    T s = fem ? fem_off(W, P, W.x, i, j, k, dg) : mac_off(W, P, W.x, i, j, k, dg);
+                    W.x[e] = (W.b[e] - s) / dg;
+                }
+                __syncthreads();
+            }
+    }
+    for (int l = last - 1; l >= 0; l--) {
+        const MGLevel<T> L = lv[l];
+        const MGLevel<T> C = lv[l + 1];
+        float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
+        for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
+            int i, j, k;
+            unflat(e, L.d[0], L.d[1], ra, rb, i, j, k);
+            prolong_at(L, C, P, fem, i, j, k);
+        }
+        __syncthreads();
+        for (int sw = 0; sw < vup; sw++)
+            for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
+    }
+    if (lds)
+        for (int e = threadIdx.x; e < lv[0].nn; e += blockDim.x) t.L[0].x[e] = lv[0].x[e];
 }
 
 // coarsest level: coloured Gauss-Seidel sweeps inside one workgroup, on an
@@ -297,13 +489,13 @@ template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color) {
     Prd P = mkprd(prd);
     if (!fem) {
-        int hx = (L.d[0] + 1) / 2;
-        dim3 gr((hx + 63) / 64, L.d[1], L.d[2]);
-        hipLaunchKernelGGL(k_mac_smooth<T>, gr, dim3(64), 0, s, L, P, color);
+        int hx = (L.d[0] + 1) / 2, n = hx * L.d[1] * L.d[2];
+        hipLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, hx, 1.0f / hx,
+                           1.0f / L.d[1]);
     } else {
-        int hy = (L.d[1] + 1) / 2, hz = (L.d[2] + 1) / 2;
-        dim3 gr((L.d[0] + 63) / 64, hy, hz);
-        hipLaunchKernelGGL(k_fem_smooth<T>, gr, dim3(64), 0, s, L, P, color);
+        int hy = (L.d[1] + 1) / 2, hz = (L.d[2] + 1) / 2, n = L.d[0] * hy * hz;
+        hipLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, hy,
+                           1.0f / L.d[0], 1.0f / hy);
     }
 }
 template <typename T>
@@ -333,6 +525,34 @@ void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd
     int use = lds <= 48 * 1024;
     hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(256), use ? lds : 0, s, L, mkprd(prd), fem, sweeps, use);
 }
+template <typename T>
+size_t mg_tail_lds_bytes(const MGLevel<T> *L, int n) {
+    size_t b = 0;
+    for (int l = 0; l < n; l++) b += 3 * sizeof(T) * (size_t)L[l].nn;
+    return b;
+}
+
+template <typename T>
+int mg_tail_enable_lds(size_t bytes) {
+    int dev = 0, maxb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    if (bytes + sizeof(MGLevel<T>) * MG_TAIL_MAXLEV > (size_t)maxb) return 0;
+    // raising the dynamic limit is required on some runtimes and a no-op on others
+    (void)hipFuncSetAttribute((const void *)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipGetLastError();
+    return 1;
+}
+
+template <typename T>
+void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const int prd[3], int vdown, int vup,
+                    int csweeps, int lds) {
+    MGTail<T> t;
+    t.n = n;
+    for (int l = 0; l < n; l++) t.L[l] = L[l];
+    size_t bytes = lds ? mg_tail_lds_bytes(L, n) : 0;
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), bytes, s, t, mkprd(prd), fem, vdown, vup, csweeps, lds);
+}
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n) {
     hipLaunchKernelGGL(k_cast_d2f, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);
 }
@@ -348,7 +568,10 @@ void launch_zero_f(hipStream_t s, float *a, int n) {
     template void launch_mg_resid<T>(hipStream_t, int, const MGLevel<T> &, const int *);                    \
     template void launch_mg_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *); \
     template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *);  \
-    template void launch_mg_coarse<T>(hipStream_t, int, const MGLevel<T> &, const int *, int);
+    template void launch_mg_coarse<T>(hipStream_t, int, const MGLevel<T> &, const int *, int);            \
+    template void launch_mg_tail<T>(hipStream_t, int, const MGLevel<T> *, int, const int *, int, int, int, int); \
+    template size_t mg_tail_lds_bytes<T>(const MGLevel<T> *, int);                                          \
+    template int mg_tail_enable_lds<T>(size_t);
 INST(double)
 INST(float)
 

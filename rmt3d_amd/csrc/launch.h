@@ -21,7 +21,9 @@ struct InitSpec {
 };
 
 // ---- k_fluid.hip ----
-void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M);
+// list: Ng ints of scratch for the solid cells, count: one int
+void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                   int *list, int *count);
 void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    int *status);
 void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status);
@@ -53,7 +55,17 @@ struct MGLevel {
     T *x, *b, *r;
 };
 
-// smooth one colour (MAC: 2 colours red-black, FEM: 8 colours 2x2x2)
+// the levels below a size threshold, processed by one workgroup
+constexpr int MG_TAIL_MAXLEV = 12;
+constexpr int MG_TAIL_NN = 6144;       // largest level handed to the tail kernel
+constexpr size_t MG_TAIL_LDS_MAX = 160 * 1024 - 1024;   // dynamic LDS of the tail (gfx950: 160 KiB per workgroup)
+template <typename T>
+struct MGTail {
+    MGLevel<T> L[MG_TAIL_MAXLEV];
+    int n;
+};
+
+// smooth one colour (MAC: red-black, FEM: four colours by (j,k) parity)
 template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color);
 template <typename T>
@@ -64,6 +76,16 @@ template <typename T>
 void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
 template <typename T>
 void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int sweeps);
+// the rest of a V-cycle from L[0] (finest of the tail) to L[n-1] (coarsest)
+template <typename T>
+void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const int prd[3], int vdown, int vup,
+                    int csweeps, int lds);
+// LDS bytes the tail needs for levels L[0..n-1]; enable returns 1 when the
+// kernel may use that much dynamic LDS on the current device
+template <typename T>
+size_t mg_tail_lds_bytes(const MGLevel<T> *L, int n);
+template <typename T>
+int mg_tail_enable_lds(size_t bytes);
 // FP64 residual r64 = b - A x64 with max|r| accumulated into *maxbits
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits);
