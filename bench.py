@@ -22,6 +22,11 @@ The JSON line carries:
                kernel runs on
   cpu_baseline the FP64 oracle (C restatement of the reference) on a bounded
                sample of the same workload on this host
+  config.serving  the serving workload: Poisson arrivals (--serve-rate per
+               second, --serve-requests requests) of independent solves, grid
+               from {32,48,64,96,128}^3, 1-8 spheres, lid speed 0.5-2, 200
+               steps each; p50/p99 time-to-completion and aggregate
+               cell-steps/s (rmt3d_amd/serve.py)
 """
 import argparse
 import json
@@ -46,6 +51,9 @@ def parse():
                     help="both: time the FP64 path, then the mixed path; value is the mixed path")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=16)
     return ap.parse_args()
 
 
@@ -191,6 +199,11 @@ def main():
         "config": config,
         "roofline": g["roofline"],
     }
+    if not args.no_serve:
+        from rmt3d_amd import serve
+        config["serving"] = serve.poisson_run(rate=args.serve_rate, n_requests=args.serve_requests)
+        print("bench: serving p50 %.2f s p99 %.2f s" % (config["serving"]["p50_s"], config["serving"]["p99_s"]),
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu(args)
     print(json.dumps(line), flush=True)

@@ -199,6 +199,56 @@ int rmt3d_profile_read(rmt3d_ctx *ctx, int which, double *ms, int *launches);
 /* hipDeviceSynchronize on the context's device. */
 int rmt3d_device_synchronize(rmt3d_ctx *ctx);
 
+/* ---------------------------------------------------------------------- */
+/* Serving: many independent solves on one GPU.  No counterpart in the
+ * reference, which runs one solve per process (run_sim.cc:39-165); a request
+ * is exactly that run (create, initialize, prepare_run, `steps` steps) on its
+ * own context and stream.  Requests are admitted first-in first-out when
+ * their estimated device footprint fits the budget beside the running ones;
+ * up to max_active run concurrently, one worker thread each, and their
+ * kernels overlap on the device.  A request's stage order is the single-solve
+ * order, so its result is the same alone and in the mix. */
+typedef struct rmt3d_server rmt3d_server;
+
+#define RMT3D_REQ_QUEUED  0
+#define RMT3D_REQ_RUNNING 1
+#define RMT3D_REQ_DONE    2
+#define RMT3D_REQ_FAILED  3
+
+typedef struct rmt3d_request_info {
+    int       status;            /* RMT3D_REQ_*                                  */
+    int       rc;                /* 0, or the step / library code that ended it  */
+    int       steps_done;
+    long long cell_steps;        /* m*n*o per completed step                     */
+    double    t_submit, t_admit, t_done;  /* seconds since server creation; -1 = not yet */
+    double    device_bytes;      /* held while running (estimate while queued)   */
+    double    sim_time;          /* simulated time reached                       */
+} rmt3d_request_info;
+
+/* Device bytes rmt3d_create would allocate for this configuration. */
+int rmt3d_estimate_bytes(const rmt3d_config *cfg, const rmt3d_object *objs,
+                         int precision, double *bytes);
+/* max_active in 1..16 (worker threads, one stream each). */
+int rmt3d_server_create(double hbm_budget_bytes, int max_active, rmt3d_server **out);
+/* Queues a request; RMT3D_ERR_ARG if its estimate exceeds the whole budget. */
+int rmt3d_server_submit(rmt3d_server *srv, const rmt3d_config *cfg,
+                        const rmt3d_object *objs, int precision, int steps, int *req);
+/* Blocks until the request is done or failed (timeout_s < 0: no limit);
+ * returns 0 when it ended, 1 on timeout.  Do not release a request that
+ * another thread waits on. */
+int rmt3d_server_wait(rmt3d_server *srv, int req, double timeout_s, rmt3d_request_info *info);
+int rmt3d_server_info(rmt3d_server *srv, int req, rmt3d_request_info *info);
+const char *rmt3d_server_error(rmt3d_server *srv, int req);
+/* Final u, v, w, p, rm_c of an ended request (Ng entries each, padded order). */
+int rmt3d_server_result(rmt3d_server *srv, int req, rmt3d_state *st);
+/* Frees an ended request's host copy of its fields. */
+int rmt3d_server_release(rmt3d_server *srv, int req);
+/* Peak device bytes held and peak concurrent requests so far; *now is the
+ * server clock in seconds. */
+int rmt3d_server_stats(rmt3d_server *srv, double *peak_bytes, int *peak_active, double *now);
+/* Cancels queued requests, waits for running ones, frees everything. */
+void rmt3d_server_destroy(rmt3d_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

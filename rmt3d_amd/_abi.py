@@ -132,3 +132,21 @@ def finalize_config(c):
     dh = min((c.xyz_bounds[2 * d + 1] - c.xyz_bounds[2 * d]) / c.sys_size[d] for d in range(3))
     c.wall_dist = max(dh, c.wall_dist)
     return c
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class RequestInfo(C.Structure):
+    """rmt3d_request_info (include/rmt3d.h)."""
+    _fields_ = [
+        ("status", C.c_int),
+        ("rc", C.c_int),
+        ("steps_done", C.c_int),
+        ("cell_steps", C.c_longlong),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+        ("device_bytes", C.c_double),
+        ("sim_time", C.c_double),
+    ]

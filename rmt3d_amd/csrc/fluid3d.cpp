@@ -42,24 +42,33 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         MGLevel<double> &l = L[nlev];
         for (int a = 0; a < 3; a++) l.d[a] = cur[a];
         l.nn = cur[0] * cur[1] * cur[2];
-        hipMalloc(&l.x, sizeof(double) * l.nn);
-        hipMalloc(&l.b, sizeof(double) * l.nn);
-        hipMalloc(&l.r, sizeof(double) * l.nn);
-        hipMemset(l.x, 0, sizeof(double) * l.nn);
-        hipMemset(l.b, 0, sizeof(double) * l.nn);
-        hipMemset(l.r, 0, sizeof(double) * l.nn);
+        l.x = l.b = l.r = nullptr;
+        bool ok_alloc = hipMalloc(&l.x, sizeof(double) * l.nn) == hipSuccess &&
+                        hipMalloc(&l.b, sizeof(double) * l.nn) == hipSuccess &&
+                        hipMalloc(&l.r, sizeof(double) * l.nn) == hipSuccess;
+        bytes += 3 * sizeof(double) * (size_t)l.nn;
+        if (ok_alloc) {
+            hipMemset(l.x, 0, sizeof(double) * l.nn);
+            hipMemset(l.b, 0, sizeof(double) * l.nn);
+            hipMemset(l.r, 0, sizeof(double) * l.nn);
+        }
         if (mixed) {
             MGLevel<float> &f = Lf[nlev];
             for (int a = 0; a < 3; a++) f.d[a] = cur[a];
             f.nn = l.nn;
-            hipMalloc(&f.x, sizeof(float) * f.nn);
-            hipMalloc(&f.b, sizeof(float) * f.nn);
-            hipMalloc(&f.r, sizeof(float) * f.nn);
-            hipMemset(f.x, 0, sizeof(float) * f.nn);
-            hipMemset(f.b, 0, sizeof(float) * f.nn);
-            hipMemset(f.r, 0, sizeof(float) * f.nn);
+            f.x = f.b = f.r = nullptr;
+            ok_alloc = ok_alloc && hipMalloc(&f.x, sizeof(float) * f.nn) == hipSuccess &&
+                       hipMalloc(&f.b, sizeof(float) * f.nn) == hipSuccess &&
+                       hipMalloc(&f.r, sizeof(float) * f.nn) == hipSuccess;
+            bytes += 3 * sizeof(float) * (size_t)f.nn;
+            if (ok_alloc) {
+                hipMemset(f.x, 0, sizeof(float) * f.nn);
+                hipMemset(f.b, 0, sizeof(float) * f.nn);
+                hipMemset(f.r, 0, sizeof(float) * f.nn);
+            }
         }
         nlev++;
+        if (!ok_alloc) { alloc_failed = true; break; }
         bool ok = nlev < 16;
         for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
         if (!ok) break;
@@ -83,8 +92,14 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
 
 void MGSolver::release() {
     for (int l = 0; l < nlev; l++) {
-        hipFree(L[l].x); hipFree(L[l].b); hipFree(L[l].r);
-        if (mixed) { hipFree(Lf[l].x); hipFree(Lf[l].b); hipFree(Lf[l].r); }
+        if (L[l].x) hipFree(L[l].x);
+        if (L[l].b) hipFree(L[l].b);
+        if (L[l].r) hipFree(L[l].r);
+        if (mixed) {
+            if (Lf[l].x) hipFree(Lf[l].x);
+            if (Lf[l].b) hipFree(Lf[l].b);
+            if (Lf[l].r) hipFree(Lf[l].r);
+        }
     }
     nlev = 0;
     if (d_max) hipFree(d_max);
@@ -201,15 +216,42 @@ double MGSolver::run(hipStream_t s) {
     return res;
 }
 
+// Device bytes a context of this configuration allocates, from the same sizes
+// as Sim::create (used for admission before anything is allocated).
+double Sim::estimate_bytes(const rmt3d_config &c, const rmt3d_object *o, int prec) {
+    int n_obj = (c.sim_type == RMT3D_SIM_FTEST) ? 0 : c.n_obj;
+    double m = c.sys_size[0], n = c.sys_size[1], k = c.sys_size[2];
+    double Ng = (m + 4) * (n + 4) * (k + 4);
+    double per_cell = 8.0 * (36 + 39 + 3) + 4.0 * (1 + 2 + 1 + 1 + 4) + 2;
+    double dh = std::min((c.xyz_bounds[1] - c.xyz_bounds[0]) / m,
+                         std::min((c.xyz_bounds[3] - c.xyz_bounds[2]) / n, (c.xyz_bounds[5] - c.xyz_bounds[4]) / k));
+    double cap = 65536;
+    for (int i = 0; i < n_obj; i++) {
+        double R0 = o[i].basic[3] + (c.nlayers + 4) * dh;
+        cap += 2.0 * (4.0 / 3.0) * M_PI * R0 * R0 * R0 / (dh * dh * dh);
+    }
+    cap = std::min(cap, Ng * std::max(1, n_obj));
+    double per_entry = 2 * (8.0 + 8.0 * 39) + 60;
+    double fm = c.prd[0] ? m : m + 1, fn = c.prd[1] ? n : n + 1, fo = c.prd[2] ? k : k + 1;
+    double mg_words = (m * n * k + fm * fn * fo) * 8.0 / 7.0;
+    double mg = 3 * 8 * mg_words + (prec == RMT3D_PREC_MIXED ? 3 * 4 * mg_words : 0);
+    return Ng * per_cell + cap * per_entry + mg + 16.0 * fm * fn * fo + (1 << 20);
+}
+
 // ---------------------------------------------------------------------------
 // construction (fluid_3d.cc:12-77, sim_manager.cc:19-81, sim_manager.hh:363-444)
 // ---------------------------------------------------------------------------
 void *Sim::dalloc(size_t bytes) {
     void *p = nullptr;
     if (bytes == 0) bytes = 8;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        dalloc_failed = true;
+        return nullptr;
+    }
     hipMemset(p, 0, bytes);
     allocs.push_back(p);
+    dev_bytes += bytes;
     return p;
 }
 
@@ -436,7 +478,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     }
     for (int a = 0; a < 18; a++) R.fx[a] = (double *)dalloc(sizeof(double) * Ng);
     for (int a = 0; a < 9; a++) R.gphi[a] = (double *)dalloc(sizeof(double) * Ng);
-    if (!R.gphi[8]) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    if (dalloc_failed) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
 
     // band capacity: twice the padded shells of all objects, at least 64k
     double cap_est = 65536;
@@ -480,7 +522,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
         d_wpow = (double *)dalloc(sizeof(wp));
         HCHK(hipMemcpy(d_wpow, wp, sizeof(wp), hipMemcpyHostToDevice));
     }
-    if (!W.err || !d_rhs_tmp || !d_wpow) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    if (dalloc_failed) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
     HCHK(hipHostMalloc((void **)&h_status, sizeof(int) * 4, hipHostMallocDefault));
     HCHK(hipHostMalloc((void **)&h_int, sizeof(int) * 16, hipHostMallocDefault));
     HCHK(hipHostMalloc((void **)&h_dbl, sizeof(double) * 16, hipHostMallocDefault));
@@ -493,13 +535,19 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     bool mixed = (prec == RMT3D_PREC_MIXED);
     mac.setup(0, dm, g.prd, mixed);
     fem.setup(1, df, g.prd, mixed);
+    if (dalloc_failed) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    if (mac.alloc_failed || fem.alloc_failed) { err = "hipMalloc failed (multigrid levels)"; return RMT3D_ERR_NOMEM; }
+    dev_bytes += mac.bytes + fem.bytes;
     if (mixed) {
         // defect correction stops on the FP64 residual relative to the rhs
         // scale; the rule itself is mg_prob::run's
         mac.tol_override = -1;
         fem.tol_override = -1;
     }
-    HCHK(hipDeviceSynchronize());
+    // the set-up copies and fills ran on the null stream; the context's own
+    // stream is non-blocking, so wait for both (not for other contexts)
+    HCHK(hipStreamSynchronize(nullptr));
+    HCHK(hipStreamSynchronize(stream));
     return 0;
 }
 

@@ -49,6 +49,8 @@ struct MGSolver {
     KProf *prof = nullptr;     // times the finest-level smoother launches
     KProf *vprof = nullptr;    // times whole V-cycles (one defect-correction step when mixed)
     unsigned long long *d_max = nullptr;
+    size_t bytes = 0;                      // device bytes of the level arrays
+    bool alloc_failed = false;
     unsigned long long *h_max = nullptr;   // pinned
     double tol_override = 0;               // > 0 replaces max_norm_tol
 
@@ -114,7 +116,11 @@ class Sim {
     bool timing = false;
     hipEvent_t tev[8];
 
+    size_t dev_bytes = 0;       // device bytes allocated by this context
+    bool dalloc_failed = false;
+
     ~Sim();
+    static double estimate_bytes(const rmt3d_config &c, const rmt3d_object *o, int prec);
     int create(const rmt3d_config *c, const rmt3d_object *o, int prec);
     int initialize();
     int init_iter(int init_err);

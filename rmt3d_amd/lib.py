@@ -21,6 +21,9 @@ EXPORTS = [
     "rmt3d_get_dt", "rmt3d_get_time", "rmt3d_set_time", "rmt3d_upload_state", "rmt3d_download_state",
     "rmt3d_band_count", "rmt3d_mg_solve", "rmt3d_mg_dims", "rmt3d_diag", "rmt3d_stage_times",
     "rmt3d_stream", "rmt3d_profile", "rmt3d_profile_read", "rmt3d_device_synchronize",
+    "rmt3d_estimate_bytes", "rmt3d_server_create", "rmt3d_server_submit", "rmt3d_server_wait",
+    "rmt3d_server_info", "rmt3d_server_error", "rmt3d_server_result", "rmt3d_server_release",
+    "rmt3d_server_stats", "rmt3d_server_destroy",
 ]
 
 
@@ -66,6 +69,18 @@ def lib():
         L.rmt3d_profile.argtypes = [P, C.c_int]
         L.rmt3d_profile_read.argtypes = [P, C.c_int, dp, ip]
         L.rmt3d_device_synchronize.argtypes = [P]
+        L.rmt3d_estimate_bytes.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_int, dp]
+        L.rmt3d_server_create.argtypes = [D, C.c_int, C.POINTER(P)]
+        L.rmt3d_server_submit.argtypes = [P, C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_int, C.c_int, ip]
+        L.rmt3d_server_wait.argtypes = [P, C.c_int, D, C.POINTER(_abi.RequestInfo)]
+        L.rmt3d_server_info.argtypes = [P, C.c_int, C.POINTER(_abi.RequestInfo)]
+        L.rmt3d_server_error.argtypes = [P, C.c_int]
+        L.rmt3d_server_error.restype = C.c_char_p
+        L.rmt3d_server_result.argtypes = [P, C.c_int, C.POINTER(_abi.State)]
+        L.rmt3d_server_release.argtypes = [P, C.c_int]
+        L.rmt3d_server_stats.argtypes = [P, dp, ip, dp]
+        L.rmt3d_server_destroy.argtypes = [P]
+        L.rmt3d_server_destroy.restype = None
         _lib = L
     return _lib
 
