@@ -18,8 +18,11 @@ sit in config.
 The JSON line carries:
   value        whole-job cell-steps/s of the measured precision path
   roofline     the dominant kernel's algorithmic HBM bytes per launch over its
-               average launch time, timed live with HIP events on the stream the
-               kernel runs on
+               average launch time, timed with HIP events on the stream the
+               kernel runs on, over --kernel-steps steps run right after the
+               timed region with per-launch events (the timed region replays
+               the multigrid V-cycles as captured graphs, where single
+               launches cannot be bracketed)
   cpu_baseline the FP64 oracle (C restatement of the reference) on a bounded
                sample of the same workload on this host
   config.serving  the serving workload: Poisson arrivals (--serve-rate per
@@ -49,6 +52,8 @@ def parse():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="both: time the FP64 path, then the mixed path; value is the mixed path")
+    ap.add_argument("--kernel-steps", type=int, default=3,
+                    help="steps of the per-kernel event window after the timed region")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -83,7 +88,9 @@ def run_gpu(args, precision):
         if rc:
             raise RuntimeError("step returned %d" % rc)
     sim.synchronize()
-    sim.profile(True)
+    # timed region: stage buckets and whole V-cycles are event-timed; the
+    # V-cycles replay their captured graphs
+    sim.profile(1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         rc = sim.step()
@@ -93,7 +100,19 @@ def run_gpu(args, precision):
     t1 = time.perf_counter()
     prof = {k: sim.profile_read(i) for i, k in enumerate(
         ("fem_smooth", "mac_smooth", "stress", "advect", "fem_vcycle", "mac_vcycle"))}
-    sim.profile(False)
+    stages = {k: round(v / args.steps, 3) for k, v in sim.stage_times().items()}
+    # per-kernel window right after: the same steps with every launch of the
+    # reported kernel bracketed by events (V-cycles launched one kernel at a
+    # time, which leaves each kernel's own duration unchanged)
+    sim.profile(2)
+    for _ in range(args.kernel_steps):
+        rc = sim.step()
+        if rc:
+            raise RuntimeError("step returned %d" % rc)
+    sim.synchronize()
+    for i, k in enumerate(("fem_smooth", "mac_smooth", "stress", "advect")):
+        prof[k] = sim.profile_read(i)
+    sim.profile(0)
     elapsed = t1 - t0
     m, n, o = cfg.sys_size
     cells = m * n * o
@@ -119,7 +138,7 @@ def run_gpu(args, precision):
     fields = sim.download(band=False)
     res = {"value": cells * args.steps / elapsed, "elapsed": elapsed, "roofline": roof, "vcycle": vc,
            "mac_cycles": sim.diag(1), "fem_cycles": sim.diag(2), "dt": sim.dt,
-           "stress_ms_per_step": prof["stress"][0] / args.steps,
+           "stress_ms_per_step": prof["stress"][0] / max(args.kernel_steps, 1), "stage_ms_per_step": stages,
            "fields": {k: fields[k] for k in ("u", "v", "w", "p")}}
     sim.close()
     return res
@@ -171,7 +190,8 @@ def main():
               "lambda": 1.3, "G": 1.0, "mu": 0.01, "nlayers": 4, "precision": head,
               "dt": g["dt"], "mac_vcycles_last_step": g["mac_cycles"],
               "fem_vcycles_last_step": g["fem_cycles"], "vcycle": g["vcycle"],
-              "stress_ms_per_step": round(g["stress_ms_per_step"], 3)}
+              "stress_ms_per_step": round(g["stress_ms_per_step"], 3),
+              "stage_ms_per_step": g["stage_ms_per_step"]}
     if len(runs) == 2:
         f64, fmx = runs["fp64"]["fields"], runs["mixed"]["fields"]
         import numpy as np
@@ -181,7 +201,7 @@ def main():
         config["fp64_ms_per_step"] = round(1e3 * runs["fp64"]["elapsed"] / args.steps, 4)
         config["mixed_speedup_vs_fp64"] = round(runs["mixed"]["value"] / runs["fp64"]["value"], 3)
         config["mixed_rel_l2_vs_fp64"] = {"vel": _rel(velmx, vel64), "p": _rel(fmx["p"], f64["p"]),
-                                          "steps": args.warmup + args.steps}
+                                          "steps": args.warmup + args.steps + args.kernel_steps}
         config["fp64_vcycle"] = runs["fp64"]["vcycle"]
     line = {
         "metric": METRIC,

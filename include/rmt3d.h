@@ -180,7 +180,8 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * 14/15 -> tail levels held in LDS (bit 0 FP64 hierarchy, bit 1 FP32). */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
-/* Per-stage device time accumulated by the step (ms), in the order of the
+/* Per-stage device time of the steps run while rmt3d_profile is enabled
+ * (ms, from event pairs on the context stream), in the order of the
  * reference's timer buckets "Extrapolation, Pressure Poisson, Communication,
  * Stresses, MAC" (fluid_3d.cc:59) followed by "Advection".  n <= 6. */
 int rmt3d_stage_times(rmt3d_ctx *ctx, double *ms, int n);
@@ -188,9 +189,12 @@ int rmt3d_stage_times(rmt3d_ctx *ctx, double *ms, int n);
 /* Stream the context's kernels run on (a hipStream_t). */
 void *rmt3d_stream(rmt3d_ctx *ctx);
 
-/* Event timing of kernel classes on the context stream (measurement only;
- * no counterpart in the reference, whose timer buckets are wall clocks,
- * Utilities/timer.hh).  which: 0 finest-level FEM smoother colour pass,
+/* Event timing on the context stream (measurement only; no counterpart in
+ * the reference, whose timer buckets are wall clocks, Utilities/timer.hh).
+ * enable: 0 off; 1 stage buckets and whole V-cycles; 2 also the kernel
+ * classes 0-3 (the multigrid V-cycles then run as individual launches
+ * instead of replaying their captured graph).  Enabling resets the counts.
+ * which: 0 finest-level FEM smoother colour pass,
  * 1 finest-level MAC smoother colour pass, 2 compute_stress, 3 advection,
  * 4 one FEM V-cycle, 5 one MAC V-cycle (with the mixed path, one FP64
  * defect-correction step around an FP32 V-cycle). */

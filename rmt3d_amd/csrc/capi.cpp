@@ -115,9 +115,8 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out) {
     return RMT3D_ERR_ARG;
 }
 int rmt3d_stage_times(rmt3d_ctx *ctx, double *ms, int n) {
-    if (!ctx || !ms) return RMT3D_ERR_ARG;
-    for (int a = 0; a < n && a < 6; a++) ms[a] = ctx->sim.stage_ms[a];
-    return 0;
+    if (!ms) return RMT3D_ERR_ARG;
+    GUARD(ctx->sim.stage_times(ms, n));
 }
 void *rmt3d_stream(rmt3d_ctx *ctx) { return ctx ? (void *)ctx->sim.stream : nullptr; }
 int rmt3d_profile(rmt3d_ctx *ctx, int enable) { GUARD(ctx->sim.profile(enable)); }

@@ -1,7 +1,9 @@
 // Host orchestration of the MI355X FSI step (see fluid3d.h).
+#include <mutex>
 #include "fluid3d.h"
 #include <float.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <cstdio>
@@ -80,6 +82,10 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     while (tail_lev > 0 && L[tail_lev - 1].nn <= MG_TAIL_NN && nlev - (tail_lev - 1) <= MG_TAIL_MAXLEV &&
            mg_tail_lds_bytes(L + tail_lev - 1, nlev - tail_lev + 1) <= MG_TAIL_LDS_MAX)
         tail_lev--;
+    // RMT3D_NO_GRAPH=1 launches the V-cycles one kernel at a time (for
+    // tracing tools that do not follow graph launches)
+    const char *ng = getenv("RMT3D_NO_GRAPH");
+    no_graph = ng && ng[0] == '1';
     tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
     tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
     double eps = DBL_EPSILON, c = fem ? 32.0 : 6.0;
@@ -91,6 +97,8 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
 }
 
 void MGSolver::release() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
     for (int l = 0; l < nlev; l++) {
         if (L[l].x) hipFree(L[l].x);
         if (L[l].b) hipFree(L[l].b);
@@ -169,7 +177,7 @@ void MGSolver::vcycle32(hipStream_t s, int lev) {
 
 // one iteration: an FP64 V-cycle, or (mixed) one FP64 defect-correction step
 // preconditioned by an FP32 V-cycle: r = b - A x (FP64), A e ~= r (FP32), x += e
-void MGSolver::cycle(hipStream_t s) {
+void MGSolver::cycle_body(hipStream_t s) {
     if (!mixed) {
         vcycle64(s, 0);
         return;
@@ -179,6 +187,52 @@ void MGSolver::cycle(hipStream_t s) {
     launch_zero_f(s, Lf[0].x, Lf[0].nn);
     vcycle32(s, 0);
     launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
+}
+
+// The kernel sequence of one iteration and all its arguments are fixed for
+// the solver's life, so it is captured once into a HIP graph and replayed:
+// the device runs the sixty-odd dependent launches without a host round trip
+// between them.  Per-kernel event timing (profiling) runs the same sequence
+// eagerly.  Capture is thread-local, so other contexts' threads may allocate
+// meanwhile.
+void MGSolver::cycle(hipStream_t s) {
+    bool eager = graph_failed || no_graph || (prof && prof->enabled);
+    if (!eager && !gexec) {
+        // no other context allocates, frees or uses the null stream while
+        // this stream is being captured
+        std::lock_guard<std::mutex> lk(hip_setup_mutex());
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            graph_failed = true;
+        } else {
+            cycle_body(s);
+            hipError_t e1 = hipStreamEndCapture(s, &gr);
+            hipError_t e2 = (e1 == hipSuccess) ? hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0) : e1;
+            if (gr) hipGraphDestroy(gr);
+            if (e2 != hipSuccess) {
+                // leave no half-open capture behind, then run eagerly
+                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                    hipGraph_t g2 = nullptr;
+                    (void)hipStreamEndCapture(s, &g2);
+                    if (g2) hipGraphDestroy(g2);
+                }
+                (void)hipGetLastError();
+                gexec = nullptr;
+                graph_failed = true;
+            }
+        }
+        if (graph_failed) {
+            cycle_body(s);
+            return;
+        }
+    }
+    if (eager) {
+        cycle_body(s);
+        return;
+    }
+    hipGraphLaunch(gexec, s);
 }
 
 void MGSolver::timed_cycle(hipStream_t s) {
@@ -273,7 +327,10 @@ void Sim::alloc_band(Band &B, int cap) {
     B.n = (int *)dalloc(sizeof(int));
 }
 
-int Sim::profile(int enable) {
+// level 0 off; 1 stage buckets and whole V-cycles (graph replay unaffected);
+// 2 also per-kernel classes, which runs the V-cycles eagerly
+int Sim::profile(int level) {
+    if (level < 0 || level > 2) return RMT3D_ERR_ARG;
     for (int k = 0; k < NPROF; k++) {
         KProf &P = kprof[k];
         if (!P.ev) {
@@ -281,12 +338,22 @@ int Sim::profile(int enable) {
             for (int a = 0; a < 2 * KProf::CAP; a++) hipEventCreate(&P.ev[a]);
         }
         P.n = 0;
-        P.enabled = enable;
+        bool kernel_class = k < 4;
+        P.enabled = kernel_class ? (level >= 2) : (level >= 1);
     }
     fem.prof = &kprof[0];
     mac.prof = &kprof[1];
     fem.vprof = &kprof[4];
     mac.vprof = &kprof[5];
+    return 0;
+}
+
+int Sim::stage_times(double *ms, int n) {
+    for (int k = 0; k < n && k < NPROF - NPROF_KERNEL; k++) {
+        int cnt;
+        int rc = profile_read(NPROF_KERNEL + k, &ms[k], &cnt);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -306,6 +373,7 @@ int Sim::profile_read(int which, double *ms, int *launches) {
 }
 
 Sim::~Sim() {
+    std::lock_guard<std::mutex> lk(hip_setup_mutex());
     for (int k = 0; k < NPROF; k++)
         if (kprof[k].ev) {
             for (int a = 0; a < 2 * KProf::CAP; a++) hipEventDestroy(kprof[k].ev[a]);
@@ -317,12 +385,18 @@ Sim::~Sim() {
     if (h_status) hipHostFree(h_status);
     if (h_int) hipHostFree(h_int);
     if (h_dbl) hipHostFree(h_dbl);
-    if (timing)
-        for (int a = 0; a < 8; a++) hipEventDestroy(tev[a]);
     if (stream) hipStreamDestroy(stream);
 }
 
+// Serialises what must not overlap another context's stream capture:
+// allocation, freeing and null-stream set-up copies.
+std::mutex &hip_setup_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
+    std::lock_guard<std::mutex> lk(hip_setup_mutex());
     cfg = *c;
     precision = prec;
     int st = cfg.sim_type;
@@ -694,12 +768,10 @@ int Sim::initialize() {
 // ---------------------------------------------------------------------------
 // step stages
 // ---------------------------------------------------------------------------
-void Sim::mark(int k) {
-    if (timing) hipEventRecord(tev[2 * k], stream);
-}
-void Sim::mark_end(int k) {
-    (void)k;
-}
+// stage buckets of fluid_3d::watch (fluid_3d.cc:59): 0 extrapolation,
+// 1 pressure Poisson, 2 communication, 3 stresses, 4 MAC, 5 advection
+void Sim::mark(int k) { kprof[NPROF_KERNEL + k].begin(stream); }
+void Sim::mark_end(int k) { kprof[NPROF_KERNEL + k].end(stream); }
 
 int Sim::mac_solve() {
     launch_mac_rhs(stream, g, U, mac.L[0].b, g.dx / (2 * dt));
@@ -748,43 +820,72 @@ int Sim::step(int debug) {
     P.time = time;
     launch_zero_i32(stream, d_status, 1);
     Maps M = maps();
+    mark(3);
     kprof[2].begin(stream);
     launch_stress(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n);
     kprof[2].end(stream);
+    mark_end(3);
+    mark(5);
     kprof[3].begin(stream);
     launch_advect(stream, g, P, d_obj, U, M, d_status);
     kprof[3].end(stream);
+    mark_end(5);
     time += 0.5 * dt;
     P.time = time;
+    mark(2);
     fill_bc_cf();
     comm_fluid(ORC_FVEL | ORC_FX);
     launch_periodic_band_fx(stream, g, ext);
+    mark_end(2);
+    mark(5);
     launch_upwind(stream, g, U, M, d_status);
+    mark_end(5);
+    mark(2);
     fill_bc_cf();
+    mark_end(2);
     int e = check_status("step(godunov)");
     if (e) return e;
+    mark(4);
     e = mac_solve();
+    mark_end(4);
     if (e) return e;
+    mark(2);
     fill_bc_cf();
+    mark_end(2);
+    mark(0);
     int prob = update_reference_map();
+    mark_end(0);
     M = maps();
     if (prob) {
         launch_revert(stream, g, R, ext, tmp);
         hipStreamSynchronize(stream);
         return RMT3D_REFMAP_UPDATE_ERR;
     }
+    mark(3);
     launch_stress(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n);
+    mark_end(3);
+    mark(5);
     launch_ustar(stream, g, P, d_obj, U, M);
+    mark_end(5);
+    mark(2);
     fill_bc_cc();
     comm_fluid(ORC_DVEL);
+    mark_end(2);
     time += 0.5 * dt;
     P.time = time;
-    if (pp_solve()) return RMT3D_PROJECTION_ERR;
+    mark(1);
+    int pe = pp_solve();
+    mark_end(1);
+    if (pe) return RMT3D_PROJECTION_ERR;
+    mark(0);
     launch_full_update(stream, g, R, ext);
     launch_redraw(stream, g, P, d_obj, R, ext, W, cfg.nlayers, d_status);
     band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status);
+    mark_end(0);
+    mark(2);
     fill_bc_cc();
     comm_fluid(ORC_DVEL);
+    mark_end(2);
     e = check_status("step");
     if (e) return e;
     return RMT3D_NORMAL_EXIT;
@@ -854,44 +955,46 @@ int Sim::prepare_run(int init_err, double *dt_out, int *total) {
 // ---------------------------------------------------------------------------
 // state transfer
 // ---------------------------------------------------------------------------
+// Copies run on the context's own stream and wait only for it, so other
+// contexts on the device are neither waited for nor disturbed.
 int Sim::upload(const rmt3d_state *st) {
     size_t n = (size_t)g.Ng;
-    auto up = [&](double *dst, const double *src) {
-        if (src) hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyHostToDevice);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (src) hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream);
     };
-    up(U.vel[0], st->u); up(U.vel[1], st->v); up(U.vel[2], st->w);
-    up(U.p, st->p); up(U.dp, st->dp); up(U.q, st->q);
-    if (st->dvel) for (int a = 0; a < 3; a++) up(U.dvel[a], st->dvel + a * n);
-    if (st->fvel) for (int a = 0; a < 18; a++) up(U.fvel[a], st->fvel + a * n);
-    if (st->sigma) for (int a = 0; a < 9; a++) up(U.sigma[a], st->sigma + a * n);
-    if (st->rm_c) hipMemcpy(R.c, st->rm_c, sizeof(uint32_t) * n, hipMemcpyHostToDevice);
-    if (st->rm_x) for (int a = 0; a < 3; a++) up(R.x[a], st->rm_x + a * n);
-    if (st->rm_xpred) for (int a = 0; a < 3; a++) up(R.xpred[a], st->rm_xpred + a * n);
-    HCHK(hipDeviceSynchronize());
+    size_t w = sizeof(double) * n;
+    up(U.vel[0], st->u, w); up(U.vel[1], st->v, w); up(U.vel[2], st->w, w);
+    up(U.p, st->p, w); up(U.dp, st->dp, w); up(U.q, st->q, w);
+    if (st->dvel) for (int a = 0; a < 3; a++) up(U.dvel[a], st->dvel + a * n, w);
+    if (st->fvel) for (int a = 0; a < 18; a++) up(U.fvel[a], st->fvel + a * n, w);
+    if (st->sigma) for (int a = 0; a < 9; a++) up(U.sigma[a], st->sigma + a * n, w);
+    up(R.c, st->rm_c, sizeof(uint32_t) * n);
+    if (st->rm_x) for (int a = 0; a < 3; a++) up(R.x[a], st->rm_x + a * n, w);
+    if (st->rm_xpred) for (int a = 0; a < 3; a++) up(R.xpred[a], st->rm_xpred + a * n, w);
+    HCHK(hipStreamSynchronize(stream));
     return 0;
 }
 
 int Sim::band_count() {
-    int nb = 0;
+    hipMemcpyAsync(h_int, ext.n, sizeof(int), hipMemcpyDeviceToHost, stream);
     hipStreamSynchronize(stream);
-    hipMemcpy(&nb, ext.n, sizeof(int), hipMemcpyDeviceToHost);
-    return nb;
+    return h_int[0];
 }
 
 int Sim::download(rmt3d_state *st) {
-    HCHK(hipStreamSynchronize(stream));
     size_t n = (size_t)g.Ng;
-    auto dn = [&](double *dst, const double *src) {
-        if (dst) hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost);
+    auto dn = [&](void *dst, const void *src, size_t bytes) {
+        if (dst) hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
     };
-    dn(st->u, U.vel[0]); dn(st->v, U.vel[1]); dn(st->w, U.vel[2]);
-    dn(st->p, U.p); dn(st->dp, U.dp); dn(st->q, U.q);
-    if (st->dvel) for (int a = 0; a < 3; a++) dn(st->dvel + a * n, U.dvel[a]);
-    if (st->fvel) for (int a = 0; a < 18; a++) dn(st->fvel + a * n, U.fvel[a]);
-    if (st->sigma) for (int a = 0; a < 9; a++) dn(st->sigma + a * n, U.sigma[a]);
-    if (st->rm_c) hipMemcpy(st->rm_c, R.c, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
-    if (st->rm_x) for (int a = 0; a < 3; a++) dn(st->rm_x + a * n, R.x[a]);
-    if (st->rm_xpred) for (int a = 0; a < 3; a++) dn(st->rm_xpred + a * n, R.xpred[a]);
+    size_t w = sizeof(double) * n;
+    dn(st->u, U.vel[0], w); dn(st->v, U.vel[1], w); dn(st->w, U.vel[2], w);
+    dn(st->p, U.p, w); dn(st->dp, U.dp, w); dn(st->q, U.q, w);
+    if (st->dvel) for (int a = 0; a < 3; a++) dn(st->dvel + a * n, U.dvel[a], w);
+    if (st->fvel) for (int a = 0; a < 18; a++) dn(st->fvel + a * n, U.fvel[a], w);
+    if (st->sigma) for (int a = 0; a < 9; a++) dn(st->sigma + a * n, U.sigma[a], w);
+    dn(st->rm_c, R.c, sizeof(uint32_t) * n);
+    if (st->rm_x) for (int a = 0; a < 3; a++) dn(st->rm_x + a * n, R.x[a], w);
+    if (st->rm_xpred) for (int a = 0; a < 3; a++) dn(st->rm_xpred + a * n, R.xpred[a], w);
     int nb = band_count();
     if (st->band_cell || st->band_c || st->band_x || st->band_xpred) {
         if (st->nband < nb) {
@@ -899,29 +1002,30 @@ int Sim::download(rmt3d_state *st) {
             err = "band arrays too small";
             return RMT3D_ERR_ARG;
         }
-        if (st->band_cell) hipMemcpy(st->band_cell, ext.cell, sizeof(int) * nb, hipMemcpyDeviceToHost);
-        if (st->band_c) hipMemcpy(st->band_c, ext.c, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost);
+        size_t wb = sizeof(double) * nb;
+        dn(st->band_cell, ext.cell, sizeof(int) * nb);
+        dn(st->band_c, ext.c, sizeof(uint32_t) * nb);
         for (int a = 0; a < 3; a++) {
-            if (st->band_x) hipMemcpy(st->band_x + (size_t)a * nb, ext.x[a], sizeof(double) * nb, hipMemcpyDeviceToHost);
-            if (st->band_xpred)
-                hipMemcpy(st->band_xpred + (size_t)a * nb, ext.xpred[a], sizeof(double) * nb, hipMemcpyDeviceToHost);
+            if (st->band_x) dn(st->band_x + (size_t)a * nb, ext.x[a], wb);
+            if (st->band_xpred) dn(st->band_xpred + (size_t)a * nb, ext.xpred[a], wb);
         }
     }
     st->nband = nb;
-    HCHK(hipDeviceSynchronize());
+    HCHK(hipStreamSynchronize(stream));
     return 0;
 }
 
 int Sim::mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res) {
     MGSolver &G = which ? fem : mac;
     size_t nn = (size_t)G.L[0].nn;
-    HCHK(hipMemcpy(G.L[0].b, rhs, sizeof(double) * nn, hipMemcpyHostToDevice));
-    HCHK(hipMemcpy(G.L[0].x, x, sizeof(double) * nn, hipMemcpyHostToDevice));
+    HCHK(hipMemcpyAsync(G.L[0].b, rhs, sizeof(double) * nn, hipMemcpyHostToDevice, stream));
+    HCHK(hipMemcpyAsync(G.L[0].x, x, sizeof(double) * nn, hipMemcpyHostToDevice, stream));
     double save = G.tol_override;
     if (tol > 0) G.tol_override = tol;
     double r = G.run(stream);
     G.tol_override = save;
-    HCHK(hipMemcpy(x, G.L[0].x, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    HCHK(hipMemcpyAsync(x, G.L[0].x, sizeof(double) * nn, hipMemcpyDeviceToHost, stream));
+    HCHK(hipStreamSynchronize(stream));
     if (ncyc) *ncyc = G.last_cycles;
     if (res) *res = r;
     return 0;

@@ -4,12 +4,15 @@
 // of step_forward is a kernel launch on the context's stream.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/rmt3d.h"
 #include "launch.h"
 
 namespace rmt {
+
+std::mutex &hip_setup_mutex();
 
 // Event timing of one kernel class on the context stream: every launch of
 // the class is bracketed by two events; durations are summed on collect.
@@ -60,6 +63,10 @@ struct MGSolver {
     void vcycle64(hipStream_t s, int lev);
     void vcycle32(hipStream_t s, int lev);
     void cycle(hipStream_t s);              // one V-cycle (or one defect-correction step)
+    void cycle_body(hipStream_t s);         // its launches
+    hipGraphExec_t gexec = nullptr;         // the launches captured once
+    bool graph_failed = false;
+    bool no_graph = false;
     void timed_cycle(hipStream_t s);
     double run(hipStream_t s);
 };
@@ -106,15 +113,16 @@ class Sim {
     double *d_rhs_tmp = nullptr;       // FEM rhs for div_u
     MGSolver mac, fem;
     std::vector<void *> allocs;
-    double stage_ms[6] = {0, 0, 0, 0, 0, 0};
     // 0 FEM smoother colour pass (finest), 1 MAC smoother colour pass
-    // (finest), 2 stress, 3 advection, 4 FEM V-cycle, 5 MAC V-cycle
-    static const int NPROF = 6;
+    // (finest), 2 stress, 3 advection, 4 FEM V-cycle, 5 MAC V-cycle, then the
+    // six stage buckets (extrapolation, pressure Poisson, communication,
+    // stresses, MAC, advection)
+    static const int NPROF_KERNEL = 6;
+    static const int NPROF = 12;
     KProf kprof[NPROF];
     int profile(int enable);
     int profile_read(int which, double *ms, int *launches);
-    bool timing = false;
-    hipEvent_t tev[8];
+    int stage_times(double *ms, int n);
 
     size_t dev_bytes = 0;       // device bytes allocated by this context
     bool dalloc_failed = false;

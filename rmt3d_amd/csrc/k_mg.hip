@@ -23,28 +23,46 @@ __device__ __forceinline__ int nbw(int i, int di, int d, int prd) {
 
 struct Prd { int p[3]; };
 
+// A missing (wall) neighbour contributes +0 at its place in the sum instead
+// of being skipped: the running sum starts at +0 and is never -0, so adding
+// a zero leaves it bit-identical, and every lane of a wavefront executes the
+// same instructions.
 template <typename T>
 __device__ __forceinline__ T mac_off(const MGLevel<T> &L, const Prd &P, const T *x, int i, int j, int k, T &dg) {
     int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
+    int c = i + D0 * (j + D1 * k);
     T s = 0;
     int miss = 0, ii;
-    ii = nbw(i, -1, D0, P.p[0]); if (ii < 0) miss++; else s += x[ii + D0 * (j + D1 * k)];
-    ii = nbw(i, 1, D0, P.p[0]);  if (ii < 0) miss++; else s += x[ii + D0 * (j + D1 * k)];
-    ii = nbw(j, -1, D1, P.p[1]); if (ii < 0) miss++; else s += x[i + D0 * (ii + D1 * k)];
-    ii = nbw(j, 1, D1, P.p[1]);  if (ii < 0) miss++; else s += x[i + D0 * (ii + D1 * k)];
-    ii = nbw(k, -1, D2, P.p[2]); if (ii < 0) miss++; else s += x[i + D0 * (j + D1 * ii)];
-    ii = nbw(k, 1, D2, P.p[2]);  if (ii < 0) miss++; else s += x[i + D0 * (j + D1 * ii)];
+    bool v;
+#define MAC_TERM(expr, idx)                       \
+    ii = (expr);                                  \
+    v = ii >= 0;                                  \
+    miss += !v;                                   \
+    {                                             \
+        T xv = x[v ? (idx) : c];                  \
+        s += v ? xv : T(0);                       \
+    }
+    MAC_TERM(nbw(i, -1, D0, P.p[0]), ii + D0 * (j + D1 * k))
+    MAC_TERM(nbw(i, 1, D0, P.p[0]), ii + D0 * (j + D1 * k))
+    MAC_TERM(nbw(j, -1, D1, P.p[1]), i + D0 * (ii + D1 * k))
+    MAC_TERM(nbw(j, 1, D1, P.p[1]), i + D0 * (ii + D1 * k))
+    MAC_TERM(nbw(k, -1, D2, P.p[2]), i + D0 * (j + D1 * ii))
+    MAC_TERM(nbw(k, 1, D2, P.p[2]), i + D0 * (j + D1 * ii))
+#undef MAC_TERM
     dg = T(-6.0 + miss);
     return s;
 }
 
 // The wall halving of a weight is a multiplication by 1/2 per wall dimension
 // with zero offset: exact, so it equals the division by 2, 4 or 8 of the
-// closed form.  rdg returns 1/dg, exact because dg is 32 times a power of two.
+// closed form.  A missing neighbour enters with weight 0 at a valid address
+// (see mac_off).  rdg returns 1/dg, exact because dg is 32 times a power of
+// two.
 template <typename T>
 __device__ __forceinline__ T fem_off(const MGLevel<T> &L, const Prd &P, const T *x, int i, int j, int k, T &dg,
                                      T &rdg) {
     int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
+    int c = i + D0 * (j + D1 * k);
     int ia[3] = {nbw(i, -1, D0, P.p[0]), i, nbw(i, 1, D0, P.p[0])};
     int ja[3] = {nbw(j, -1, D1, P.p[1]), j, nbw(j, 1, D1, P.p[1])};
     int ka[3] = {nbw(k, -1, D2, P.p[2]), k, nbw(k, 1, D2, P.p[2])};
@@ -53,27 +71,27 @@ __device__ __forceinline__ T fem_off(const MGLevel<T> &L, const Prd &P, const T 
     bool w2 = !P.p[2] && (k == 0 || k == D2 - 1);
     T sx = w0 ? T(0.5) : T(1), sy = w1 ? T(0.5) : T(1), sz = w2 ? T(0.5) : T(1);
     T s = 0;
-    dg = 0;
 #pragma unroll
     for (int ok = 0; ok < 3; ok++) {
-        if (ka[ok] < 0) continue;
 #pragma unroll
         for (int oj = 0; oj < 3; oj++) {
-            if (ja[oj] < 0) continue;
+            int row = D0 * (ja[oj] + D1 * ka[ok]);
+            bool vjk = ja[oj] >= 0 && ka[ok] >= 0;
 #pragma unroll
             for (int oi = 0; oi < 3; oi++) {
-                if (ia[oi] < 0) continue;
                 const int nz = (oi != 1) + (oj != 1) + (ok != 1);
-                if (nz == 1) continue;
-                T w = (nz == 0) ? T(32) : (nz == 2) ? T(-2) : T(-1);
+                if (nz < 2) continue;               // centre and face terms
+                bool v = vjk && ia[oi] >= 0;
+                T w = (nz == 2) ? T(-2) : T(-1);
                 if (oi == 1) w *= sx;
                 if (oj == 1) w *= sy;
                 if (ok == 1) w *= sz;
-                if (nz == 0) dg = w;
-                else s += w * x[ia[oi] + D0 * (ja[oj] + D1 * ka[ok])];
+                T xv = x[v ? ia[oi] + row : c];
+                s += (v ? w : T(0)) * xv;
             }
         }
     }
+    dg = T(32) * sx * sy * sz;
     rdg = T(0.03125);
     if (w0) rdg *= T(2);
     if (w1) rdg *= T(2);
@@ -119,38 +137,11 @@ __global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, 
     mac_smooth_at(L, P, i, j, k);
 }
 
-// interior node (no wall, no wrap): the 20 non-zero neighbours in the order
-// of fem_off's (k, j, i) loop, so the sum is bit-identical to the general path
-template <typename T>
-__device__ __forceinline__ T fem_off_interior(const T *x, int c, int D0, int D01) {
-    T s = 0;
-    const T *p = x + c - D01;
-    s += T(-1) * p[-D0 - 1]; s += T(-2) * p[-D0]; s += T(-1) * p[-D0 + 1];
-    s += T(-2) * p[-1];      s += T(-2) * p[1];
-    s += T(-1) * p[D0 - 1];  s += T(-2) * p[D0];  s += T(-1) * p[D0 + 1];
-    p = x + c;
-    s += T(-2) * p[-D0 - 1]; s += T(-2) * p[-D0 + 1];
-    s += T(-2) * p[D0 - 1];  s += T(-2) * p[D0 + 1];
-    p = x + c + D01;
-    s += T(-1) * p[-D0 - 1]; s += T(-2) * p[-D0]; s += T(-1) * p[-D0 + 1];
-    s += T(-2) * p[-1];      s += T(-2) * p[1];
-    s += T(-1) * p[D0 - 1];  s += T(-2) * p[D0];  s += T(-1) * p[D0 + 1];
-    return s;
-}
-
-// x = (b - s) / dg, evaluated as a product with the exact reciprocal of the
-// power-of-two diagonal
 template <typename T>
 __device__ __forceinline__ void fem_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
-    int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
-    int c = i + D0 * (j + D1 * k);
-    T s, dg, rdg;
-    if (i >= 1 && i <= D0 - 2 && j >= 1 && j <= D1 - 2 && k >= 1 && k <= D2 - 2) {
-        s = fem_off_interior(L.x, c, D0, D0 * D1);
-        rdg = T(0.03125);
-    } else {
-        s = fem_off(L, P, L.x, i, j, k, dg, rdg);
-    }
+    int c = i + L.d[0] * (j + L.d[1] * k);
+    T dg, rdg;
+    T s = fem_off(L, P, L.x, i, j, k, dg, rdg);
     L.x[c] = (L.b[c] - s) * rdg;
 }
 
@@ -171,11 +162,7 @@ __global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, int hy, float rd0, 
 
 template <typename T>
 __device__ __forceinline__ T apply_off(const MGLevel<T> &L, const Prd &P, int fem, int i, int j, int k, int c, T &dg) {
-    int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
-    if (fem && i >= 1 && i <= D0 - 2 && j >= 1 && j <= D1 - 2 && k >= 1 && k <= D2 - 2) {
-        dg = T(32);
-        return fem_off_interior(L.x, c, D0, D0 * D1);
-    }
+    (void)c;
     return fem ? fem_off(L, P, L.x, i, j, k, dg) : mac_off(L, P, L.x, i, j, k, dg);
 }
 
@@ -187,11 +174,19 @@ __device__ __forceinline__ void resid_at(const MGLevel<T> &L, const Prd &P, int 
     L.r[c] = L.b[c] - (s + dg * L.x[c]);
 }
 
+// one thread per node of a level, t -> (i, j, k) with float reciprocals of
+// the two fastest dimensions
+#define FLAT_IJK(D0, D1, r0, r1)              \
+    int t = blockIdx.x * blockDim.x + threadIdx.x; \
+    int row = qdiv(t, D0, r0);                 \
+    int i = t - row * (D0);                    \
+    int k = qdiv(row, D1, r1);                 \
+    int j = row - k * (D1);
+
 template <typename T>
-__global__ void k_resid(MGLevel<T> L, Prd P, int fem) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= L.d[0]) return;
+__global__ void k_resid(MGLevel<T> L, Prd P, int fem, float r0, float r1) {
+    FLAT_IJK(L.d[0], L.d[1], r0, r1)
+    if (k >= L.d[2]) return;
     resid_at(L, P, fem, i, j, k);
 }
 
@@ -239,17 +234,22 @@ __device__ __forceinline__ void restrict_at(const MGLevel<T> &F, const MGLevel<T
                     s += F.r[(2 * I + a) + F.d[0] * ((2 * J + b) + F.d[1] * (2 * K + c))];
         s *= T(0.5);
     } else {
+        // full weighting; a fine node outside the grid enters with weight 0
+        // at a valid address (see mac_off)
+        int c0 = 2 * I + F.d[0] * (2 * J + F.d[1] * 2 * K);
+#pragma unroll
         for (int dc = -1; dc <= 1; dc++) {
             int kk = nbw(2 * K, dc, F.d[2], P.p[2]);
-            if (kk < 0) continue;
+#pragma unroll
             for (int db = -1; db <= 1; db++) {
                 int jj = nbw(2 * J, db, F.d[1], P.p[1]);
-                if (jj < 0) continue;
+#pragma unroll
                 for (int da = -1; da <= 1; da++) {
                     int ii = nbw(2 * I, da, F.d[0], P.p[0]);
-                    if (ii < 0) continue;
+                    bool v = ii >= 0 && jj >= 0 && kk >= 0;
                     T w = (da ? T(0.5) : T(1.0)) * (db ? T(0.5) : T(1.0)) * (dc ? T(0.5) : T(1.0));
-                    s += w * F.r[ii + F.d[0] * (jj + F.d[1] * kk)];
+                    T rv = F.r[v ? ii + F.d[0] * (jj + F.d[1] * kk) : c0];
+                    s += (v ? w : T(0)) * rv;
                 }
             }
         }
@@ -261,11 +261,10 @@ __device__ __forceinline__ void restrict_at(const MGLevel<T> &F, const MGLevel<T
 }
 
 template <typename T>
-__global__ void k_restrict(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
-    int I = blockIdx.x * blockDim.x + threadIdx.x;
-    int J = blockIdx.y, K = blockIdx.z;
-    if (I >= C.d[0]) return;
-    restrict_at(F, C, P, fem, I, J, K);
+__global__ void k_restrict(MGLevel<T> F, MGLevel<T> C, Prd P, int fem, float r0, float r1) {
+    FLAT_IJK(C.d[0], C.d[1], r0, r1)
+    if (k >= C.d[2]) return;
+    restrict_at(F, C, P, fem, i, j, k);
 }
 
 template <typename T>
@@ -309,10 +308,9 @@ __device__ __forceinline__ void prolong_at(const MGLevel<T> &F, const MGLevel<T>
 }
 
 template <typename T>
-__global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= F.d[0]) return;
+__global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem, float r0, float r1) {
+    FLAT_IJK(F.d[0], F.d[1], r0, r1)
+    if (k >= F.d[2]) return;
     prolong_at(F, C, P, fem, i, j, k);
 }
 
@@ -498,10 +496,12 @@ void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd
                            1.0f / L.d[0], 1.0f / hy);
     }
 }
+static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
+
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]) {
-    dim3 gr((L.d[0] + 63) / 64, L.d[1], L.d[2]);
-    hipLaunchKernelGGL(k_resid<T>, gr, dim3(64), 0, s, L, mkprd(prd), fem);
+    hipLaunchKernelGGL(k_resid<T>, flat_grid(L.nn), dim3(256), 0, s, L, mkprd(prd), fem, 1.0f / L.d[0],
+                       1.0f / L.d[1]);
 }
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits) {
@@ -511,13 +511,13 @@ void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, con
 }
 template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
-    dim3 gr((C.d[0] + 63) / 64, C.d[1], C.d[2]);
-    hipLaunchKernelGGL(k_restrict<T>, gr, dim3(64), 0, s, F, C, mkprd(prd), fem);
+    hipLaunchKernelGGL(k_restrict<T>, flat_grid(C.nn), dim3(256), 0, s, F, C, mkprd(prd), fem, 1.0f / C.d[0],
+                       1.0f / C.d[1]);
 }
 template <typename T>
 void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
-    dim3 gr((F.d[0] + 63) / 64, F.d[1], F.d[2]);
-    hipLaunchKernelGGL(k_prolong<T>, gr, dim3(64), 0, s, F, C, mkprd(prd), fem);
+    hipLaunchKernelGGL(k_prolong<T>, flat_grid(F.nn), dim3(256), 0, s, F, C, mkprd(prd), fem, 1.0f / F.d[0],
+                       1.0f / F.d[1]);
 }
 template <typename T>
 void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int sweeps) {

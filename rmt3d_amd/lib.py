@@ -223,14 +223,24 @@ class Simulation:
                     "mg_solve")
         return x, nc.value, res.value
 
-    def profile(self, enable=True):
-        self._check(lib().rmt3d_profile(self.h, 1 if enable else 0), "profile")
+    def profile(self, level=1):
+        """0 off, 1 stage buckets and V-cycles, 2 also per-kernel classes
+        (V-cycles then launch eagerly instead of replaying their graph)."""
+        self._check(lib().rmt3d_profile(self.h, int(level)), "profile")
 
     def profile_read(self, which):
         ms = C.c_double()
         n = C.c_int()
         self._check(lib().rmt3d_profile_read(self.h, which, C.byref(ms), C.byref(n)), "profile_read")
         return ms.value, n.value
+
+    STAGES = ("extrapolation", "pressure_poisson", "communication", "stresses", "mac", "advection")
+
+    def stage_times(self):
+        """Device ms per stage bucket over the steps run while profiling."""
+        ms = (C.c_double * 6)()
+        self._check(lib().rmt3d_stage_times(self.h, ms, 6), "rmt3d_stage_times")
+        return dict(zip(self.STAGES, list(ms)))
 
     def synchronize(self):
         self._check(lib().rmt3d_device_synchronize(self.h), "synchronize")
