@@ -20,7 +20,7 @@
  * (mg_opt defaults, mg_probs.hh:92-93).
  *
  * V-cycle (this project's design; PGMG's is absent, so its iterates are
- * parity unpinned): MAC -- red-black Gauss-Seidel, restriction = sum of the
+ * parity unpinned): MAC -- red-black SOR (omega 1.25), restriction = sum of the
  * 8 children / 2, cell-centred trilinear prolongation; FEM -- four-colour
  * Gauss-Seidel by (j,k) parity (whole x-rows per colour, valid because the
  * Q1 face weights vanish), full-weighting restriction (= P^T) with the
@@ -35,6 +35,9 @@
 #include "orc.h"
 
 #define NCOARSE_SWEEPS 4
+/* over-relaxation of the MAC red-black sweeps: 1.25 brings the V-cycles per
+ * solve from about 13 to 9-10 on the sphere and Taylor-Green workloads */
+#define MAC_OMEGA 1.25
 
 /* a periodic dimension is halved only while the half stays even, so that
  * the parity colourings of the smoothers stay valid across the wrap */
@@ -220,7 +223,9 @@ static void smooth(orc_mg *G, int lev)
                     double dg, off;
                     if (G->fem) fem_Ax(L, G->prd, L->x, i, j, k, &dg, &off);
                     else mac_Ax(L, G->prd, L->x, i, j, k, &dg, &off);
-                    L->x[c] = (L->b[c] - off) / dg;
+                    double xn = (L->b[c] - off) / dg;
+                    if (G->fem) L->x[c] = xn;
+                    else L->x[c] = (1 - MAC_OMEGA) * L->x[c] + MAC_OMEGA * xn;
                 }
 }
 

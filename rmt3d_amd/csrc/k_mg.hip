@@ -6,8 +6,9 @@
 // (Utilities/mg_probs.hh:281-343, 458-530) in closed form: MAC 7-point with
 // the missing wall face folded into the centre; FEM 27-point Q1 with wall
 // entries halved per wall direction whose offset is zero.  Periodic
-// dimensions wrap.  Smoothers: red-black (MAC) and four-colour by (j,k)
-// parity (FEM: whole x-rows per colour) Gauss-Seidel, one launch per colour; inside a colour no two unknowns are
+// dimensions wrap.  Smoothers: red-black SOR (MAC, omega 1.25) and
+// four-colour Gauss-Seidel by (j,k) parity (FEM: whole x-rows per colour),
+// one launch per colour; inside a colour no two unknowns are
 // neighbours, so the update order is free and the result is deterministic.
 #include "dev.h"
 #include "launch.h"
@@ -115,12 +116,16 @@ __device__ __forceinline__ int qdiv(int a, int d, float rd) {
     return q;
 }
 
+// red-black SOR: x <- (1 - w) x + w (b - s) / dg, w = 1.25
+constexpr double MAC_OMEGA = 1.25;
+
 template <typename T>
 __device__ __forceinline__ void mac_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
     int c = i + L.d[0] * (j + L.d[1] * k);
     T dg;
     T s = mac_off(L, P, L.x, i, j, k, dg);
-    L.x[c] = (L.b[c] - s) / dg;
+    T xn = (L.b[c] - s) / dg;
+    L.x[c] = T(1 - MAC_OMEGA) * L.x[c] + T(MAC_OMEGA) * xn;
 }
 
 // one thread per node of the colour: t -> (ih, j, k), i = 2 ih + parity
@@ -412,9 +417,8 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
                     unflat(e, W.d[0], W.d[1], ra, rb, i, j, k);
                     int my = fem ? ((j & 1) + 2 * (k & 1)) : ((i + j + k) & 1);
                     if (my != c) continue;
-                    T dg;
-                    T s = fem ? fem_off(W, P, W.x, i, j, k, dg) : mac_off(W, P, W.x, i, j, k, dg);
-                    W.x[e] = (W.b[e] - s) / dg;
+                    if (fem) fem_smooth_at(W, P, i, j, k);
+                    else mac_smooth_at(W, P, i, j, k);
                 }
                 __syncthreads();
             }
@@ -434,38 +438,6 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
     }
     if (lds)
         for (int e = threadIdx.x; e < lv[0].nn; e += blockDim.x) t.L[0].x[e] = lv[0].x[e];
-}
-
-// coarsest level: coloured Gauss-Seidel sweeps inside one workgroup, on an
-// LDS copy when it fits
-template <typename T>
-__global__ void k_coarse(MGLevel<T> L, Prd P, int fem, int sweeps, int use_lds) {
-    extern __shared__ unsigned char smem[];
-    int n = L.nn;
-    int ncol = fem ? 4 : 2;
-    MGLevel<T> W = L;
-    if (use_lds) {
-        T *xs = (T *)smem, *bs = xs + n;
-        for (int c = threadIdx.x; c < n; c += blockDim.x) { xs[c] = L.x[c]; bs[c] = L.b[c]; }
-        W.x = xs;
-        W.b = bs;
-        __syncthreads();
-    }
-    for (int sw = 0; sw < sweeps; sw++) {
-        for (int col = 0; col < ncol; col++) {
-            for (int c = threadIdx.x; c < n; c += blockDim.x) {
-                int i = c % W.d[0], j = (c / W.d[0]) % W.d[1], k = c / (W.d[0] * W.d[1]);
-                int my = fem ? ((j & 1) + 2 * (k & 1)) : ((i + j + k) & 1);
-                if (my != col) continue;
-                T dg;
-                T s = fem ? fem_off(W, P, W.x, i, j, k, dg) : mac_off(W, P, W.x, i, j, k, dg);
-                W.x[c] = (W.b[c] - s) / dg;
-            }
-            __syncthreads();
-        }
-    }
-    if (use_lds)
-        for (int c = threadIdx.x; c < n; c += blockDim.x) L.x[c] = W.x[c];
 }
 
 __global__ void k_cast_d2f(const double *a, float *b, int n) {
@@ -520,12 +492,6 @@ void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLeve
                        1.0f / F.d[1]);
 }
 template <typename T>
-void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int sweeps) {
-    size_t lds = 2 * sizeof(T) * (size_t)L.nn;
-    int use = lds <= 48 * 1024;
-    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(256), use ? lds : 0, s, L, mkprd(prd), fem, sweeps, use);
-}
-template <typename T>
 size_t mg_tail_lds_bytes(const MGLevel<T> *L, int n) {
     size_t b = 0;
     for (int l = 0; l < n; l++) b += 3 * sizeof(T) * (size_t)L[l].nn;
@@ -568,7 +534,6 @@ void launch_zero_f(hipStream_t s, float *a, int n) {
     template void launch_mg_resid<T>(hipStream_t, int, const MGLevel<T> &, const int *);                    \
     template void launch_mg_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *); \
     template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *);  \
-    template void launch_mg_coarse<T>(hipStream_t, int, const MGLevel<T> &, const int *, int);            \
     template void launch_mg_tail<T>(hipStream_t, int, const MGLevel<T> *, int, const int *, int, int, int, int); \
     template size_t mg_tail_lds_bytes<T>(const MGLevel<T> *, int);                                          \
     template int mg_tail_enable_lds<T>(size_t);

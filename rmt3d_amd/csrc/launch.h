@@ -65,7 +65,7 @@ struct MGTail {
     int n;
 };
 
-// smooth one colour (MAC: red-black, FEM: four colours by (j,k) parity)
+// smooth one colour (MAC: red-black SOR, FEM: four colours by (j,k) parity)
 template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color);
 template <typename T>
@@ -74,8 +74,6 @@ template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
 template <typename T>
 void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
-template <typename T>
-void launch_mg_coarse(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int sweeps);
 // the rest of a V-cycle from L[0] (finest of the tail) to L[n-1] (coarsest)
 template <typename T>
 void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const int prd[3], int vdown, int vup,
