@@ -827,7 +827,7 @@ int Sim::step(int debug) {
     mark_end(3);
     mark(5);
     kprof[3].begin(stream);
-    launch_advect(stream, g, P, d_obj, U, M, d_status);
+    launch_advect(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n, d_status);
     kprof[3].end(stream);
     mark_end(5);
     time += 0.5 * dt;

@@ -24,8 +24,9 @@ struct InitSpec {
 // list: Ng ints of scratch for the solid cells, count: one int
 void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    int *list, int *count);
+// list/count: the solid-path cells gathered by launch_stress of this step
 void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
-                   int *status);
+                   const int *list, const int *count, int *status);
 void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status);
 void launch_bc_cc(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U);
 void launch_bc_cf(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U);
