@@ -231,10 +231,23 @@ __device__ __forceinline__ double nb_weight(const Geo &g, const Scratch &W, int 
     return wpow[dist];
 }
 
-__global__ void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt, LsqArgs A,
-                             int *status) {
+// One wavefront per cell of the layer.  The lanes evaluate the candidate
+// stencil points in parallel, 64 at a time in the (r, q, p) order of the
+// reference's loops, and park each point's terms in LDS; then one lane per
+// sum accumulates its sum over the points in that order.  Every sum is thus
+// the same sequence of additions as the serial loop of extrap_helper
+// (bit-identical), while the stencil gathers of a cell run 64-wide.
+constexpr int LSQ_T1 = 7;    // ravg[3], davg[3], wsum
+constexpr int LSQ_T2 = 15;   // am[6], rhs[9]
+
+__global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt,
+                                                   LsqArgs A, int *status) {
+    __shared__ double term[LSQ_T2][64];
+    __shared__ int valid[64];
+    __shared__ double bc[LSQ_T2];
+    const int lane = threadIdx.x;
     int n = *cnt;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
         int ind = list[t];
         if (!within_extrap(g, ind)) continue;
         int ii, jj, kk;
@@ -245,46 +258,89 @@ __global__ void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, 
         bool ok = false;
         double davg[3], ravg[3], coeff[9];
         for (int sr = 2; !ok && sr <= A.max_rs; sr++) {
+            const int D = 2 * sr + 1, nc = D * D * D;
             // pass 1: weighted sums (extrap_helper.cc:49-62)
+            double acc = 0;
             int npt = 0;
-            double wsum = 0;
-            for (int a = 0; a < 3; a++) { ravg[a] = 0; davg[a] = 0; }
-            for (int r = -sr; r <= sr; r++)
-                for (int q = -sr; q <= sr; q++)
-                    for (int p = -sr; p <= sr; p++) {
-                        double w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
-                        if (w < 0) continue;
-                        int nb = ind + p + g.m4 * q + g.mn4 * r;
-                        ravg[0] += W.val[0][nb] * w; ravg[1] += W.val[1][nb] * w; ravg[2] += W.val[2][nb] * w;
-                        davg[0] += g.lx[ii + p] * w; davg[1] += g.ly[jj + q] * w; davg[2] += g.lz[kk + r] * w;
-                        wsum += w;
-                        npt++;
-                    }
+            for (int k0 = 0; k0 < nc; k0 += 64) {
+                int k = k0 + lane;
+                double w = -1;
+                int p = 0, q = 0, r = 0;
+                if (k < nc) {
+                    p = k % D - sr; q = (k / D) % D - sr; r = k / (D * D) - sr;
+                    w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
+                }
+                bool v = w >= 0;
+                valid[lane] = v;
+                if (v) {
+                    int nb = ind + p + g.m4 * q + g.mn4 * r;
+                    term[0][lane] = W.val[0][nb] * w;
+                    term[1][lane] = W.val[1][nb] * w;
+                    term[2][lane] = W.val[2][nb] * w;
+                    term[3][lane] = g.lx[ii + p] * w;
+                    term[4][lane] = g.ly[jj + q] * w;
+                    term[5][lane] = g.lz[kk + r] * w;
+                    term[6][lane] = w;
+                }
+                npt += __popcll(__ballot(v));
+                __syncthreads();
+                if (lane < LSQ_T1) {
+                    int ne = min(64, nc - k0);
+                    for (int e = 0; e < ne; e++)
+                        if (valid[e]) acc += term[lane][e];
+                }
+                __syncthreads();
+            }
             if (npt == 0) continue;
+            if (lane < LSQ_T1) bc[lane] = acc;
+            __syncthreads();
+            double wsum = bc[6];
+            for (int a = 0; a < 3; a++) { ravg[a] = bc[a]; davg[a] = bc[3 + a]; }
+            __syncthreads();
             for (int a = 0; a < 3; a++) ravg[a] /= wsum;
             for (int a = 0; a < 3; a++) davg[a] /= wsum;
             // pass 2: centred moments (extrap_helper.cc:68-75, 129-146)
-            double am[6] = {0, 0, 0, 0, 0, 0}, rhs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int r = -sr; r <= sr; r++)
-                for (int q = -sr; q <= sr; q++)
-                    for (int p = -sr; p <= sr; p++) {
-                        double Q = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
-                        if (Q < 0) continue;
-                        int nb = ind + p + g.m4 * q + g.mn4 * r;
-                        double dp0 = g.lx[ii + p], dp1 = g.ly[jj + q], dp2 = g.lz[kk + r];
-                        am[0] += Q * (dp0 - davg[0]) * (dp0 - davg[0]);
-                        am[1] += Q * (dp1 - davg[1]) * (dp1 - davg[1]);
-                        am[2] += Q * (dp2 - davg[2]) * (dp2 - davg[2]);
-                        am[3] += Q * (dp0 - davg[0]) * (dp1 - davg[1]);
-                        am[4] += Q * (dp0 - davg[0]) * (dp2 - davg[2]);
-                        am[5] += Q * (dp1 - davg[1]) * (dp2 - davg[2]);
-                        double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
-                        double dd[3] = {dp0 - davg[0], dp1 - davg[1], dp2 - davg[2]};
-                        for (int i = 0; i < 3; i++) {
-                            double lr = Q * (rv[i] - ravg[i]);
-                            for (int j = 0; j < 3; j++) rhs[3 * i + j] += dd[j] * lr;
-                        }
+            acc = 0;
+            for (int k0 = 0; k0 < nc; k0 += 64) {
+                int k = k0 + lane;
+                double Q = -1;
+                int p = 0, q = 0, r = 0;
+                if (k < nc) {
+                    p = k % D - sr; q = (k / D) % D - sr; r = k / (D * D) - sr;
+                    Q = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
+                }
+                bool v = Q >= 0;
+                valid[lane] = v;
+                if (v) {
+                    int nb = ind + p + g.m4 * q + g.mn4 * r;
+                    double dp0 = g.lx[ii + p], dp1 = g.ly[jj + q], dp2 = g.lz[kk + r];
+                    term[0][lane] = Q * (dp0 - davg[0]) * (dp0 - davg[0]);
+                    term[1][lane] = Q * (dp1 - davg[1]) * (dp1 - davg[1]);
+                    term[2][lane] = Q * (dp2 - davg[2]) * (dp2 - davg[2]);
+                    term[3][lane] = Q * (dp0 - davg[0]) * (dp1 - davg[1]);
+                    term[4][lane] = Q * (dp0 - davg[0]) * (dp2 - davg[2]);
+                    term[5][lane] = Q * (dp1 - davg[1]) * (dp2 - davg[2]);
+                    double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
+                    double dd[3] = {dp0 - davg[0], dp1 - davg[1], dp2 - davg[2]};
+                    for (int i = 0; i < 3; i++) {
+                        double lr = Q * (rv[i] - ravg[i]);
+                        for (int j = 0; j < 3; j++) term[6 + 3 * i + j][lane] = dd[j] * lr;
                     }
+                }
+                __syncthreads();
+                if (lane < LSQ_T2) {
+                    int ne = min(64, nc - k0);
+                    for (int e = 0; e < ne; e++)
+                        if (valid[e]) acc += term[lane][e];
+                }
+                __syncthreads();
+            }
+            if (lane < LSQ_T2) bc[lane] = acc;
+            __syncthreads();
+            double am[6], rhs[9];
+            for (int a = 0; a < 6; a++) am[a] = bc[a];
+            for (int a = 0; a < 9; a++) rhs[a] = bc[6 + a];
+            __syncthreads();
             // mat3::inverse (mat3.hh:61-67) and extrap_helper::dot
             double a11 = am[0], a12 = am[3], a13 = am[4], a21 = am[3], a22 = am[1], a23 = am[5], a31 = am[4],
                    a32 = am[5], a33 = am[2];
@@ -302,6 +358,7 @@ __global__ void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, 
             }
             ok = true;
         }
+        if (lane != 0) continue;
         if (!ok) { atomicOr(status, ST_FATAL_EXTRAP); continue; }
         // fluid_3d::transfer (fluid_3d_ex.cc:293-313)
         double ps[3] = {g.lx[ii] - davg[0], g.ly[jj] - davg[1], g.lz[kk] - davg[2]};
@@ -512,7 +569,7 @@ void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
             nmask++;
             LsqArgs A;
             A.L = L; A.max_rs = max_rs; A.oid = (uint32_t)oid; A.wpow = wpow_dev;
-            hipLaunchKernelGGL(k_gather_lsq, dim3(GRID_STRIDE_BLOCKS), dim3(64), 0, s, g, R, W.tband, W,
+            hipLaunchKernelGGL(k_gather_lsq, dim3(8192), dim3(64), 0, s, g, R, W.tband, W,
                                W.list[cur], W.cnt + cur, A, status);
             int nxt = cur ^ 1;
             launch_zero_i32(s, W.cnt + nxt, 1);
