@@ -14,6 +14,14 @@ rmt3d_amd/sim_configs/ for the C++ driver.
   stretched_sphere(n)  pre-stretched_sphere.cfg at n^3 with no-slip walls,
                        fmu 0.01, sphere r=0.25 at the centre pre-stretched to
                        F=diag(1.3, 1.3^-1/2, 1.3^-1/2), G=1, nlayers 4.
+  shear_spheres(n, k)  n^3 box periodic in x and y, no-slip walls at z = 0
+                       and 1 moving at -0.5 and +0.5 along x (simple shear),
+                       fmu 0.01; k spheres with radius uniform in
+                       [0.04, 0.08] and G log-uniform in [0.5, 5], density
+                       ratio 1, placed by random sequential addition without
+                       overlap from a fixed seed, each kept inside the box
+                       (the reference map of a sphere is not shifted across
+                       a periodic boundary, fluid_3d_comm.cc).
 """
 import math
 
@@ -122,3 +130,89 @@ def taylor_green_field(cfg):
     v = -np.cos(tp * X) * np.sin(tp * Y) * np.cos(tp * Z)
     w = np.zeros_like(u)
     return u.ravel(), v.ravel(), w.ravel()
+
+
+def shear_spheres(n=256, count=64, steps=300, seed=20261202, rmin=0.04, rmax=0.08):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    c = default_config()
+    for i in range(3):
+        c.sys_size[i] = n
+    c.prd[0] = c.prd[1] = 1
+    for f in (4, 5):
+        for comp in range(3):
+            c.bct[comp][f] = 1
+            c.bcv[comp][f] = 0.0
+    c.bcv[0][4] = -0.5
+    c.bcv[0][5] = 0.5
+    c.fmu = 0.01
+    c.dt = 10.0
+    c.T = 30.0
+    c.frames = 300
+    c.sim_type = SIM_OBJECTS
+    c.nlayers = 4
+    c.wt_n = 1.5
+    c.ex_visc_mult = 1.0
+    c.ev_trans_mult = 0.0
+    c.stop_short = steps
+    c.dump_code = 0
+    objs = []
+    placed = []
+    for _ in range(200000):
+        if len(placed) == count:
+            break
+        r = rng.uniform(rmin, rmax)
+        ctr = rng.uniform(r + 0.02, 1 - r - 0.02, 3)
+        if all(np.linalg.norm(ctr - c2) > r + r2 + 0.02 for c2, r2 in placed):
+            placed.append((ctr, r))
+    for ctr, r in placed:
+        G = float(np.exp(rng.uniform(np.log(0.5), np.log(5.0))))
+        objs.append(sphere(tuple(ctr), r, G=G))
+    c.n_obj = len(objs)
+    finalize_config(c)
+    return c, objs
+
+
+_SIM_TYPES = {0: "ftest", 1: "stest", 2: "objects", 3: "object_mills"}
+
+
+def write_cfg(cfg, objs, path, comment=""):
+    """Write cfg/objs as a .cfg file in the reference's keyword format
+    (Sim/sim_params.cc), readable by rmt3d_parse_cfg and by the reference."""
+    def f(v):
+        return repr(float(v))
+    lines = []
+    for ln in comment.splitlines():
+        lines.append("# " + ln)
+    lines.append("#Keyword                  Input")
+    put = lambda k, *v: lines.append("%-25s %s" % (k, " ".join(str(x) for x in v)))
+    put("frames", cfg.frames)
+    put("dump_code", cfg.dump_code)
+    put("x_prd", cfg.prd[0]); put("y_prd", cfg.prd[1]); put("z_prd", cfg.prd[2])
+    for fi, name in enumerate(("bc_xl", "bc_xh", "bc_yl", "bc_yh", "bc_zl", "bc_zh")):
+        put(name, *([cfg.bct[c][fi] for c in range(3)] + [f(cfg.bcv[c][fi]) for c in range(3)]))
+    put("walls", *[cfg.walls[i] for i in range(6)])
+    put("wall_pos", *[f(cfg.wall_pos[i]) for i in range(6)])
+    put("sys_size", *list(cfg.sys_size))
+    put("xyz_bounds", *[f(cfg.xyz_bounds[i]) for i in range(6)])
+    put("dt", f(cfg.dt))
+    put("fdt_pad", f(cfg.fdt_pad))
+    put("T", f(cfg.T))
+    put("stop_short", cfg.stop_short)
+    put("sim_type", _SIM_TYPES[cfg.sim_type])
+    put("fmu", f(cfg.fmu))
+    put("nlayers", cfg.nlayers)
+    put("max_extrap_rs", cfg.max_extrap_rs)
+    put("wt_n", f(cfg.wt_n))
+    put("ex_visc_mult", f(cfg.ex_visc_mult))
+    put("ev_trans_mult", f(cfg.ev_trans_mult))
+    put("weight_fac", f(cfg.weight_fac))
+    put("n_obj", len(objs))
+    for i, o in enumerate(objs):
+        put("object_list", i, o.type)
+        put("basic_specs", i, *[f(o.basic[a]) for a in range(4)])
+        put("extra_specs", i, *[f(o.extra[a]) for a in range(10)])
+        put("srho", i, f(o.srho))
+        put("shear_mod", i, f(o.shear_mod))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")

@@ -166,6 +166,20 @@ __device__ __forceinline__ double tderiv_func_in(const Phys &P, double x) {
 }
 __device__ __forceinline__ double coll_func(const Phys &P, double x) { return x > P.eps ? 0 : (0.5 - x * P.tf1); }
 
+// Slot of this lane in an append to a device list: one atomic per wavefront
+// for all its appending lanes (-1 for a lane that does not append).  Every
+// active lane of the wavefront must reach the call.
+__device__ __forceinline__ int wave_append(bool pred, int *counter) {
+    unsigned long long bal = __ballot(pred);
+    if (!bal) return -1;
+    int lane = __lane_id();
+    int first = __ffsll((long long)bal) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(counter, __popcll(bal));
+    base = __shfl(base, first, 64);
+    return pred ? base + __popcll(bal & ((1ull << lane) - 1)) : -1;
+}
+
 __device__ __forceinline__ int symind(int i, int j) {
     if (i > j) { int t = j; j = i; i = t; }
     return j + i * (5 - i) / 2;

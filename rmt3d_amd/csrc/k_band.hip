@@ -138,8 +138,9 @@ __global__ void k_scan_ghost(Geo g, Scratch W, uint32_t nmask) {
 __global__ void k_compact(Geo g, Scratch W, int *list, int *cnt) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.Ng) return;
-    if (W.flag[t]) {
-        int p = atomicAdd(cnt, 1);
+    bool f = W.flag[t] != 0;
+    int p = wave_append(f, cnt);
+    if (f) {
         list[p] = t;
         W.flag[t] = 0;
     }
@@ -391,8 +392,8 @@ __global__ void k_stage_obj(Geo g, Scratch W, uint32_t oid, int *status) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.Ng) return;
     int L = W.lid[t];
+    int p = wave_append(L >= 1, W.st_n);
     if (L < 1) return;
-    int p = atomicAdd(W.st_n, 1);
     if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); return; }
     W.st_cell[p] = t;
     W.st_c[p] = oid | ((uint32_t)L << LDIG);

@@ -826,12 +826,14 @@ struct GradPhi { double v[8][3]; };
 
 struct Faces6 { FaceC f[6]; };
 
+// one thread per node (flat index over fm x fn x fo)
 __global__ void k_pp_rhs(Geo g, Fluid U, GradPhi gp, Faces6 faces, int nwall, double *r) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    int b = blockIdx.y, cz = blockIdx.z;
-    if (a >= g.fm) return;
+    int tt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= g.fm * g.fn * g.fo) return;
+    int a = tt % g.fm, b = (tt / g.fm) % g.fn, cz = tt / (g.fm * g.fn);
     double acc = 0;
     // elements in (k, j, i) ascending order: corner n = 7..0
+#pragma unroll
     for (int n = 7; n >= 0; n--) {
         int ei = a - n % 2, ej = b - (n / 2) % 2, ek = cz - n / 4;
         if (ei < -1 || ej < -1 || ek < -1) continue;
@@ -1125,7 +1127,8 @@ void launch_pp_rhs(hipStream_t s, const Geo &g, const Fluid &U, const double gra
     Faces6 fc;
     memset(&fc, 0, sizeof fc);
     for (int w = 0; w < nwall && w < 6; w++) fc.f[w] = make_face(g, faces[w]);
-    hipLaunchKernelGGL(k_pp_rhs, rows(g.fm, g.fn, g.fo, 64), dim3(64), 0, s, g, U, gp, fc, nwall, r);
+    int nn = g.fm * g.fn * g.fo;
+    hipLaunchKernelGGL(k_pp_rhs, dim3((nn + 255) / 256), dim3(256), 0, s, g, U, gp, fc, nwall, r);
 }
 size_t face_struct_size() { return sizeof(FaceC) * 6; }
 void launch_pp_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x, const double *mean) {

@@ -58,3 +58,25 @@ def test_reference_sample_configs_parse():
     assert list(objs[0].basic) == [0.6, 0.5, 0.5, 0.2] and objs[0].extra[1] == 1.0
     for f in ("ftest.cfg", "stest.cfg", "lid_driven_cavity.cfg"):
         lib.parse_cfg(os.path.join(REF_CFG, f))
+
+
+@pytest.mark.parametrize("builder", [
+    lambda: configs.taylor_green(32), lambda: configs.cavity_sphere(48),
+    lambda: configs.stretched_sphere(64), lambda: configs.shear_spheres(64, 12),
+])
+def test_write_cfg_round_trip(builder, tmp_path):
+    cfg, objs = builder()
+    path = str(tmp_path / "w.cfg")
+    configs.write_cfg(cfg, objs, path, comment="round trip")
+    c2, o2 = lib.parse_cfg(path)
+    assert _fields(c2) == _fields(cfg)
+    assert len(o2) == len(objs)
+    for a, b in zip(o2, objs):
+        assert _fields(a) == _fields(b)
+
+
+def test_shipped_shear_cfg_is_the_builder_output():
+    cfg, objs = lib.parse_cfg(os.path.join(CFG_DIR, "shear_spheres_256.cfg"))
+    cb, ob = configs.shear_spheres(256, 64)
+    assert _fields(cfg) == _fields(cb)
+    assert [_fields(o) for o in objs] == [_fields(o) for o in ob]

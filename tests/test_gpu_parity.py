@@ -136,3 +136,21 @@ def test_graph_replay_matches_eager_launches(prec):
         s.close()
     for k in ("u", "v", "w", "p", "rm_x", "band_x"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_shear_spheres_match_oracle():
+    """Simple shear (moving walls in z, periodic in x and y) with several
+    spheres of different moduli."""
+    cfg, objs = configs.shear_spheres(32, 5, rmin=0.09, rmax=0.14, seed=4)
+    g, o = _init_pair(cfg, objs)
+    _, dtg, _ = g.prepare_run(1)
+    _, dto, _ = o.prepare_run(1)
+    assert dtg == dto
+    for s in range(4):
+        rg, ro = g.step(), o.step()
+        assert rg == ro == 0, (rg, ro)
+    r = compare_states(g.download(), o.download(), cfg)
+    print("shear spheres", len(objs), r)
+    assert r["rm_c_equal"] and r.get("band_equal", True)
+    for k in ("vel", "p"):
+        assert r[k] < TOL, (k, r[k])
