@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=16)
     return ap.parse_args()
@@ -172,8 +175,25 @@ def _rel(a, b):
     return float(d / n) if n > 0 else float(d)
 
 
+def calibrate_copy():
+    """A plain device copy of 1 GiB, five times: read 1 GiB and write 1 GiB
+    per launch.  Under rocprofv3 --pmc FETCH_SIZE (or WRITE_SIZE) its counter
+    reading per dispatch gives the factor from counter units to bytes."""
+    import torch
+    n = 1 << 30
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 5}), flush=True)
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs the FSI step on one GPU only; --gpus %d is not supported "

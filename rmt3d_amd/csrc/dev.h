@@ -180,6 +180,29 @@ __device__ __forceinline__ int wave_append(bool pred, int *counter) {
     return pred ? base + __popcll(bal & ((1ull << lane) - 1)) : -1;
 }
 
+// The same for a whole workgroup (blockDim a multiple of 64, at most 1024):
+// one atomic per workgroup.  Every thread of the workgroup must reach the
+// call; sh is __shared__ int[17].
+__device__ __forceinline__ int block_append(bool pred, int *counter, int *sh) {
+    unsigned long long bal = __ballot(pred);
+    int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) sh[w] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int a = 0; a < nw; a++) {
+            int c = sh[a];
+            sh[a] = run;
+            run += c;
+        }
+        sh[16] = run ? atomicAdd(counter, run) : 0;
+    }
+    __syncthreads();
+    int slot = pred ? sh[16] + sh[w] + __popcll(bal & ((1ull << lane) - 1)) : -1;
+    __syncthreads();
+    return slot;
+}
+
 __device__ __forceinline__ int symind(int i, int j) {
     if (i > j) { int t = j; j = i; i = t; }
     return j + i * (5 - i) / 2;

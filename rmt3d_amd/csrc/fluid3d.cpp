@@ -182,9 +182,7 @@ void MGSolver::cycle_body(hipStream_t s) {
         vcycle64(s, 0);
         return;
     }
-    launch_mg_resid(s, fem, L[0], prd);
-    launch_cast_d2f(s, L[0].r, Lf[0].b, L[0].nn);
-    launch_zero_f(s, Lf[0].x, Lf[0].nn);
+    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, Lf[0].x);
     vcycle32(s, 0);
     launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
 }

@@ -135,11 +135,11 @@ __global__ void k_scan_ghost(Geo g, Scratch W, uint32_t nmask) {
 }
 
 // update_list: the flagged cells become the next list (set semantics)
-__global__ void k_compact(Geo g, Scratch W, int *list, int *cnt) {
+__global__ __launch_bounds__(1024) void k_compact(Geo g, Scratch W, int *list, int *cnt) {
+    __shared__ int sh[17];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.Ng) return;
-    bool f = W.flag[t] != 0;
-    int p = wave_append(f, cnt);
+    bool f = t < g.Ng && W.flag[t] != 0;
+    int p = block_append(f, cnt, sh);
     if (f) {
         list[p] = t;
         W.flag[t] = 0;
@@ -388,11 +388,11 @@ __global__ void k_wrap_layer(Geo g, Scratch W, int L) {
 }
 
 // the object's layers become staged band entries
-__global__ void k_stage_obj(Geo g, Scratch W, uint32_t oid, int *status) {
+__global__ __launch_bounds__(1024) void k_stage_obj(Geo g, Scratch W, uint32_t oid, int *status) {
+    __shared__ int sh[17];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.Ng) return;
-    int L = W.lid[t];
-    int p = wave_append(L >= 1, W.st_n);
+    int L = t < g.Ng ? W.lid[t] : 0;
+    int p = block_append(L >= 1, W.st_n, sh);
     if (L < 1) return;
     if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); return; }
     W.st_cell[p] = t;
@@ -565,7 +565,7 @@ void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
         hipLaunchKernelGGL(k_scan_ghost, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
         int cur = 0;
         launch_zero_i32(s, W.cnt + cur, 1);
-        hipLaunchKernelGGL(k_compact, g1(g.Ng), dim3(256), 0, s, g, W, W.list[cur], W.cnt + cur);
+        hipLaunchKernelGGL(k_compact, dim3((g.Ng + 1023) / 1024), dim3(1024), 0, s, g, W, W.list[cur], W.cnt + cur);
         for (int L = 1; L <= nlayers; L++) {
             nmask++;
             LsqArgs A;
@@ -580,12 +580,12 @@ void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
                 hipLaunchKernelGGL(k_apply_flags, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
                 if (per) hipLaunchKernelGGL(k_comm_mask, g1(g.Ng), dim3(256), 0, s, g, W);
                 hipLaunchKernelGGL(k_scan_ghost, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
-                hipLaunchKernelGGL(k_compact, g1(g.Ng), dim3(256), 0, s, g, W, W.list[nxt], W.cnt + nxt);
+                hipLaunchKernelGGL(k_compact, dim3((g.Ng + 1023) / 1024), dim3(1024), 0, s, g, W, W.list[nxt], W.cnt + nxt);
             }
             cur = nxt;
             if (per) hipLaunchKernelGGL(k_wrap_layer, g1(g.Ng), dim3(256), 0, s, g, W, L);
         }
-        hipLaunchKernelGGL(k_stage_obj, g1(g.Ng), dim3(256), 0, s, g, W, (uint32_t)oid, status);
+        hipLaunchKernelGGL(k_stage_obj, dim3((g.Ng + 1023) / 1024), dim3(1024), 0, s, g, W, (uint32_t)oid, status);
     }
 }
 

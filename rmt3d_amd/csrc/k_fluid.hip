@@ -259,22 +259,20 @@ __device__ __forceinline__ void stress_face(const Geo &g, const Phys &P, const O
 }
 
 // fluid cells of padded 1..s+3 in place; solid cells appended to list
-__global__ void k_stress_fluid(Geo g, Phys P, Fluid U, Maps M, int *list, int *count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int j = blockIdx.y + 1, k = blockIdx.z + 1;
-    bool in = i <= g.m + 3;
-    int ind = in ? IX(g, i, j, k) : 0;
-    bool solid = in && stress_solid_cell(M, ind);
-    // one atomic per wavefront for the solid cells it holds
-    unsigned long long bal = __ballot(solid);
-    int lane = threadIdx.x & 63;
-    int base = 0;
-    if (bal) {
-        int first = __ffsll((long long)bal) - 1;
-        if (lane == first) base = atomicAdd(count, __popcll(bal));
-        base = __shfl(base, first, 64);
+// (one flat thread per cell, one atomic per workgroup for the appends)
+__global__ __launch_bounds__(1024) void k_stress_fluid(Geo g, Phys P, Fluid U, Maps M, int *list, int *count) {
+    __shared__ int sh[17];
+    int w0 = g.m + 3, w1 = g.n + 3, w2 = g.o + 3;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = t < w0 * w1 * w2;
+    int ind = 0;
+    if (in) {
+        int i = t % w0 + 1, j = (t / w0) % w1 + 1, k = t / (w0 * w1) + 1;
+        ind = IX(g, i, j, k);
     }
-    if (solid) list[base + __popcll(bal & ((1ull << lane) - 1))] = ind;
+    bool solid = in && stress_solid_cell(M, ind);
+    int slot = block_append(solid, count, sh);
+    if (solid) list[slot] = ind;
     if (!in || solid) return;
     const int str[3] = {1, g.m4, g.mn4};
     const double facs[3] = {g.dxsp, g.dysp, g.dzsp};
@@ -1044,7 +1042,8 @@ __global__ void k_zero1(int *p) {
 void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    int *list, int *count) {
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, count);
-    hipLaunchKernelGGL(k_stress_fluid, rows(g.m + 3, g.n + 3, g.o + 3, 64), dim3(64), 0, s, g, P, U, M, list, count);
+    int ncell = (g.m + 3) * (g.n + 3) * (g.o + 3);
+    hipLaunchKernelGGL(k_stress_fluid, dim3((ncell + 1023) / 1024), dim3(1024), 0, s, g, P, U, M, list, count);
     hipLaunchKernelGGL(k_stress_solid<0>, dim3(4096), dim3(64), 0, s, g, P, O, U, M, list, count);
     hipLaunchKernelGGL(k_stress_solid<1>, dim3(4096), dim3(64), 0, s, g, P, O, U, M, list, count);
     hipLaunchKernelGGL(k_stress_solid<2>, dim3(4096), dim3(64), 0, s, g, P, O, U, M, list, count);

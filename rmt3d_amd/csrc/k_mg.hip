@@ -150,19 +150,129 @@ __device__ __forceinline__ void fem_smooth_at(const MGLevel<T> &L, const Prd &P,
     L.x[c] = (L.b[c] - s) * rdg;
 }
 
-// four colours by (j,k) parity: one colour is a set of whole x-rows; one
-// thread per node of the colour, t -> (i, jj, kk)
+// Interior node (no wall, no wrap): the 20 non-zero neighbours at fixed
+// offsets, in fem_off's (k, j, i) order with its weights, so the sum is
+// bit-identical to the general path.
 template <typename T>
-__global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, int hy, float rd0, float rhy) {
+__device__ __forceinline__ T fem_off_interior(const T *x, int c, int D0, int D01) {
+    T s = 0;
+    const T *p = x + c - D01;
+    s += T(-1) * p[-D0 - 1]; s += T(-2) * p[-D0]; s += T(-1) * p[-D0 + 1];
+    s += T(-2) * p[-1];      s += T(-2) * p[1];
+    s += T(-1) * p[D0 - 1];  s += T(-2) * p[D0];  s += T(-1) * p[D0 + 1];
+    p = x + c;
+    s += T(-2) * p[-D0 - 1]; s += T(-2) * p[-D0 + 1];
+    s += T(-2) * p[D0 - 1];  s += T(-2) * p[D0 + 1];
+    p = x + c + D01;
+    s += T(-1) * p[-D0 - 1]; s += T(-2) * p[-D0]; s += T(-1) * p[-D0 + 1];
+    s += T(-2) * p[-1];      s += T(-2) * p[1];
+    s += T(-1) * p[D0 - 1];  s += T(-2) * p[D0];  s += T(-1) * p[D0 + 1];
+    return s;
+}
+
+// Work-item order of one FEM colour (rows j = jc mod 2, k = kc mod 2):
+//   [0, NI)          interior nodes of interior rows (fast path)
+//   [NI, NI+NA)      the two end nodes (i = 0, D0-1) of interior rows
+//   [NI+NA, +NB)     every node of the colour's boundary rows
+// so that a wavefront runs either the fast path or the general one.
+struct FemColour {
+    int plain;                // 1: no interior rows; t -> row t / D0 of all colour rows
+    int jI0, njI, kI0, nkI;   // interior rows: j = jI0 + 2a (a < njI), k = kI0 + 2b
+    int hyc;                  // rows of the colour per k: j = jc + 2a, a < hyc
+    int kc;
+    int nks, ks[2];           // boundary k values of the colour
+    int njs, js[2];           // boundary j values of the colour
+    int NI, NA, NB;
+};
+
+template <typename T>
+__device__ __forceinline__ bool fem_colour_node(const MGLevel<T> &L, const FemColour &F, int jc, int t, int &i, int &j,
+                                                int &k) {
+    int D0 = L.d[0];
+    if (F.plain) {
+        int row = t / D0;
+        i = t - row * D0;
+        int b = row / F.hyc;
+        j = jc + 2 * (row - b * F.hyc);
+        k = F.kc + 2 * b;
+        return false;
+    }
+    if (t < F.NI) {
+        int w = D0 - 2;
+        int row = t / w;
+        i = 1 + (t - row * w);
+        int b = row / F.njI;
+        j = F.jI0 + 2 * (row - b * F.njI);
+        k = F.kI0 + 2 * b;
+        return true;
+    }
+    t -= F.NI;
+    if (t < F.NA) {
+        int row = t >> 1;
+        i = (t & 1) ? D0 - 1 : 0;
+        int b = row / F.njI;
+        j = F.jI0 + 2 * (row - b * F.njI);
+        k = F.kI0 + 2 * b;
+        return false;
+    }
+    t -= F.NA;
+    int rb = t / D0;
+    i = t - rb * D0;
+    int n1 = F.nks * F.hyc;
+    if (rb < n1) {
+        int a = rb / F.hyc;
+        k = F.ks[a];
+        j = jc + 2 * (rb - a * F.hyc);
+    } else {
+        rb -= n1;
+        int a = rb / F.nkI;
+        j = F.js[a];
+        k = F.kI0 + 2 * (rb - a * F.nkI);
+    }
+    return false;
+}
+
+template <typename T>
+__global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, FemColour F) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int row = qdiv(t, L.d[0], rd0);
-    int i = t - row * L.d[0];
-    int kk = qdiv(row, hy, rhy);
-    int jj = row - kk * hy;
-    int j = 2 * jj + (color & 1);
-    int k = 2 * kk + ((color >> 1) & 1);
-    if (j >= L.d[1] || k >= L.d[2]) return;
-    fem_smooth_at(L, P, i, j, k);
+    if (t >= F.NI + F.NA + F.NB) return;
+    int i, j, k;
+    bool interior = fem_colour_node(L, F, color & 1, t, i, j, k);
+    int c = i + L.d[0] * (j + L.d[1] * k);
+    if (interior) {
+        T s = fem_off_interior(L.x, c, L.d[0], L.d[0] * L.d[1]);
+        L.x[c] = (L.b[c] - s) * T(0.03125);
+    } else {
+        fem_smooth_at(L, P, i, j, k);
+    }
+}
+
+// host side: the work-item layout of one colour of a level
+static FemColour fem_colour(const int d[3], int color) {
+    FemColour F{};
+    int jc = color & 1, kc = (color >> 1) & 1;
+    int D0 = d[0], D1 = d[1], D2 = d[2];
+    F.kc = kc;
+    F.hyc = (D1 - jc + 1) / 2;
+    int hzc = (D2 - kc + 1) / 2;
+    // interior = no wall and no wrap in that dimension
+    F.jI0 = jc ? 1 : 2;
+    F.njI = (F.jI0 <= D1 - 2) ? (D1 - 2 - F.jI0) / 2 + 1 : 0;
+    F.kI0 = kc ? 1 : 2;
+    F.nkI = (F.kI0 <= D2 - 2) ? (D2 - 2 - F.kI0) / 2 + 1 : 0;
+    if (D0 < 3 || F.njI == 0 || F.nkI == 0) {
+        F.plain = 1;
+        F.NB = D0 * F.hyc * hzc;
+        return F;
+    }
+    if (kc == 0) F.ks[F.nks++] = 0;
+    if ((D2 - 1) % 2 == kc) F.ks[F.nks++] = D2 - 1;
+    if (jc == 0) F.js[F.njs++] = 0;
+    if ((D1 - 1) % 2 == jc) F.js[F.njs++] = D1 - 1;
+    F.NI = (D0 - 2) * F.njI * F.nkI;
+    F.NA = 2 * F.njI * F.nkI;
+    F.NB = (F.nks * F.hyc + F.njs * F.nkI) * D0;
+    return F;
 }
 
 template <typename T>
@@ -188,29 +298,81 @@ __device__ __forceinline__ void resid_at(const MGLevel<T> &L, const Prd &P, int 
     int k = qdiv(row, D1, r1);                 \
     int j = row - k * (D1);
 
+// Node order of a residual pass: MAC levels flat; FEM levels the four colour
+// layouts one after the other, so interior nodes take the fast path.
+struct NodeMap {
+    int fem, total;
+    float r0, r1;            // MAC: reciprocals of D0, D1
+    int off[5];              // FEM: first work-item of each colour
+    FemColour F[4];
+};
+
+static NodeMap node_map(const int d[3], int fem) {
+    NodeMap M{};
+    M.fem = fem;
+    if (!fem) {
+        M.total = d[0] * d[1] * d[2];
+        M.r0 = 1.0f / d[0];
+        M.r1 = 1.0f / d[1];
+        return M;
+    }
+    M.off[0] = 0;
+    for (int c = 0; c < 4; c++) {
+        M.F[c] = fem_colour(d, c);
+        M.off[c + 1] = M.off[c] + M.F[c].NI + M.F[c].NA + M.F[c].NB;
+    }
+    M.total = M.off[4];
+    return M;
+}
+
+// returns true for an interior FEM node
 template <typename T>
-__global__ void k_resid(MGLevel<T> L, Prd P, int fem, float r0, float r1) {
-    FLAT_IJK(L.d[0], L.d[1], r0, r1)
-    if (k >= L.d[2]) return;
-    resid_at(L, P, fem, i, j, k);
+__device__ __forceinline__ bool map_node(const MGLevel<T> &L, const NodeMap &M, int t, int &i, int &j, int &k) {
+    if (!M.fem) {
+        int row = qdiv(t, L.d[0], M.r0);
+        i = t - row * L.d[0];
+        k = qdiv(row, L.d[1], M.r1);
+        j = row - k * L.d[1];
+        return false;
+    }
+    int c = (t >= M.off[1]) + (t >= M.off[2]) + (t >= M.off[3]);
+    return fem_colour_node(L, M.F[c], c & 1, t - M.off[c], i, j, k);
+}
+
+// r = b - A x at one node; the interior FEM form is fem_off's sum
+template <typename T>
+__device__ __forceinline__ T resid_value(const MGLevel<T> &L, const Prd &P, int fem, bool interior, int i, int j, int k,
+                                         int c) {
+    if (interior) {
+        T s = fem_off_interior(L.x, c, L.d[0], L.d[0] * L.d[1]);
+        return L.b[c] - (s + T(32) * L.x[c]);
+    }
+    T dg;
+    T s = apply_off(L, P, fem, i, j, k, c, dg);
+    return L.b[c] - (s + dg * L.x[c]);
+}
+
+template <typename T>
+__global__ void k_resid(MGLevel<T> L, Prd P, NodeMap M) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M.total) return;
+    int i, j, k;
+    bool in = map_node(L, M, t, i, j, k);
+    int c = i + L.d[0] * (j + L.d[1] * k);
+    L.r[c] = resid_value(L, P, M.fem, in, i, j, k, c);
 }
 
 // FP64 residual and its max norm: grid-stride blocks of 256, one atomic per
 // block (non-negative doubles order as their bit patterns; a NaN compares
 // above every finite value)
-__global__ void k_resid64_max(MGLevel<double> L, Prd P, int fem, unsigned long long *maxbits) {
+__global__ void k_resid64_max(MGLevel<double> L, Prd P, NodeMap M, unsigned long long *maxbits) {
     __shared__ unsigned long long sh[4];
     unsigned long long v = 0;
-    int D0 = L.d[0], D1 = L.d[1];
-    float rd0 = 1.0f / D0, rd1 = 1.0f / D1;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < L.nn; c += gridDim.x * blockDim.x) {
-        int row = qdiv(c, D0, rd0);
-        int i = c - row * D0;
-        int k = qdiv(row, D1, rd1);
-        int j = row - k * D1;
-        double dg;
-        double s = apply_off(L, P, fem, i, j, k, c, dg);
-        double r = L.b[c] - (s + dg * L.x[c]);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < M.total; t += gridDim.x * blockDim.x) {
+        int i, j, k;
+        bool in = map_node(L, M, t, i, j, k);
+        int c = i + L.d[0] * (j + L.d[1] * k);
+        double r = resid_value(L, P, M.fem, in, i, j, k, c);
         L.r[c] = r;
         unsigned long long b = (unsigned long long)__double_as_longlong(fabs(r));
         v = b > v ? b : v;
@@ -358,38 +520,53 @@ __device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
     __syncthreads();
 }
 
-// lds != 0: every tail level lives in LDS (x, b, r of each level packed in
-// that order); the finest tail level's x and b are loaded from HBM on entry
-// and its x stored back on exit.  lds == 0: the levels stay in HBM.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int vdown, int vup, int csweeps,
-                                               int lds) {
+// IN_LDS: every tail level lives in LDS (x, b, r of each level packed in
+// that order, at offsets kept in LDS); the finest tail level's x and b are
+// loaded from HBM on entry and its x stored back on exit.  The level views
+// are rebuilt from the shared base at each use, so the compiler sees LDS
+// pointers and issues LDS instructions.  Otherwise the levels stay in HBM.
+template <typename T, bool IN_LDS>
+__device__ __forceinline__ MGLevel<T> tail_level(const MGTail<T> &t, const int (*dims)[4], const int *offs,
+                                                 unsigned char *smem, int l) {
+    MGLevel<T> L;
+    L.d[0] = dims[l][0]; L.d[1] = dims[l][1]; L.d[2] = dims[l][2]; L.nn = dims[l][3];
+    if (IN_LDS) {
+        T *base = (T *)smem + offs[l];
+        L.x = base; L.b = base + L.nn; L.r = base + 2 * L.nn;
+    } else {
+        L.x = t.L[l].x; L.b = t.L[l].b; L.r = t.L[l].r;
+    }
+    return L;
+}
+
+template <typename T, bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int vdown, int vup, int csweeps) {
     extern __shared__ unsigned char smem[];
-    __shared__ MGLevel<T> lv[MG_TAIL_MAXLEV];
+    __shared__ int dims[MG_TAIL_MAXLEV][4];
+    __shared__ int offs[MG_TAIL_MAXLEV];
     int ncol = fem ? 4 : 2;
     int last = t.n - 1;
     if (threadIdx.x == 0) {
-        T *p = (T *)smem;
+        int o = 0;
         for (int l = 0; l < t.n; l++) {
-            lv[l] = t.L[l];
-            if (lds) {
-                int nn = t.L[l].nn;
-                lv[l].x = p; lv[l].b = p + nn; lv[l].r = p + 2 * nn;
-                p += 3 * nn;
-            }
+            for (int a = 0; a < 3; a++) dims[l][a] = t.L[l].d[a];
+            dims[l][3] = t.L[l].nn;
+            offs[l] = o;
+            o += 3 * t.L[l].nn;
         }
     }
     __syncthreads();
-    if (lds) {
-        for (int e = threadIdx.x; e < lv[0].nn; e += blockDim.x) {
-            lv[0].x[e] = t.L[0].x[e];
-            lv[0].b[e] = t.L[0].b[e];
+    if (IN_LDS) {
+        MGLevel<T> L0 = tail_level<T, true>(t, dims, offs, smem, 0);
+        for (int e = threadIdx.x; e < L0.nn; e += blockDim.x) {
+            L0.x[e] = t.L[0].x[e];
+            L0.b[e] = t.L[0].b[e];
         }
         __syncthreads();
     }
     for (int l = 0; l < last; l++) {
-        const MGLevel<T> L = lv[l];
-        const MGLevel<T> C = lv[l + 1];
+        const MGLevel<T> L = tail_level<T, IN_LDS>(t, dims, offs, smem, l);
+        const MGLevel<T> C = tail_level<T, IN_LDS>(t, dims, offs, smem, l + 1);
         for (int sw = 0; sw < vdown; sw++)
             for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
         float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
@@ -408,7 +585,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
         __syncthreads();
     }
     {
-        const MGLevel<T> W = lv[last];
+        const MGLevel<T> W = tail_level<T, IN_LDS>(t, dims, offs, smem, last);
         float ra = 1.0f / W.d[0], rb = 1.0f / W.d[1];
         for (int sw = 0; sw < csweeps; sw++)
             for (int c = 0; c < ncol; c++) {
@@ -424,8 +601,8 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
             }
     }
     for (int l = last - 1; l >= 0; l--) {
-        const MGLevel<T> L = lv[l];
-        const MGLevel<T> C = lv[l + 1];
+        const MGLevel<T> L = tail_level<T, IN_LDS>(t, dims, offs, smem, l);
+        const MGLevel<T> C = tail_level<T, IN_LDS>(t, dims, offs, smem, l + 1);
         float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
         for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
             int i, j, k;
@@ -436,8 +613,22 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
         for (int sw = 0; sw < vup; sw++)
             for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
     }
-    if (lds)
-        for (int e = threadIdx.x; e < lv[0].nn; e += blockDim.x) t.L[0].x[e] = lv[0].x[e];
+    if (IN_LDS) {
+        MGLevel<T> L0 = tail_level<T, true>(t, dims, offs, smem, 0);
+        for (int e = threadIdx.x; e < L0.nn; e += blockDim.x) t.L[0].x[e] = L0.x[e];
+    }
+}
+
+// mixed defect correction, first half: FP64 residual of the FP64 iterate
+// straight into the FP32 right-hand side, FP32 correction zeroed
+__global__ void k_resid_to_f32(MGLevel<double> L, Prd P, NodeMap M, float *bf, float *xf) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M.total) return;
+    int i, j, k;
+    bool in = map_node(L, M, t, i, j, k);
+    int c = i + L.d[0] * (j + L.d[1] * k);
+    bf[c] = (float)resid_value(L, P, M.fem, in, i, j, k, c);
+    xf[c] = 0.f;
 }
 
 __global__ void k_cast_d2f(const double *a, float *b, int n) {
@@ -463,23 +654,23 @@ void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd
         hipLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, hx, 1.0f / hx,
                            1.0f / L.d[1]);
     } else {
-        int hy = (L.d[1] + 1) / 2, hz = (L.d[2] + 1) / 2, n = L.d[0] * hy * hz;
-        hipLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, hy,
-                           1.0f / L.d[0], 1.0f / hy);
+        FemColour F = fem_colour(L.d, color);
+        int n = F.NI + F.NA + F.NB;
+        hipLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, F);
     }
 }
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
 
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]) {
-    hipLaunchKernelGGL(k_resid<T>, flat_grid(L.nn), dim3(256), 0, s, L, mkprd(prd), fem, 1.0f / L.d[0],
-                       1.0f / L.d[1]);
+    NodeMap M = node_map(L.d, fem);
+    hipLaunchKernelGGL(k_resid<T>, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M);
 }
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits) {
     int nb = (L.nn + 255) / 256;
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_resid64_max, dim3(nb), dim3(256), 0, s, L, mkprd(prd), fem, maxbits);
+    hipLaunchKernelGGL(k_resid64_max, dim3(nb), dim3(256), 0, s, L, mkprd(prd), node_map(L.d, fem), maxbits);
 }
 template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
@@ -503,9 +694,9 @@ int mg_tail_enable_lds(size_t bytes) {
     int dev = 0, maxb = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-    if (bytes + sizeof(MGLevel<T>) * MG_TAIL_MAXLEV > (size_t)maxb) return 0;
+    if (bytes + sizeof(int) * 5 * MG_TAIL_MAXLEV > (size_t)maxb) return 0;
     // raising the dynamic limit is required on some runtimes and a no-op on others
-    (void)hipFuncSetAttribute((const void *)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute((const void *)k_tail<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     (void)hipGetLastError();
     return 1;
 }
@@ -516,8 +707,15 @@ void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const in
     MGTail<T> t;
     t.n = n;
     for (int l = 0; l < n; l++) t.L[l] = L[l];
-    size_t bytes = lds ? mg_tail_lds_bytes(L, n) : 0;
-    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), bytes, s, t, mkprd(prd), fem, vdown, vup, csweeps, lds);
+    if (lds)
+        hipLaunchKernelGGL((k_tail<T, true>), dim3(1), dim3(1024), mg_tail_lds_bytes(L, n), s, t, mkprd(prd), fem, vdown,
+                           vup, csweeps);
+    else
+        hipLaunchKernelGGL((k_tail<T, false>), dim3(1), dim3(1024), 0, s, t, mkprd(prd), fem, vdown, vup, csweeps);
+}
+void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf) {
+    NodeMap M = node_map(L.d, fem);
+    hipLaunchKernelGGL(k_resid_to_f32, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M, bf, xf);
 }
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n) {
     hipLaunchKernelGGL(k_cast_d2f, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);

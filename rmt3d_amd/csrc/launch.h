@@ -89,6 +89,7 @@ int mg_tail_enable_lds(size_t bytes);
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits);
 // mixed precision helpers
+void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf);
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n);
 void launch_axpy_f2d(hipStream_t s, double *x, const float *e, int n);
 void launch_zero_f(hipStream_t s, float *a, int n);
