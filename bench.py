@@ -54,7 +54,7 @@ def parse():
                     help="both: time the FP64 path, then the mixed path; value is the mixed path")
     ap.add_argument("--kernel-steps", type=int, default=3,
                     help="steps of the per-kernel event window after the timed region")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -149,23 +149,26 @@ def run_gpu(args, precision):
 
 def run_cpu(args):
     import multiprocessing
-    from oracle.oracle import OracleSim
+    from oracle.oracle import OracleSim, lib
     from rmt3d_amd import configs
     cfg, objs = configs.stretched_sphere(args.n)
     S = OracleSim(cfg, objs)
     e = S.initialize()
-    S.prepare_run(0)
+    S.prepare_run(e)
     t0 = time.perf_counter()
     for _ in range(args.cpu_steps):
         S.step()
     t1 = time.perf_counter()
     m, n, o = cfg.sys_size
     S.close()
-    return {"value": m * n * o * args.cpu_steps / (t1 - t0), "unit": "cell-steps/s", "cores": 1,
+    threads = lib().oracle_threads()
+    return {"value": m * n * o * args.cpu_steps / (t1 - t0), "unit": "cell-steps/s", "cores": threads,
             "kind": "port",
-            "sample": "%d step(s) of the %d^3 stretched-sphere workload after initialize(), FP64 oracle "
-                      "(plain-C restatement of the reference step), one thread; host has %d logical CPUs"
-                      % (args.cpu_steps, args.n, multiprocessing.cpu_count())}
+            "sample": "%d step(s) of the %d^3 stretched-sphere workload after initialize() and the init_iter "
+                      "warm-up, FP64 oracle (plain-C restatement of the reference step) with OpenMP over %d "
+                      "threads on its per-cell and per-colour loops (extrapolation, upwinding and FEM rhs "
+                      "serial); host reports %d logical CPUs"
+                      % (args.cpu_steps, args.n, threads, multiprocessing.cpu_count())}
 
 
 def _rel(a, b):

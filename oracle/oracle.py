@@ -26,7 +26,9 @@ def build(force=False):
         newest = max(os.path.getmtime(s) for s in srcs + [os.path.join(HERE, "orc.h"), os.path.join(HERE, "orc_detmath.h")])
         if os.path.getmtime(LIB_PATH) >= newest:
             return LIB_PATH
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-D_GNU_SOURCE",
+    # OpenMP only on loops whose iterations write disjoint outputs (per cell,
+    # per node of one colour), so results do not depend on the thread count
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-D_GNU_SOURCE", "-fopenmp",
            "-ffp-contract=off", "-Wall", "-o", LIB_PATH] + srcs + ["-lm"]
     subprocess.check_call(cmd)
     return LIB_PATH
@@ -118,6 +120,7 @@ def lib():
         L.oracle_det_cos.argtypes = [D]
         L.oracle_det_cos.restype = D
         L.oracle_mg_set_coarse_sweeps.argtypes = [P, C.c_int]
+        L.oracle_threads.restype = C.c_int
         L.oracle_heaviside.argtypes = [P, D]
         L.oracle_heaviside.restype = D
         L.oracle_lsq.argtypes = [C.c_int, dp, dp, dp, D, dp, dp, dp, dp]

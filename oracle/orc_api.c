@@ -6,6 +6,9 @@
  */
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "orc.h"
 #include "orc_detmath.h"
 
@@ -168,3 +171,13 @@ void oracle_mg_set_coarse_sweeps(orc_sim *S, int n) { S->mac.coarse_sweeps = n; 
 /* the deterministic sine/cosine shared with the HIP path */
 double oracle_det_sin(double x) { return det_sin(x); }
 double oracle_det_cos(double x) { return det_cos(x); }
+
+/* threads the oracle's parallel loops use (OpenMP; 1 without it) */
+int oracle_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -509,6 +509,7 @@ static void normal_derivatives_velocity(orc_sim *S)
 {
     int str[3] = {1, S->m4, S->mn4};
     double dhsp[3] = {S->dxsp, S->dysp, S->dzsp};
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < S->o; k++)
         for (int j = 0; j < S->n; j++)
             for (int i = 0; i < S->m; i++) {
@@ -639,6 +640,7 @@ static void half_time_edge_velocities(orc_sim *S)
 {
     double dh2[3] = {S->dx / 2, S->dy / 2, S->dz / 2};
     double dt2 = S->dt / 2.0;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < S->o; k++)
         for (int j = 0; j < S->n; j++)
             for (int i = 0; i < S->m; i++) {
@@ -678,6 +680,7 @@ static void refmap_godunov_extrap(const orc_sim *S, orc_rmap *r, const orc_field
 /* fluid_3d::compute_half_time_edge_refmap (fluid_3d.cc:1071-1100) */
 static void half_time_edge_refmap(orc_sim *S)
 {
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < S->o; k++)
         for (int j = 0; j < S->n; j++)
             for (int i = 0; i < S->m; i++) {
@@ -1242,6 +1245,7 @@ void orc_compute_stress(orc_sim *S)
 {
     int str[3] = {1, S->m4, S->mn4};
     double facs[3] = {S->dxsp, S->dysp, S->dzsp};
+#pragma omp parallel for schedule(static)
     for (int k = -1; k <= S->o + 1; k++)
         for (int j = -1; j <= S->n + 1; j++)
             for (int i = -1; i <= S->m + 1; i++) {
@@ -1265,6 +1269,7 @@ void orc_compute_stress(orc_sim *S)
 static void compute_ustar(orc_sim *S)
 {
     double cdt = S->dt;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < S->o; k++)
         for (int j = 0; j < S->n; j++)
             for (int i = 0; i < S->m; i++) {

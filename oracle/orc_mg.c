@@ -173,6 +173,7 @@ static inline double fem_Ax(const orc_mglev *L, const int *prd, const double *x,
 void orc_mg_apply(const orc_mg *G, int lev, const double *x, double *y)
 {
     const orc_mglev *L = &G->lev[lev];
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->d[2]; k++)
         for (int j = 0; j < L->d[1]; j++)
             for (int i = 0; i < L->d[0]; i++)
@@ -183,6 +184,7 @@ void orc_mg_apply(const orc_mg *G, int lev, const double *x, double *y)
 static void residual(orc_mg *G, int lev)
 {
     orc_mglev *L = &G->lev[lev];
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->d[2]; k++)
         for (int j = 0; j < L->d[1]; j++)
             for (int i = 0; i < L->d[0]; i++) {
@@ -213,7 +215,9 @@ static void smooth(orc_mg *G, int lev)
 {
     orc_mglev *L = &G->lev[lev];
     int ncol = G->fem ? 4 : 2;
+    /* the nodes of one colour are not coupled: any order, any thread */
     for (int col = 0; col < ncol; col++)
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < L->d[2]; k++)
             for (int j = 0; j < L->d[1]; j++)
                 for (int i = 0; i < L->d[0]; i++) {
@@ -233,6 +237,7 @@ static void smooth(orc_mg *G, int lev)
 static void restrict_res(orc_mg *G, int lev)
 {
     orc_mglev *F = &G->lev[lev], *C = &G->lev[lev + 1];
+#pragma omp parallel for schedule(static)
     for (int K = 0; K < C->d[2]; K++)
         for (int J = 0; J < C->d[1]; J++)
             for (int I = 0; I < C->d[0]; I++) {
@@ -289,6 +294,7 @@ static inline void vx_w(int f, int D, int prd, int *i0, int *i1, double *w0, dou
 static void prolong_add(orc_mg *G, int lev)
 {
     orc_mglev *F = &G->lev[lev], *C = &G->lev[lev + 1];
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < F->d[2]; k++) {
         int k0, k1; double wk0, wk1;
         if (G->fem) vx_w(k, C->d[2], G->prd[2], &k0, &k1, &wk0, &wk1);

@@ -863,7 +863,7 @@ int Sim::step(int debug) {
     launch_stress(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n);
     mark_end(3);
     mark(5);
-    launch_ustar(stream, g, P, d_obj, U, M);
+    launch_ustar(stream, g, P, d_obj, U, M, d_solid_list, d_solid_n);
     mark_end(5);
     mark(2);
     fill_bc_cc();

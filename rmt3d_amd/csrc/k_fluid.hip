@@ -365,7 +365,9 @@ __device__ __forceinline__ void fluid_acc(const Phys &P, double x, double y, dou
 // the manufactured-solution forcing (ftest, mills); without it the fluid
 // acceleration is (0, 0, gravity for objects), computed by the same
 // operations, and the kernel does not carry the forcing's trigonometry.
-template <bool FORCING>
+// SOLID = false: the cell holds no map (no primary, no band entry), so the
+// solid acceleration is zero; the same additions of zero are kept.
+template <bool FORCING, bool SOLID = true>
 __device__ __forceinline__ void acceleration(const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                                              int c, double x, double y, double z, double acc[3]) {
     for (int a = 0; a < 3; a++)
@@ -376,7 +378,7 @@ __device__ __forceinline__ void acceleration(const Geo &g, const Phys &P, const 
     for (int a = 0; a < 3; a++) acc[a] += tmp[a];
     tmp[0] = tmp[1] = tmp[2] = 0;
     double sfrac = 0;
-    if (M.r.c[c] != 0 || M.b.cnt[c] > 0) {
+    if (SOLID && (M.r.c[c] != 0 || M.b.cnt[c] > 0)) {
         for (int s = 0; s < P.n_obj; s++) {
             int rp = M.get(c, (unsigned)(s + 1));
             if (rp < 0) continue;
@@ -452,13 +454,10 @@ __device__ __forceinline__ bool abut_primary(const Geo &g, const Maps &M, unsign
     return false;
 }
 
-// interior cells, fluid face states (one thread per cell, flat index)
-template <bool FORCING>
-__global__ void k_advect(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.m * g.n * g.o) return;
-    int i = t % g.m, j = (t / g.m) % g.n, k = t / (g.m * g.n);
-    int c = IX(g, i + 2, j + 2, k + 2);
+// fluid face states of interior cell c (fluid_3d.cc:582-617, 980-1066)
+template <bool FORCING, bool SOLID>
+__device__ __forceinline__ void advect_cell(const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                                            int c, int i, int j, int k) {
     const int str[3] = {1, g.m4, g.mn4};
     const double dhsp[3] = {g.dxsp, g.dysp, g.dzsp};
     double vel[3] = {U.vel[0][c], U.vel[1][c], U.vel[2][c]};
@@ -471,7 +470,7 @@ __global__ void k_advect(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
         }
     }
     double acc[3] = {0, 0, 0};
-    acceleration<FORCING>(g, P, O, U, M, c, g.lx[i + 2], g.ly[j + 2], g.lz[k + 2], acc);
+    acceleration<FORCING, SOLID>(g, P, O, U, M, c, g.lx[i + 2], g.ly[j + 2], g.lz[k + 2], acc);
     const double dh2[3] = {g.dx / 2, g.dy / 2, g.dz / 2};
     double dt2 = P.dt / 2.0;
     double adv[3] = {0, 0, 0};
@@ -486,16 +485,31 @@ __global__ void k_advect(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
     }
 }
 
-// interior cells of the solid-path list (primary map or band entries):
-// reference-map normal derivatives and face states
-// (normal_derivatives_refmap + compute_half_time_edge_refmap); they read only
-// the cell-centred velocity, not the fluid face states
-__global__ void k_advect_maps(Geo g, Phys P, Fluid U, Maps M, const int *list, const int *count, int *status) {
+// interior cells without maps (one thread per cell, flat index); the cells
+// with maps are the solid-path list's (k_advect_solid)
+template <bool FORCING>
+__global__ void k_advect(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.m * g.n * g.o) return;
+    int i = t % g.m, j = (t / g.m) % g.n, k = t / (g.m * g.n);
+    int c = IX(g, i + 2, j + 2, k + 2);
+    if (stress_solid_cell(M, c)) return;
+    advect_cell<FORCING, false>(g, P, O, U, M, c, i, j, k);
+}
+
+// interior cells of the solid-path list: fluid face states with the solid
+// acceleration, then their maps' normal derivatives and face states
+// (normal_derivatives_refmap + compute_half_time_edge_refmap), which read
+// only the cell-centred velocity
+template <bool FORCING>
+__global__ void k_advect_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count,
+                               int *status) {
     int n = *count;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         int c = list[t];
         int i = c % g.m4 - 2, j = (c / g.m4) % g.n4 - 2, k = c / g.mn4 - 2;
         if (i < 0 || i >= g.m || j < 0 || j >= g.n || k < 0 || k >= g.o) continue;
+        advect_cell<FORCING, true>(g, P, O, U, M, c, i, j, k);
         double rn[3][3];
         if ((M.r.c[c] & OMASK) > 0) {
             if (!refmap_nvd(g, M, c, c, i, j, k, rn)) { atomicOr(status, ST_FATAL_NVD); continue; }
@@ -770,14 +784,11 @@ __global__ void k_sub_qgrad(Geo g, Fluid U, double f2) {
 // ---------------------------------------------------------------------------
 // intermediate velocity (fluid_3d.cc:1309-1352)
 // ---------------------------------------------------------------------------
-template <bool FORCING>
-__global__ void k_ustar(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m) return;
-    int c = IX(g, i + 2, j + 2, k + 2);
+template <bool FORCING, bool SOLID>
+__device__ __forceinline__ void ustar_cell(const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                                           int c, int i, int j, int k) {
     double acc[3] = {0, 0, 0};
-    acceleration<FORCING>(g, P, O, U, M, c, g.lx[i + 2], g.ly[j + 2], g.lz[k + 2], acc);
+    acceleration<FORCING, SOLID>(g, P, O, U, M, c, g.lx[i + 2], g.ly[j + 2], g.lz[k + 2], acc);
     double cdt = P.dt;
     double fv[18];
     for (int q = 0; q < 18; q++) fv[q] = U.fvel[q][c];
@@ -796,6 +807,28 @@ __global__ void k_ustar(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
         dd += dv[d];
         U.dvel[d][c] = dd;
         U.vel[d][c] += dd;
+    }
+}
+
+// interior cells without maps; the others are the solid-path list's
+template <bool FORCING>
+__global__ void k_ustar(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.m * g.n * g.o) return;
+    int i = t % g.m, j = (t / g.m) % g.n, k = t / (g.m * g.n);
+    int c = IX(g, i + 2, j + 2, k + 2);
+    if (stress_solid_cell(M, c)) return;
+    ustar_cell<FORCING, false>(g, P, O, U, M, c, i, j, k);
+}
+
+template <bool FORCING>
+__global__ void k_ustar_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
+    int n = *count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int c = list[t];
+        int i = c % g.m4 - 2, j = (c / g.m4) % g.n4 - 2, k = c / g.mn4 - 2;
+        if (i < 0 || i >= g.m || j < 0 || j >= g.n || k < 0 || k >= g.o) continue;
+        ustar_cell<FORCING, true>(g, P, O, U, M, c, i, j, k);
     }
 }
 
@@ -1051,11 +1084,13 @@ void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, co
 void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    const int *list, const int *count, int *status) {
     int n = g.m * g.n * g.o;
-    if (P.sim_type == 0 || P.sim_type == 3)
+    if (P.sim_type == 0 || P.sim_type == 3) {
         hipLaunchKernelGGL(k_advect<true>, dim3((n + 255) / 256), dim3(256), 0, s, g, P, O, U, M);
-    else
+        hipLaunchKernelGGL(k_advect_solid<true>, dim3(2048), dim3(64), 0, s, g, P, O, U, M, list, count, status);
+    } else {
         hipLaunchKernelGGL(k_advect<false>, dim3((n + 255) / 256), dim3(256), 0, s, g, P, O, U, M);
-    hipLaunchKernelGGL(k_advect_maps, dim3(2048), dim3(64), 0, s, g, P, U, M, list, count, status);
+        hipLaunchKernelGGL(k_advect_solid<false>, dim3(2048), dim3(64), 0, s, g, P, O, U, M, list, count, status);
+    }
 }
 void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status) {
     hipLaunchKernelGGL(k_upwind, rows(g.m + 1, g.n + 1, g.o + 1, 64), dim3(64), 0, s, g, U, M, status);
@@ -1111,11 +1146,16 @@ void launch_mac_extract(hipStream_t s, const Geo &g, const Fluid &U, const doubl
 void launch_sub_qgrad(hipStream_t s, const Geo &g, const Fluid &U, double f2) {
     hipLaunchKernelGGL(k_sub_qgrad, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, U, f2);
 }
-void launch_ustar(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M) {
-    if (P.sim_type == 0 || P.sim_type == 3)
-        hipLaunchKernelGGL(k_ustar<true>, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, P, O, U, M);
-    else
-        hipLaunchKernelGGL(k_ustar<false>, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, P, O, U, M);
+void launch_ustar(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                  const int *list, const int *count) {
+    int n = g.m * g.n * g.o;
+    if (P.sim_type == 0 || P.sim_type == 3) {
+        hipLaunchKernelGGL(k_ustar<true>, dim3((n + 255) / 256), dim3(256), 0, s, g, P, O, U, M);
+        hipLaunchKernelGGL(k_ustar_solid<true>, dim3(2048), dim3(64), 0, s, g, P, O, U, M, list, count);
+    } else {
+        hipLaunchKernelGGL(k_ustar<false>, dim3((n + 255) / 256), dim3(256), 0, s, g, P, O, U, M);
+        hipLaunchKernelGGL(k_ustar_solid<false>, dim3(2048), dim3(64), 0, s, g, P, O, U, M, list, count);
+    }
 }
 void launch_pp_rhs(hipStream_t s, const Geo &g, const Fluid &U, const double gradphi[8][3], const FaceSpec *faces,
                    int nwall, void *faces_dev, double *r) {

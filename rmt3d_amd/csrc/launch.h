@@ -35,7 +35,9 @@ void launch_periodic_band_fx(hipStream_t s, const Geo &g, const Band &B);
 void launch_mac_rhs(hipStream_t s, const Geo &g, const Fluid &U, double *b, double fac);
 void launch_mac_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x);
 void launch_sub_qgrad(hipStream_t s, const Geo &g, const Fluid &U, double f2);
-void launch_ustar(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M);
+// list/count: the solid-path cells gathered by the launch_stress just before
+void launch_ustar(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
+                  const int *list, const int *count);
 void launch_pp_rhs(hipStream_t s, const Geo &g, const Fluid &U, const double gradphi[8][3], const FaceSpec *faces,
                    int nwall, void *faces_dev, double *r);
 size_t face_struct_size();
