@@ -58,6 +58,7 @@ struct Fluid {
 
 struct RMap {
     uint32_t *c;
+    double *phi;         // level-set value of x for its object (NaN: no object), per stress pass
     double *x[3], *xpred[3];
     double *fx[18];
     double *gphi[9];
@@ -69,6 +70,7 @@ struct Band {
     int *cnt;            // Ng
     int *cell;           // cap
     uint32_t *c;         // cap
+    double *phi;         // cap, as RMap::phi
     double *x[3], *xpred[3];
     double *fx[18];
     double *gphi[9];
@@ -106,6 +108,9 @@ struct Maps {
     }
     __device__ __forceinline__ uint32_t c(int ref) const {
         return ref < Ng ? r.c[ref] : b.c[ref - Ng];
+    }
+    __device__ __forceinline__ double phi(int ref) const {
+        return ref < Ng ? r.phi[ref] : b.phi[ref - Ng];
     }
     // fluid_3d::get_refmap (fluid_3d_utils.cc:31-42)
     __device__ __forceinline__ int get(int ind, unsigned oid) const {

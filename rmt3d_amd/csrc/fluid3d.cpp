@@ -314,6 +314,7 @@ void Sim::alloc_band(Band &B, int cap) {
     B.cnt = (int *)dalloc(sizeof(int) * Ng);
     B.cell = (int *)dalloc(sizeof(int) * cap);
     B.c = (uint32_t *)dalloc(sizeof(uint32_t) * cap);
+    B.phi = (double *)dalloc(sizeof(double) * cap);
     for (int a = 0; a < 3; a++) {
         B.x[a] = (double *)dalloc(sizeof(double) * cap);
         B.xpred[a] = (double *)dalloc(sizeof(double) * cap);
@@ -542,6 +543,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     for (int a = 0; a < 18; a++) U.fvel[a] = (double *)dalloc(sizeof(double) * Ng);
     for (int a = 0; a < 9; a++) U.sigma[a] = (double *)dalloc(sizeof(double) * Ng);
     R.c = (uint32_t *)dalloc(sizeof(uint32_t) * Ng);
+    R.phi = (double *)dalloc(sizeof(double) * Ng);
     for (int a = 0; a < 3; a++) {
         R.x[a] = (double *)dalloc(sizeof(double) * Ng);
         R.xpred[a] = (double *)dalloc(sizeof(double) * Ng);

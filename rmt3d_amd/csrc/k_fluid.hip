@@ -72,7 +72,7 @@ __device__ __forceinline__ void solid_stress(const Geo &g, const Phys &P, const 
         q[F] -= 1;
         int rn = get_pos(g, M, ind - strd, oid, q[0], q[1], q[2]);
         if (rn < 0) continue;
-        double phr = map_phi(M, O, P.n_obj, rf), phn = map_phi(M, O, P.n_obj, rn);
+        double phr = M.phi(rf), phn = M.phi(rn);
         double phiv = 0.5 * (phr + phn);
         if (phiv > P.eps) continue;
         double tf = heaviside(P, phiv);
@@ -84,11 +84,11 @@ __device__ __forceinline__ void solid_stress(const Geo &g, const Phys &P, const 
         double gp[3];
         gp[F] = facs[F] * (phr - phn);
         gp[ad1] = 0.25 * facs[ad1] *
-                  (map_phi(M, O, P.n_obj, n.r1p) + map_phi(M, O, P.n_obj, n.r1p1) - map_phi(M, O, P.n_obj, n.r1m) -
-                   map_phi(M, O, P.n_obj, n.r1m1));
+                  (M.phi(n.r1p) + M.phi(n.r1p1) - M.phi(n.r1m) -
+                   M.phi(n.r1m1));
         gp[ad2] = 0.25 * facs[ad2] *
-                  (map_phi(M, O, P.n_obj, n.r2p) + map_phi(M, O, P.n_obj, n.r2p1) - map_phi(M, O, P.n_obj, n.r2m) -
-                   map_phi(M, O, P.n_obj, n.r2m1));
+                  (M.phi(n.r2p) + M.phi(n.r2p1) - M.phi(n.r2m) -
+                   M.phi(n.r2m1));
         {
             double nr = 0;
             for (int a = 0; a < 3; a++) nr += gp[a] * gp[a];
@@ -170,18 +170,18 @@ __device__ __forceinline__ void collision_stress(const Geo &g, const Phys &P, co
         double G0 = O[o0 - 1].G;
         int rn0 = get_pos(g, M, ind - strd, o0, lp[0] - (F == 0), lp[1] - (F == 1), lp[2] - (F == 2));
         if (rn0 < 0) continue;
-        double phiv0 = 0.5 * (map_phi(M, O, P.n_obj, rf0) + map_phi(M, O, P.n_obj, rn0));
+        double phiv0 = 0.5 * (M.phi(rf0) + M.phi(rn0));
         if (phiv0 > P.eps) continue;
         Nb8 n;
         if (!nbr_set<F>(g, M, ind, o0, lp, n)) continue;
         double d0[3];
-        d0[F] = facs[F] * (map_phi(M, O, P.n_obj, rf0) - map_phi(M, O, P.n_obj, rn0));
+        d0[F] = facs[F] * (M.phi(rf0) - M.phi(rn0));
         d0[ad1] = 0.25 * facs[ad1] *
-                  (map_phi(M, O, P.n_obj, n.r1p) + map_phi(M, O, P.n_obj, n.r1p1) - map_phi(M, O, P.n_obj, n.r1m) -
-                   map_phi(M, O, P.n_obj, n.r1m1));
+                  (M.phi(n.r1p) + M.phi(n.r1p1) - M.phi(n.r1m) -
+                   M.phi(n.r1m1));
         d0[ad2] = 0.25 * facs[ad2] *
-                  (map_phi(M, O, P.n_obj, n.r2p) + map_phi(M, O, P.n_obj, n.r2p1) - map_phi(M, O, P.n_obj, n.r2m) -
-                   map_phi(M, O, P.n_obj, n.r2m1));
+                  (M.phi(n.r2p) + M.phi(n.r2p1) - M.phi(n.r2m) -
+                   M.phi(n.r2m1));
         double nr = 0;
         for (int a = 0; a < 3; a++) nr += d0[a] * d0[a];
         if (nr > 0) {
@@ -193,18 +193,18 @@ __device__ __forceinline__ void collision_stress(const Geo &g, const Phys &P, co
             unsigned o1 = M.c(rf1) & OMASK;
             int rn1 = M.get(ind - strd, o1);
             if (rn1 < 0) continue;
-            double phiv1 = 0.5 * (map_phi(M, O, P.n_obj, rf1) + map_phi(M, O, P.n_obj, rn1));
+            double phiv1 = 0.5 * (M.phi(rf1) + M.phi(rn1));
             if (phiv1 > P.eps || phiv0 + phiv1 > P.eps) continue;
             Nb8 s;
             if (!nbr_set<F>(g, M, ind, o1, lp, s)) continue;
             double d1[3];
-            d1[F] = facs[F] * (map_phi(M, O, P.n_obj, rf1) - map_phi(M, O, P.n_obj, rn1));
+            d1[F] = facs[F] * (M.phi(rf1) - M.phi(rn1));
             d1[ad1] = 0.25 * facs[ad1] *
-                      (map_phi(M, O, P.n_obj, s.r1p) + map_phi(M, O, P.n_obj, s.r1p1) -
-                       map_phi(M, O, P.n_obj, s.r1m) - map_phi(M, O, P.n_obj, s.r1m1));
+                      (M.phi(s.r1p) + M.phi(s.r1p1) -
+                       M.phi(s.r1m) - M.phi(s.r1m1));
             d1[ad2] = 0.25 * facs[ad2] *
-                      (map_phi(M, O, P.n_obj, s.r2p) + map_phi(M, O, P.n_obj, s.r2p1) -
-                       map_phi(M, O, P.n_obj, s.r2m) - map_phi(M, O, P.n_obj, s.r2m1));
+                      (M.phi(s.r2p) + M.phi(s.r2p1) -
+                       M.phi(s.r2m) - M.phi(s.r2m1));
             nr = 0;
             for (int a = 0; a < 3; a++) nr += d1[a] * d1[a];
             if (nr > 0) {
@@ -228,6 +228,23 @@ __device__ __forceinline__ void collision_stress(const Geo &g, const Phys &P, co
 }
 
 // compute_stress over eid in -1..s+1 (padded 1..s+3)
+// The level-set value of every map for its object, once per stress pass
+// (the stencils below read each about ten times): primary maps over the
+// whole padded grid, then the band entries.
+__global__ void k_phi_prim(Geo g, Phys P, const ObjC *O, RMap R) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.Ng) return;
+    unsigned o = R.c[t] & OMASK;
+    R.phi[t] = (o == 0) ? __builtin_nan("") : obj_phi(O, P.n_obj, (int)o - 1, R.x[0][t], R.x[1][t], R.x[2][t]);
+}
+__global__ void k_phi_band(Phys P, const ObjC *O, Band B) {
+    int n = *B.n;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        unsigned o = B.c[e] & OMASK;
+        B.phi[e] = (o == 0) ? __builtin_nan("") : obj_phi(O, P.n_obj, (int)o - 1, B.x[0][e], B.x[1][e], B.x[2][e]);
+    }
+}
+
 // A cell takes the solid path when it holds a primary map of an object or a
 // band entry; every other cell's lower-face stresses are the fluid's alone,
 // mu du/dn, with the reference's blend (1 - sfrac) fs + ss at sfrac = ss = 0.
@@ -1074,6 +1091,8 @@ __global__ void k_zero1(int *p) {
 
 void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    int *list, int *count) {
+    hipLaunchKernelGGL(k_phi_prim, dim3((g.Ng + 255) / 256), dim3(256), 0, s, g, P, O, M.r);
+    hipLaunchKernelGGL(k_phi_band, dim3(1024), dim3(256), 0, s, P, O, M.b);
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, count);
     int ncell = (g.m + 3) * (g.n + 3) * (g.o + 3);
     hipLaunchKernelGGL(k_stress_fluid, dim3((ncell + 1023) / 1024), dim3(1024), 0, s, g, P, U, M, list, count);
