@@ -216,3 +216,41 @@ def write_cfg(cfg, objs, path, comment=""):
         put("shear_mod", i, f(o.shear_mod))
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
+
+
+def manufactured(n=32, fmu=0.01):
+    """sim_ftest (Sim/sim_manager.cc:139-178): the manufactured solution
+    u = cos(2 pi t) (-sin 2pi x cos 2pi y cos 2pi z, 1/2 cos sin cos, 1/2 cos cos sin)
+    with its forcing, on the triply periodic unit cube (the field is
+    1-periodic; the reference's sample ftest.cfg uses walls, where this
+    velocity does not vanish)."""
+    c = default_config()
+    for i in range(3):
+        c.sys_size[i] = n
+    c.fmu = fmu
+    c.dt = 10.0
+    c.fdt_pad = 0.5
+    c.T = 1.0
+    c.frames = 100
+    c.sim_type = 0
+    c.n_obj = 0
+    c.dump_code = 0
+    finalize_config(c)
+    return c, []
+
+
+def manufactured_velocity(cfg, t):
+    """The exact velocity of manufactured() at time t on the padded grid."""
+    import numpy as np
+    m, n, o = cfg.sys_size
+    b = cfg.xyz_bounds
+    x = b[0] + (np.arange(m + 4) - 1.5) * ((b[1] - b[0]) / m)
+    y = b[2] + (np.arange(n + 4) - 1.5) * ((b[3] - b[2]) / n)
+    z = b[4] + (np.arange(o + 4) - 1.5) * ((b[5] - b[4]) / o)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    tp = 2 * np.pi
+    ct = np.cos(tp * t)
+    u = -np.sin(tp * X) * np.cos(tp * Y) * np.cos(tp * Z) * ct
+    v = 0.5 * np.cos(tp * X) * np.sin(tp * Y) * np.cos(tp * Z) * ct
+    w = 0.5 * np.cos(tp * X) * np.cos(tp * Y) * np.sin(tp * Z) * ct
+    return u.ravel(), v.ravel(), w.ravel()
