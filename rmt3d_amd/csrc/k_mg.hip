@@ -180,8 +180,8 @@ struct FemColour {
     int jI0, njI, kI0, nkI;   // interior rows: j = jI0 + 2a (a < njI), k = kI0 + 2b
     int hyc;                  // rows of the colour per k: j = jc + 2a, a < hyc
     int kc;
-    int nks, ks[2];           // boundary k values of the colour
-    int njs, js[2];           // boundary j values of the colour
+    int nks, ks0, ks1;        // boundary k values of the colour
+    int njs, js0, js1;        // boundary j values of the colour
     int NI, NA, NB;
 };
 
@@ -221,12 +221,12 @@ __device__ __forceinline__ bool fem_colour_node(const MGLevel<T> &L, const FemCo
     int n1 = F.nks * F.hyc;
     if (rb < n1) {
         int a = rb / F.hyc;
-        k = F.ks[a];
+        k = a ? F.ks1 : F.ks0;
         j = jc + 2 * (rb - a * F.hyc);
     } else {
         rb -= n1;
         int a = rb / F.nkI;
-        j = F.js[a];
+        j = a ? F.js1 : F.js0;
         k = F.kI0 + 2 * (rb - a * F.nkI);
     }
     return false;
@@ -248,7 +248,7 @@ __global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, FemColour F) {
 }
 
 // host side: the work-item layout of one colour of a level
-static FemColour fem_colour(const int d[3], int color) {
+static __host__ __device__ FemColour fem_colour(const int d[3], int color) {
     FemColour F{};
     int jc = color & 1, kc = (color >> 1) & 1;
     int D0 = d[0], D1 = d[1], D2 = d[2];
@@ -265,10 +265,18 @@ static FemColour fem_colour(const int d[3], int color) {
         F.NB = D0 * F.hyc * hzc;
         return F;
     }
-    if (kc == 0) F.ks[F.nks++] = 0;
-    if ((D2 - 1) % 2 == kc) F.ks[F.nks++] = D2 - 1;
-    if (jc == 0) F.js[F.njs++] = 0;
-    if ((D1 - 1) % 2 == jc) F.js[F.njs++] = D1 - 1;
+    if (kc == 0) F.ks0 = 0, F.nks = 1;
+    if ((D2 - 1) % 2 == kc) {
+        if (F.nks) F.ks1 = D2 - 1;
+        else F.ks0 = D2 - 1;
+        F.nks++;
+    }
+    if (jc == 0) F.js0 = 0, F.njs = 1;
+    if ((D1 - 1) % 2 == jc) {
+        if (F.njs) F.js1 = D1 - 1;
+        else F.js0 = D1 - 1;
+        F.njs++;
+    }
     F.NI = (D0 - 2) * F.njI * F.nkI;
     F.NA = 2 * F.njI * F.nkI;
     F.NB = (F.nks * F.hyc + F.njs * F.nkI) * D0;
@@ -307,7 +315,7 @@ struct NodeMap {
     FemColour F[4];
 };
 
-static NodeMap node_map(const int d[3], int fem) {
+static __host__ __device__ NodeMap node_map(const int d[3], int fem) {
     NodeMap M{};
     M.fem = fem;
     if (!fem) {
@@ -499,13 +507,19 @@ template <typename T>
 __device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
     int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
     if (fem) {
-        int hy = (D1 + 1) / 2, hz = (D2 + 1) / 2, n = D0 * hy * hz;
-        float ra = 1.0f / D0, rb = 1.0f / hy;
+        // the colour's interior nodes first, as in k_fem_smooth
+        FemColour F = fem_colour(L.d, col);
+        int n = F.NI + F.NA + F.NB;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            int i, jj, kk;
-            unflat(t, D0, hy, ra, rb, i, jj, kk);
-            int j = 2 * jj + (col & 1), k = 2 * kk + ((col >> 1) & 1);
-            if (j < D1 && k < D2) fem_smooth_at(L, P, i, j, k);
+            int i, j, k;
+            bool in = fem_colour_node(L, F, col & 1, t, i, j, k);
+            int c = i + D0 * (j + D1 * k);
+            if (in) {
+                T s = fem_off_interior(L.x, c, D0, D0 * D1);
+                L.x[c] = (L.b[c] - s) * T(0.03125);
+            } else {
+                fem_smooth_at(L, P, i, j, k);
+            }
         }
     } else {
         int hx = (D0 + 1) / 2, n = hx * D1 * D2;
@@ -569,11 +583,24 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
         const MGLevel<T> C = tail_level<T, IN_LDS>(t, dims, offs, smem, l + 1);
         for (int sw = 0; sw < vdown; sw++)
             for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
-        float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
-        for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
-            int i, j, k;
-            unflat(e, L.d[0], L.d[1], ra, rb, i, j, k);
-            resid_at(L, P, fem, i, j, k);
+        if (fem) {
+            for (int col = 0; col < 4; col++) {
+                FemColour F = fem_colour(L.d, col);
+                int n = F.NI + F.NA + F.NB;
+                for (int e = threadIdx.x; e < n; e += blockDim.x) {
+                    int i, j, k;
+                    bool in = fem_colour_node(L, F, col & 1, e, i, j, k);
+                    int c = i + L.d[0] * (j + L.d[1] * k);
+                    L.r[c] = resid_value(L, P, 1, in, i, j, k, c);
+                }
+            }
+        } else {
+            float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
+            for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
+                int i, j, k;
+                unflat(e, L.d[0], L.d[1], ra, rb, i, j, k);
+                resid_at(L, P, 0, i, j, k);
+            }
         }
         __syncthreads();
         float rc0 = 1.0f / C.d[0], rc1 = 1.0f / C.d[1];
