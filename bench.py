@@ -22,7 +22,11 @@ The JSON line carries:
                kernel runs on, over --kernel-steps steps run right after the
                timed region with per-launch events (the timed region replays
                the multigrid V-cycles as captured graphs, where single
-               launches cannot be bracketed)
+               launches cannot be bracketed; each launch stamps its own start
+               and end events, so the figure is the kernel's execution time);
+               traffic is the HBM bytes per launch that rocprofv3 --pmc
+               measured for the same kernel (profiles/hbm_traffic_*.json),
+               null if that summary covers another kernel
   cpu_baseline the FP64 oracle (C restatement of the reference) on a bounded
                sample of the same workload on this host
   config.serving  the serving workload: Poisson arrivals (--serve-rate per
@@ -69,6 +73,14 @@ def parse():
 # the unknowns of the three other colours (3/4 N words), its own rhs (1/4 N)
 # and writes its own unknowns (1/4 N).
 FEM_COLOUR_WORDS = 3.0 / 4 + 1.0 / 4 + 1.0 / 4
+# np consecutive colour passes in one launch (k_sweep): every unknown of the
+# level read once, the np colours' rhs read and their unknowns written.
+
+
+def sweep_words(np_):
+    return 1.0 + np_ / 4.0 + np_ / 4.0
+
+
 # One V-cycle, per finest unknown, at 2 pre + 2 post sweeps: the sum over the
 # levels (geometric factor 8/7) of the smoothing passes, residual, restriction,
 # prolongation and coarse-correction zeroing.
@@ -76,6 +88,21 @@ VCYCLE_WORDS = 21.0
 # One FP64 defect-correction step of the mixed path adds the FP64 residual
 # (read x, b, write r) per finest unknown.
 DEFECT_WORDS64 = 3.0
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fem_smooth_129.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
+    for a plain 1 GiB copy), or None when the summary covers another kernel."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("traffic_bytes_per_launch") if t.get("kernel") == kernel else None
 
 
 def run_gpu(args, precision):
@@ -116,6 +143,7 @@ def run_gpu(args, precision):
     for i, k in enumerate(("fem_smooth", "mac_smooth", "stress", "advect")):
         prof[k] = sim.profile_read(i)
     sim.profile(0)
+    npass = int(sim.diag(17))
     elapsed = t1 - t0
     m, n, o = cfg.sys_size
     cells = m * n * o
@@ -123,12 +151,17 @@ def run_gpu(args, precision):
     word = 4 if precision == "mixed" else 8
     ms, launches = prof["fem_smooth"]
     avg_s = ms / max(launches, 1) * 1e-3
-    bytes_launch = word * nodes * FEM_COLOUR_WORDS
+    tname = "float" if word == 4 else "double"
+    if npass <= 1:
+        bytes_launch = word * nodes * FEM_COLOUR_WORDS
+        kname = "k_fem_smooth<%s> colour pass, finest level (%d^3 nodes)" % (tname, m + 1)
+    else:
+        bytes_launch = word * nodes * sweep_words(npass)
+        kname = "k_sweep<%s, FEM> %d colour passes, finest level (%d^3 nodes)" % (tname, npass, m + 1)
     achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_fem_smooth<%s> colour pass, finest level (%d^3 nodes)"
-                      % ("float" if word == 4 else "double", m + 1),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
+            "kernel": kname,
             "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 3),
             "launches": launches}
     vc = {}

@@ -177,7 +177,8 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * 6 -> eps (transition zone width), 8 -> vmax (sim_manager), 9 -> K_stiff;
  * multigrid layout (no counterpart in the reference): 10/11 -> MAC/FEM
  * level count, 12/13 -> first level run by the one-workgroup tail kernel,
- * 14/15 -> tail levels held in LDS (bit 0 FP64 hierarchy, bit 1 FP32). */
+ * 14/15 -> tail levels held in LDS (bit 0 FP64 hierarchy, bit 1 FP32),
+ * 16/17 -> colour passes per smoothing launch on the finest level. */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
 /* Per-stage device time of the steps run while rmt3d_profile is enabled

@@ -108,6 +108,8 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out) {
         case 13: *out = S.fem.tail_lev; return 0;
         case 14: *out = S.mac.tail_lds64 + 2 * S.mac.tail_lds32; return 0;
         case 15: *out = S.fem.tail_lds64 + 2 * S.fem.tail_lds32; return 0;
+        case 16: *out = S.mac.npass[0]; return 0;
+        case 17: *out = S.fem.npass[0]; return 0;
         }
     } catch (...) {
         return RMT3D_ERR_FATAL;

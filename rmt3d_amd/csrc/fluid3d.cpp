@@ -86,6 +86,21 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     // tracing tools that do not follow graph launches)
     const char *ng = getenv("RMT3D_NO_GRAPH");
     no_graph = ng && ng[0] == '1';
+    // colour passes per launch above the tail: one per launch on the finest
+    // level (bandwidth-bound, where a halo's redundant loads cost more than
+    // launches save), a whole sweep per launch below it (latency-bound);
+    // RMT3D_SWEEP="n0,n1,..." sets them per level (1: one launch per colour)
+    for (int l = 0; l < 16; l++) npass[l] = (l == 0) ? 1 : (fem ? 4 : 2);
+    if (const char *sw = getenv("RMT3D_SWEEP")) {
+        int l = 0;
+        for (const char *p = sw; *p && l < 16; l++) {
+            int v = atoi(p);
+            if (v >= 1 && v <= (fem ? 4 : 2)) npass[l] = v;
+            while (*p && *p != ',') p++;
+            if (*p == ',') p++;
+        }
+    }
+    mg_sweep_enable();
     tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
     tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
     double eps = DBL_EPSILON, c = fem ? 32.0 : 6.0;
@@ -125,54 +140,64 @@ double MGSolver::resid_max(hipStream_t s) {
     return v;
 }
 
-void MGSolver::vcycle64(hipStream_t s, int lev) {
+// nsweeps smoothing sweeps: each sweep is the level's colour passes in
+// order, np of them per launch (np = 1: one in-place single-colour launch per
+// pass).  A multi-pass launch reads the unknowns from one array and writes
+// them to another, so the launches alternate between x and the level's
+// residual array (free while smoothing); an odd count ends with a copy back
+// into x.  Per-kernel profiling stamps each launch's own start and end.
+template <typename T>
+static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3], int np, int nsweeps, KProf *kp) {
     int ncol = fem ? 4 : 2;
+    // a halo no wider than a periodic extent wraps at most once
+    for (int a = 0; a < 3; a++)
+        if (prd[a] && l.d[a] < np) np = l.d[a];
+    if (np < 1) np = 1;
+    int m = 0;
+    for (int sw = 0; sw < nsweeps; sw++)
+        for (int c0 = 0; c0 < ncol; c0 += np) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (kp) kp->pair(e0, e1);
+            if (np == 1) {
+                launch_mg_smooth(s, fem, l, prd, c0, e0, e1);
+                continue;
+            }
+            SweepPlan P = mg_sweep_plan(fem, l.d, prd, (int)sizeof(T), std::min(np, ncol - c0));
+            P.c0 = c0;
+            const T *src = (m & 1) ? l.r : l.x;
+            T *dst = (m & 1) ? l.x : l.r;
+            launch_mg_sweep(s, fem, l, prd, P, src, dst, e0, e1);
+            m++;
+        }
+    if (m & 1) launch_mg_copy(s, (const T *)l.r, l.x, l.nn);
+}
+
+void MGSolver::vcycle64(hipStream_t s, int lev) {
     if (lev >= tail_lev) {
         launch_mg_tail(s, fem, L + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds64);
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
-    for (int sw = 0; sw < v_down; sw++)
-        for (int c = 0; c < ncol; c++) {
-            if (kp) kp->begin(s);
-            launch_mg_smooth(s, fem, L[lev], prd, c);
-            if (kp) kp->end(s);
-        }
+    smooth(s, fem, L[lev], prd, npass[lev], v_down, kp);
     launch_mg_resid(s, fem, L[lev], prd);
     launch_mg_restrict(s, fem, L[lev], L[lev + 1], prd);
     vcycle64(s, lev + 1);
     launch_mg_prolong(s, fem, L[lev], L[lev + 1], prd);
-    for (int sw = 0; sw < v_up; sw++)
-        for (int c = 0; c < ncol; c++) {
-            if (kp) kp->begin(s);
-            launch_mg_smooth(s, fem, L[lev], prd, c);
-            if (kp) kp->end(s);
-        }
+    smooth(s, fem, L[lev], prd, npass[lev], v_up, kp);
 }
 
 void MGSolver::vcycle32(hipStream_t s, int lev) {
-    int ncol = fem ? 4 : 2;
     if (lev >= tail_lev) {
         launch_mg_tail(s, fem, Lf + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds32);
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
-    for (int sw = 0; sw < v_down; sw++)
-        for (int c = 0; c < ncol; c++) {
-            if (kp) kp->begin(s);
-            launch_mg_smooth(s, fem, Lf[lev], prd, c);
-            if (kp) kp->end(s);
-        }
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp);
     launch_mg_resid(s, fem, Lf[lev], prd);
     launch_mg_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
     vcycle32(s, lev + 1);
     launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
-    for (int sw = 0; sw < v_up; sw++)
-        for (int c = 0; c < ncol; c++) {
-            if (kp) kp->begin(s);
-            launch_mg_smooth(s, fem, Lf[lev], prd, c);
-            if (kp) kp->end(s);
-        }
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp);
 }
 
 // one iteration: an FP64 V-cycle, or (mixed) one FP64 defect-correction step

@@ -27,6 +27,15 @@ struct KProf {
     void end(hipStream_t s) {
         if (enabled && n < CAP) { hipEventRecord(ev[2 * n + 1], s); n++; }
     }
+    // the next event pair, for a launch that stamps its own start and end
+    // (the kernel's execution alone, without the event records around it)
+    bool pair(hipEvent_t &a, hipEvent_t &b) {
+        if (!enabled || n >= CAP) return false;
+        a = ev[2 * n];
+        b = ev[2 * n + 1];
+        n++;
+        return true;
+    }
 };
 
 // One projection problem: geometric multigrid behind mg_prob::run
@@ -44,6 +53,7 @@ struct MGSolver {
     int coarse_sweeps = 4;
     int tail_lev = 0;                      // first level run by the one-workgroup tail kernel
     int tail_lds64 = 0, tail_lds32 = 0;    // tail levels held in LDS
+    int npass[16];                         // colour passes per smoothing launch, per level
     // mg_pred (mg_probs.hh:37-78)
     int iP = 16, iV = 32;
     double V_avg = 32, V_var = 1e6, alpha = 0;

@@ -10,6 +10,8 @@
 // four-colour Gauss-Seidel by (j,k) parity (FEM: whole x-rows per colour),
 // one launch per colour; inside a colour no two unknowns are
 // neighbours, so the update order is free and the result is deterministic.
+#include <hip/hip_ext.h>
+
 #include "dev.h"
 #include "launch.h"
 
@@ -489,12 +491,6 @@ __global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem, float r0, 
     prolong_at(F, C, P, fem, i, j, k);
 }
 
-// ---- the lower part of a V-cycle in one workgroup --------------------------
-// Every level from the first one small enough down to the coarsest is
-// processed by a single workgroup: the same per-node operations as the
-// per-level kernels, in the same colour order, with workgroup barriers in
-// place of kernel boundaries, so the result is identical and the launch cost
-// of the small levels is paid once per V-cycle.
 // (i, j, k) of a flat index e of dims (a, b, .) with float reciprocals ra, rb
 __device__ __forceinline__ void unflat(int e, int a, int b, float ra, float rb, int &i, int &j, int &k) {
     int row = qdiv(e, a, ra);
@@ -502,6 +498,229 @@ __device__ __forceinline__ void unflat(int e, int a, int b, float ra, float rb, 
     k = qdiv(row, b, rb);
     j = row - k * b;
 }
+
+// ---- several colour passes per launch ---------------------------------------
+// One workgroup owns a box of the level (its tile) and stages the tile plus a
+// halo of np nodes in LDS.  Colour pass q of the np (colours c0, c0+1, ...)
+// updates the nodes of its colour within Chebyshev distance np-1-q of the
+// tile.  Every node a pass updates then reads neighbours that are either of
+// another colour not yet updated (loaded values, correct) or of an earlier
+// colour updated at a distance one larger (correct), so each tile node ends
+// bit-identical to np single-colour launches over the whole level.  Halo
+// nodes are recomputed by every tile that holds them; only tile nodes are
+// stored, all of them, into xout: the tiles load their halos from xin while
+// others store, so the two must be distinct arrays.  A periodic dimension
+// wraps when loading, so the region may hold a node more than once; each
+// copy is updated from copies of its own neighbours and stays correct.
+constexpr int SWEEP_BATCH = 8;
+
+template <typename T, int FEM>
+__global__ __launch_bounds__(1024) void k_sweep(MGLevel<T> L, Prd P, SweepPlan S, const T *__restrict__ xin,
+                                                T *__restrict__ xout) {
+    extern __shared__ unsigned char smem[];
+    T *xs = (T *)smem;
+    constexpr int NCOL = FEM ? 4 : 2;
+    const int D[3] = {L.d[0], L.d[1], L.d[2]};
+    const int np = S.np;
+    int tid = blockIdx.x;
+    int tix[3];
+    tix[0] = tid % S.nt[0];
+    tid /= S.nt[0];
+    tix[1] = tid % S.nt[1];
+    tix[2] = tid / S.nt[1];
+    // tile [t0, t1) and region [r0, r0 + rn) per dimension, in unwrapped
+    // coordinates (a periodic region may start below 0 or end beyond D)
+    int t0[3], t1[3], r0[3], rn[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        t0[a] = tix[a] * S.ts[a];
+        t1[a] = min(t0[a] + S.ts[a], D[a]);
+        if (P.p[a]) {
+            r0[a] = t0[a] - np;
+            rn[a] = t1[a] - t0[a] + 2 * np;
+        } else {
+            r0[a] = max(t0[a] - np, 0);
+            rn[a] = min(t1[a] + np, D[a]) - r0[a];
+        }
+    }
+    const int R0 = rn[0], R01 = rn[0] * rn[1], RN = R01 * rn[2];
+    // the region's unknowns and right-hand sides, side by side in LDS (the
+    // passes then wait on no global load)
+    T *bs = xs + S.region;
+    {
+        // SWEEP_BATCH elements per work-item in flight before the LDS stores,
+        // so the global load latencies overlap instead of adding up
+        float ra = 1.0f / rn[0], rb = 1.0f / rn[1];
+        const int stride = blockDim.x;
+        for (int e0 = threadIdx.x; e0 < RN; e0 += SWEEP_BATCH * stride) {
+            T xv[SWEEP_BATCH], bv[SWEEP_BATCH];
+#pragma unroll
+            for (int u = 0; u < SWEEP_BATCH; u++) {
+                int e = e0 + u * stride;
+                if (e < RN) {
+                    int li, lj, lk;
+                    unflat(e, rn[0], rn[1], ra, rb, li, lj, lk);
+                    int gi = r0[0] + li, gj = r0[1] + lj, gk = r0[2] + lk;
+                    gi += (gi < 0) ? D[0] : ((gi >= D[0]) ? -D[0] : 0);
+                    gj += (gj < 0) ? D[1] : ((gj >= D[1]) ? -D[1] : 0);
+                    gk += (gk < 0) ? D[2] : ((gk >= D[2]) ? -D[2] : 0);
+                    int gc = gi + D[0] * (gj + D[1] * gk);
+                    xv[u] = xin[gc];
+                    bv[u] = L.b[gc];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SWEEP_BATCH; u++) {
+                int e = e0 + u * stride;
+                if (e < RN) {
+                    xs[e] = xv[u];
+                    bs[e] = bv[u];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < np; q++) {
+        const int col = (S.c0 + q) % NCOL;
+        const int dist = np - 1 - q;
+        // the pass's box [u0, u1) per dimension
+        int u0[3], u1[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            u0[a] = P.p[a] ? t0[a] - dist : max(t0[a] - dist, 0);
+            u1[a] = P.p[a] ? t1[a] + dist : min(t1[a] + dist, D[a]);
+        }
+        if (FEM) {
+            // rows (j, k) of the colour: j = jc, k = kc mod 2; every i
+            int jc = col & 1, kc = col >> 1;
+            int j0 = u0[1] + ((u0[1] - jc) & 1), k0 = u0[2] + ((u0[2] - kc) & 1);
+            int nx = u1[0] - u0[0];
+            int nj = u1[1] > j0 ? (u1[1] - j0 + 1) / 2 : 0;
+            int nk = u1[2] > k0 ? (u1[2] - k0 + 1) / 2 : 0;
+            int n = nx * nj * nk;
+            float ra = 1.0f / nx, rb = 1.0f / max(nj, 1);
+            for (int e = threadIdx.x; e < n; e += blockDim.x) {
+                int ii, a, b;
+                unflat(e, nx, max(nj, 1), ra, rb, ii, a, b);
+                int g[3] = {u0[0] + ii, j0 + 2 * a, k0 + 2 * b};
+                int le = (g[0] - r0[0]) + R0 * ((g[1] - r0[1]) + rn[1] * (g[2] - r0[2]));
+                int w[3];
+                bool inner = true;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    w[d] = g[d] + ((g[d] < 0) ? D[d] : ((g[d] >= D[d]) ? -D[d] : 0));
+                    inner = inner && (P.p[d] || (g[d] >= 1 && g[d] <= D[d] - 2));
+                }
+                T bv = bs[le];
+                if (inner) {
+                    T s = fem_off_interior(xs, le, R0, R01);
+                    xs[le] = (bv - s) * T(0.03125);
+                } else {
+                    // fem_off with wall tests on the wrapped global index and
+                    // addresses in the region (a periodic neighbour is the
+                    // next region node)
+                    bool wl[3];
+                    bool ok[3][3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        wl[d] = !P.p[d] && (w[d] == 0 || w[d] == D[d] - 1);
+                        ok[d][0] = P.p[d] || w[d] >= 1;
+                        ok[d][1] = true;
+                        ok[d][2] = P.p[d] || w[d] <= D[d] - 2;
+                    }
+                    T sx = wl[0] ? T(0.5) : T(1), sy = wl[1] ? T(0.5) : T(1), sz = wl[2] ? T(0.5) : T(1);
+                    T s = 0;
+#pragma unroll
+                    for (int okk = 0; okk < 3; okk++) {
+#pragma unroll
+                        for (int oj = 0; oj < 3; oj++) {
+#pragma unroll
+                            for (int oi = 0; oi < 3; oi++) {
+                                const int nz = (oi != 1) + (oj != 1) + (okk != 1);
+                                if (nz < 2) continue;
+                                bool v = ok[0][oi] && ok[1][oj] && ok[2][okk];
+                                T wt = (nz == 2) ? T(-2) : T(-1);
+                                if (oi == 1) wt *= sx;
+                                if (oj == 1) wt *= sy;
+                                if (okk == 1) wt *= sz;
+                                T xv = xs[v ? le + (oi - 1) + R0 * (oj - 1) + R01 * (okk - 1) : le];
+                                s += (v ? wt : T(0)) * xv;
+                            }
+                        }
+                    }
+                    T rdg = T(0.03125);
+                    if (wl[0]) rdg *= T(2);
+                    if (wl[1]) rdg *= T(2);
+                    if (wl[2]) rdg *= T(2);
+                    xs[le] = (bv - s) * rdg;
+                }
+            }
+        } else {
+            // red-black: i of parity (j + k + col) in each (j, k) row
+            int nj = u1[1] - u0[1], nk = u1[2] - u0[2];
+            int hx = (u1[0] - u0[0] + 1) / 2;
+            int n = hx * nj * nk;
+            float ra = 1.0f / hx, rb = 1.0f / nj;
+            for (int e = threadIdx.x; e < n; e += blockDim.x) {
+                int ih, a, b;
+                unflat(e, hx, nj, ra, rb, ih, a, b);
+                int gj = u0[1] + a, gk = u0[2] + b;
+                int gi = u0[0] + 2 * ih + ((u0[0] + gj + gk + col) & 1);
+                if (gi >= u1[0]) continue;
+                int g[3] = {gi, gj, gk};
+                int le = (g[0] - r0[0]) + R0 * ((g[1] - r0[1]) + rn[1] * (g[2] - r0[2]));
+                int w[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) w[d] = g[d] + ((g[d] < 0) ? D[d] : ((g[d] >= D[d]) ? -D[d] : 0));
+                // mac_off's order: i-1, i+1, j-1, j+1, k-1, k+1
+                const int off[3] = {1, R0, R01};
+                T s = 0;
+                int miss = 0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+#pragma unroll
+                    for (int sg = -1; sg <= 1; sg += 2) {
+                        bool v = P.p[d] || (sg < 0 ? w[d] >= 1 : w[d] <= D[d] - 2);
+                        miss += !v;
+                        T xv = xs[v ? le + sg * off[d] : le];
+                        s += v ? xv : T(0);
+                    }
+                }
+                T dg = T(-6.0 + miss);
+                T bv = bs[le];
+                T xn = (bv - s) / dg;
+                xs[le] = T(1 - MAC_OMEGA) * xs[le] + T(MAC_OMEGA) * xn;
+            }
+        }
+        __syncthreads();
+    }
+    // store every node of the tile
+    {
+        int n0 = t1[0] - t0[0], n1 = t1[1] - t0[1], n2 = t1[2] - t0[2];
+        int n = n0 * n1 * n2;
+        float ra = 1.0f / n0, rb = 1.0f / n1;
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            int ii, jj, kk;
+            unflat(e, n0, n1, ra, rb, ii, jj, kk);
+            int gi = t0[0] + ii, gj = t0[1] + jj, gk = t0[2] + kk;
+            int le = (gi - r0[0]) + R0 * ((gj - r0[1]) + rn[1] * (gk - r0[2]));
+            xout[gi + D[0] * (gj + D[1] * gk)] = xs[le];
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_copy(const T *a, T *b, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = a[t];
+}
+
+// ---- the lower part of a V-cycle in one workgroup --------------------------
+// Every level from the first one small enough down to the coarsest is
+// processed by a single workgroup: the same per-node operations as the
+// per-level kernels, in the same colour order, with workgroup barriers in
+// place of kernel boundaries, so the result is identical and the launch cost
+// of the small levels is paid once per V-cycle.
 
 template <typename T>
 __device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
@@ -571,10 +790,27 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
     }
     __syncthreads();
     if (IN_LDS) {
+        // several loads in flight per work-item before the LDS stores
         MGLevel<T> L0 = tail_level<T, true>(t, dims, offs, smem, 0);
-        for (int e = threadIdx.x; e < L0.nn; e += blockDim.x) {
-            L0.x[e] = t.L[0].x[e];
-            L0.b[e] = t.L[0].b[e];
+        const int stride = blockDim.x;
+        for (int e0 = threadIdx.x; e0 < L0.nn; e0 += SWEEP_BATCH * stride) {
+            T xv[SWEEP_BATCH], bv[SWEEP_BATCH];
+#pragma unroll
+            for (int u = 0; u < SWEEP_BATCH; u++) {
+                int e = e0 + u * stride;
+                if (e < L0.nn) {
+                    xv[u] = t.L[0].x[e];
+                    bv[u] = t.L[0].b[e];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < SWEEP_BATCH; u++) {
+                int e = e0 + u * stride;
+                if (e < L0.nn) {
+                    L0.x[e] = xv[u];
+                    L0.b[e] = bv[u];
+                }
+            }
         }
         __syncthreads();
     }
@@ -674,19 +910,74 @@ __global__ void k_zero_f(float *a, int n) {
 static inline Prd mkprd(const int p[3]) { Prd P; P.p[0] = p[0]; P.p[1] = p[1]; P.p[2] = p[2]; return P; }
 
 template <typename T>
-void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color) {
+void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color, hipEvent_t start,
+                      hipEvent_t stop) {
     Prd P = mkprd(prd);
     if (!fem) {
         int hx = (L.d[0] + 1) / 2, n = hx * L.d[1] * L.d[2];
-        hipLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, hx, 1.0f / hx,
-                           1.0f / L.d[1]);
+        hipExtLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, start, stop, 0, L, P, color,
+                              hx, 1.0f / hx, 1.0f / L.d[1]);
     } else {
         FemColour F = fem_colour(L.d, color);
         int n = F.NI + F.NA + F.NB;
-        hipLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, L, P, color, F);
+        hipExtLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, start, stop, 0, L, P, color,
+                              F);
     }
 }
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
+
+// Tiles of at most 17, 9 or 5 nodes per dimension: the largest that gives
+// 256 workgroups (one per CU) and whose region (tile plus halo, unknowns and
+// right-hand sides) fits in LDS, else 5.  A region holds at most
+// (5 + 2 np)^3 = 2197 unknowns at the smallest tile: 35 KB of LDS in FP64.
+SweepPlan mg_sweep_plan(int fem, const int d[3], const int prd[3], int esize, int np) {
+    SweepPlan S{};
+    static const int sizes[3] = {17, 9, 5};
+    for (int z = 0; z < 3; z++) {
+        int tiles = 1, region = 1;
+        for (int a = 0; a < 3; a++) {
+            S.nt[a] = (d[a] + sizes[z] - 1) / sizes[z];
+            S.ts[a] = (d[a] + S.nt[a] - 1) / S.nt[a];
+            tiles *= S.nt[a];
+            int r = S.ts[a] + 2 * np;
+            if (!prd[a] && r > d[a]) r = d[a];
+            region *= r;
+        }
+        S.region = region;
+        if (tiles >= 256 && 2 * (size_t)region * esize <= MG_TAIL_LDS_MAX) break;
+    }
+    S.np = np;
+    S.c0 = 0;
+    S.lds = 2 * S.region * esize;
+    S.block = S.region > 4096 ? 1024 : 256;
+    (void)fem;
+    return S;
+}
+
+void mg_sweep_enable() {
+    const int bytes = (int)MG_TAIL_LDS_MAX;
+    (void)hipFuncSetAttribute((const void *)k_sweep<double, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void *)k_sweep<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void *)k_sweep<float, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void *)k_sweep<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipGetLastError();
+}
+
+template <typename T>
+void launch_mg_sweep(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], const SweepPlan &plan,
+                     const T *xin, T *xout, hipEvent_t start, hipEvent_t stop) {
+    dim3 grid(plan.nt[0] * plan.nt[1] * plan.nt[2]);
+    if (fem)
+        hipExtLaunchKernelGGL((k_sweep<T, 1>), grid, dim3(plan.block), plan.lds, s, start, stop, 0, L, mkprd(prd),
+                              plan, xin, xout);
+    else
+        hipExtLaunchKernelGGL((k_sweep<T, 0>), grid, dim3(plan.block), plan.lds, s, start, stop, 0, L, mkprd(prd),
+                              plan, xin, xout);
+}
+template <typename T>
+void launch_mg_copy(hipStream_t s, const T *a, T *b, int n) {
+    hipLaunchKernelGGL(k_copy<T>, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);
+}
 
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]) {
@@ -755,7 +1046,11 @@ void launch_zero_f(hipStream_t s, float *a, int n) {
 }
 
 #define INST(T)                                                                                              \
-    template void launch_mg_smooth<T>(hipStream_t, int, const MGLevel<T> &, const int *, int);              \
+    template void launch_mg_smooth<T>(hipStream_t, int, const MGLevel<T> &, const int *, int, hipEvent_t,    \
+                                      hipEvent_t);                                                           \
+    template void launch_mg_sweep<T>(hipStream_t, int, const MGLevel<T> &, const int *, const SweepPlan &,   \
+                                     const T *, T *, hipEvent_t, hipEvent_t);                                \
+    template void launch_mg_copy<T>(hipStream_t, const T *, T *, int);                                      \
     template void launch_mg_resid<T>(hipStream_t, int, const MGLevel<T> &, const int *);                    \
     template void launch_mg_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *); \
     template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *);  \

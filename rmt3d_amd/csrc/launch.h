@@ -68,9 +68,33 @@ struct MGTail {
     int n;
 };
 
-// smooth one colour (MAC: red-black SOR, FEM: four colours by (j,k) parity)
+// np consecutive colour passes in one launch: one workgroup per tile, the
+// tile and a halo of np nodes staged in LDS (k_sweep)
+struct SweepPlan {
+    int nt[3];     // tiles per dimension
+    int ts[3];     // tile extent per dimension (the last tile may be shorter)
+    int np;        // colour passes per launch = halo width
+    int c0;        // first colour of the launch
+    int block;     // work-items per workgroup
+    int region;    // unknowns of the largest tile plus halo
+    int lds;       // bytes of LDS: unknowns and right-hand sides of that region
+};
+// the plan for np passes on a level of dims d whose unknowns take esize bytes
+SweepPlan mg_sweep_plan(int fem, const int d[3], const int prd[3], int esize, int np);
+// allows k_sweep the dynamic LDS that its largest plan can ask for
+void mg_sweep_enable();
+// reads the unknowns from xin and writes every unknown to xout (xin != xout)
 template <typename T>
-void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color);
+void launch_mg_sweep(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], const SweepPlan &plan,
+                     const T *xin, T *xout, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+template <typename T>
+void launch_mg_copy(hipStream_t s, const T *a, T *b, int n);
+
+// smooth one colour (MAC: red-black SOR, FEM: four colours by (j,k) parity);
+// non-null start/stop events are stamped with the kernel's own start and end
+template <typename T>
+void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color,
+                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]);
 template <typename T>
