@@ -120,12 +120,25 @@ def lib():
         L.oracle_det_cos.argtypes = [D]
         L.oracle_det_cos.restype = D
         L.oracle_mg_set_coarse_sweeps.argtypes = [P, C.c_int]
+        L.oracle_sym_pinv.argtypes = [C.c_int, dp, dp]
+        L.oracle_mg_direct.argtypes = [P, C.c_int]
+        L.oracle_mg_direct.restype = C.c_int
         L.oracle_threads.restype = C.c_int
         L.oracle_heaviside.argtypes = [P, D]
         L.oracle_heaviside.restype = D
         L.oracle_lsq.argtypes = [C.c_int, dp, dp, dp, D, dp, dp, dp, dp]
         _lib = L
     return _lib
+
+
+def sym_pinv(A):
+    """Moore-Penrose pseudo-inverse of a symmetric matrix (the oracle's
+    Jacobi-rotation routine)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    P = np.zeros((n, n))
+    lib().oracle_sym_pinv(n, _dp(A), _dp(P))
+    return P
 
 
 def _dp(a):
@@ -245,6 +258,9 @@ class OracleSim:
         res = C.c_double()
         lib().oracle_mg_solve(self.h, which, _dp(r), _dp(x), tol, C.byref(nc), C.byref(res))
         return x, nc.value, res.value
+
+    def mg_direct(self, which):
+        return bool(lib().oracle_mg_direct(self.h, which))
 
     def mg_apply(self, which, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

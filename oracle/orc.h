@@ -79,6 +79,13 @@ typedef struct {
     double h;
 } orc_face;
 
+/* a level with at most this many unknowns ends the hierarchy and is solved
+ * exactly (pseudo-inverse) */
+#define ORC_MG_DIRECT_NN 160
+
+/* Moore-Penrose pseudo-inverse of a symmetric n x n matrix (row-major) */
+void orc_sym_pinv(int n, const double *A, double *P);
+
 /* multigrid level */
 typedef struct {
     int d[3];              /* unknowns per dimension */
@@ -95,6 +102,9 @@ typedef struct {
     double tol, log_tol;
     int v_up, v_down;
     int coarse_sweeps;
+    /* pseudo-inverse of the coarsest operator (nn x nn, symmetric) when that
+     * level has at most ORC_MG_DIRECT_NN unknowns, else NULL */
+    double *pinv;
     /* mg_pred (mg_probs.hh:37-78) */
     int iP, iV;
     double V_avg, V_var, alpha;

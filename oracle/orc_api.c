@@ -167,6 +167,10 @@ int oracle_lsq(int npt, const double *pos, const double *val, const double *w, d
 }
 
 void oracle_mg_set_coarse_sweeps(orc_sim *S, int n) { S->mac.coarse_sweeps = n; S->fem.coarse_sweeps = n; }
+void oracle_sym_pinv(int n, const double *A, double *P) { orc_sym_pinv(n, A, P); }
+/* 1 when the coarsest level of problem `which` (0 MAC, 1 FEM) is solved
+ * exactly by its pseudo-inverse */
+int oracle_mg_direct(orc_sim *S, int which) { return (which ? S->fem.pinv : S->mac.pinv) != NULL; }
 
 /* the deterministic sine/cosine shared with the HIP path */
 double oracle_det_sin(double x) { return det_sin(x); }

@@ -25,8 +25,12 @@
  * Gauss-Seidel by (j,k) parity (whole x-rows per colour, valid because the
  * Q1 face weights vanish), full-weighting restriction (= P^T) with the
  * Galerkin coarse operator, which for nested Q1 spaces equals twice the fine
- * stencil (coarse rhs = R r / 2), vertex trilinear prolongation.  Coarsest
- * level: 4 Gauss-Seidel sweeps (more change no cycle count).
+ * stencil (coarse rhs = R r / 2), vertex trilinear prolongation.  The
+ * hierarchy ends at the first level with at most ORC_MG_DIRECT_NN unknowns,
+ * solved exactly: x = A^+ b with the Moore-Penrose pseudo-inverse (the
+ * minimum-norm least-squares solution, so the singular all-Neumann and
+ * periodic problems need no special case).  A hierarchy that cannot coarsen
+ * that far ends with 4 Gauss-Seidel sweeps on its coarsest level.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -69,12 +73,30 @@ void orc_mg_setup(orc_mg *G, int fem, const int d[3], const int prd[3])
         L->b = (double *)calloc((size_t)L->nn, sizeof(double));
         L->r = (double *)calloc((size_t)L->nn, sizeof(double));
         l++;
+        if (L->nn <= ORC_MG_DIRECT_NN) break;
         int ok = l < 16;
         for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
         if (!ok) break;
         for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
     }
     G->nlev = l;
+    G->pinv = NULL;
+    if (G->lev[l - 1].nn <= ORC_MG_DIRECT_NN) {
+        /* the coarsest operator column by column, then its pseudo-inverse */
+        int n = G->lev[l - 1].nn;
+        double *A = (double *)calloc((size_t)n * n, sizeof(double));
+        double *e = (double *)calloc((size_t)n, sizeof(double));
+        double *y = (double *)calloc((size_t)n, sizeof(double));
+        for (int c = 0; c < n; c++) {
+            e[c] = 1;
+            orc_mg_apply(G, l - 1, e, y);
+            for (int r = 0; r < n; r++) A[r * n + c] = y[r];
+            e[c] = 0;
+        }
+        G->pinv = (double *)malloc(sizeof(double) * (size_t)n * n);
+        orc_sym_pinv(n, A, G->pinv);
+        free(A); free(e); free(y);
+    }
     double eps = DBL_EPSILON;
     double c = fem ? 32.0 : 6.0;
     G->tol = c * c * eps * eps * 1e4;
@@ -95,7 +117,75 @@ void orc_mg_free(orc_mg *G)
     for (int l = 0; l < G->nlev; l++) {
         free(G->lev[l].x); free(G->lev[l].b); free(G->lev[l].r);
     }
+    free(G->pinv);
+    G->pinv = NULL;
     G->nlev = 0;
+}
+
+/* Moore-Penrose pseudo-inverse of a symmetric matrix from its eigen-
+ * decomposition by cyclic Jacobi rotations (Golub & Van Loan, Matrix
+ * Computations, 4th ed., sec. 8.5): rotations in (p, q) row order until the
+ * off-diagonal mass falls below 1e-28 of the diagonal's, then
+ * P = sum over eigenvalues |l_i| > 1e-10 max|l| of v_i v_i^T / l_i, summed
+ * in eigenvalue order; P is computed on and above the diagonal and mirrored,
+ * so it is exactly symmetric. */
+void orc_sym_pinv(int n, const double *A0, double *P)
+{
+    size_t nn = (size_t)n * n;
+    double *A = (double *)malloc(sizeof(double) * nn);
+    double *V = (double *)calloc(nn, sizeof(double));
+    memcpy(A, A0, sizeof(double) * nn);
+    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, dia = 0;
+        for (int p = 0; p < n; p++) {
+            dia += A[(size_t)p * n + p] * A[(size_t)p * n + p];
+            for (int q = p + 1; q < n; q++) off += A[(size_t)p * n + q] * A[(size_t)p * n + q];
+        }
+        if (off <= 1e-28 * dia) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[(size_t)p * n + q];
+                if (apq == 0) continue;
+                double theta = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2 * apq);
+                double t = 1 / (fabs(theta) + sqrt(theta * theta + 1));
+                if (theta < 0) t = -t;
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * akp - s * akq;
+                    A[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * apk - s * aqk;
+                    A[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = c * vkp - s * vkq;
+                    V[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double lmax = 0;
+    for (int i = 0; i < n; i++) {
+        double a = fabs(A[(size_t)i * n + i]);
+        if (a > lmax) lmax = a;
+    }
+    double thr = 1e-10 * lmax;
+    for (int r = 0; r < n; r++)
+        for (int c = r; c < n; c++) {
+            double s = 0;
+            for (int i = 0; i < n; i++) {
+                double l = A[(size_t)i * n + i];
+                if (fabs(l) > thr) s += V[(size_t)r * n + i] * V[(size_t)c * n + i] / l;
+            }
+            P[(size_t)r * n + c] = s;
+            P[(size_t)c * n + r] = s;
+        }
+    free(A);
+    free(V);
 }
 
 /* neighbour index along one dimension: -1 if outside a wall */
@@ -324,7 +414,18 @@ static void prolong_add(orc_mg *G, int lev)
 static void vcycle_rec(orc_mg *G, int lev)
 {
     if (lev == G->nlev - 1) {
-        for (int s = 0; s < G->coarse_sweeps; s++) smooth(G, lev);
+        orc_mglev *L = &G->lev[lev];
+        if (G->pinv) {
+            /* x = A^+ b, each row summed in column order */
+            int n = L->nn;
+            for (int r = 0; r < n; r++) {
+                double s = 0;
+                for (int c = 0; c < n; c++) s += G->pinv[(size_t)r * n + c] * L->b[c];
+                L->x[r] = s;
+            }
+        } else {
+            for (int s = 0; s < G->coarse_sweeps; s++) smooth(G, lev);
+        }
         return;
     }
     for (int s = 0; s < G->v_down; s++) smooth(G, lev);

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <vector>
 #include <cstdio>
 
 namespace rmt {
@@ -33,6 +34,119 @@ static bool can_coarsen(int fem, int d, int prd) {
     return (d % 2 == 1) && d >= 3 && (d - 1) / 2 + 1 >= 2;
 }
 static int coarse_dim(int fem, int d, int prd) { return (!fem || prd) ? d / 2 : (d - 1) / 2 + 1; }
+
+// The coarsest operator as a dense matrix (row-major), entry by entry as
+// mac_off / fem_off weight them: MAC +1 per face neighbour present and
+// -6 + missing on the diagonal; FEM the 20 edge and corner weights (-2, -1)
+// halved per wall dimension whose offset is zero, 32 halved likewise on the
+// diagonal.  A neighbour reached twice (a periodic extent of 2) adds twice.
+// Every entry is a small multiple of 1/8, so the sums are exact.
+static void coarse_operator(int fem, const int d[3], const int prd[3], double *A) {
+    int n = d[0] * d[1] * d[2];
+    for (int e = 0; e < n * n; e++) A[e] = 0;
+    auto nb = [&](int i, int di, int a) {
+        int j = i + di;
+        if (j < 0) return prd[a] ? j + d[a] : -1;
+        if (j >= d[a]) return prd[a] ? j - d[a] : -1;
+        return j;
+    };
+    for (int k = 0; k < d[2]; k++)
+        for (int j = 0; j < d[1]; j++)
+            for (int i = 0; i < d[0]; i++) {
+                int r = i + d[0] * (j + d[1] * k);
+                double *row = A + (size_t)r * n;
+                if (!fem) {
+                    int g[3] = {i, j, k}, miss = 0;
+                    for (int a = 0; a < 3; a++)
+                        for (int sg = -1; sg <= 1; sg += 2) {
+                            int q = nb(g[a], sg, a);
+                            if (q < 0) { miss++; continue; }
+                            int h[3] = {i, j, k};
+                            h[a] = q;
+                            row[h[0] + d[0] * (h[1] + d[1] * h[2])] += 1;
+                        }
+                    row[r] += -6.0 + miss;
+                    continue;
+                }
+                bool w0 = !prd[0] && (i == 0 || i == d[0] - 1);
+                bool w1 = !prd[1] && (j == 0 || j == d[1] - 1);
+                bool w2 = !prd[2] && (k == 0 || k == d[2] - 1);
+                double sx = w0 ? 0.5 : 1, sy = w1 ? 0.5 : 1, sz = w2 ? 0.5 : 1;
+                for (int ok = -1; ok <= 1; ok++)
+                    for (int oj = -1; oj <= 1; oj++)
+                        for (int oi = -1; oi <= 1; oi++) {
+                            int nz = (oi != 0) + (oj != 0) + (ok != 0);
+                            if (nz < 2) continue;
+                            int ii = nb(i, oi, 0), jj = nb(j, oj, 1), kk = nb(k, ok, 2);
+                            if (ii < 0 || jj < 0 || kk < 0) continue;
+                            double w = (nz == 2) ? -2 : -1;
+                            if (oi == 0) w *= sx;
+                            if (oj == 0) w *= sy;
+                            if (ok == 0) w *= sz;
+                            row[ii + d[0] * (jj + d[1] * kk)] += w;
+                        }
+                row[r] += 32 * sx * sy * sz;
+            }
+}
+
+// Moore-Penrose pseudo-inverse of a symmetric matrix from its eigen-
+// decomposition by cyclic Jacobi rotations (Golub & Van Loan, Matrix
+// Computations, 4th ed., sec. 8.5): rotations in (p, q) row order until the
+// off-diagonal mass falls below 1e-28 of the diagonal's, then
+// P = sum over eigenvalues |l_i| > 1e-10 max|l| of v_i v_i^T / l_i, summed in
+// eigenvalue order, computed on and above the diagonal and mirrored.  The
+// operation order is fixed (no contraction, no reassociation), so the result
+// is the same bits on every IEEE-754 host.
+static void sym_pinv(int n, const double *A0, double *P) {
+    size_t nn = (size_t)n * n;
+    std::vector<double> A(A0, A0 + nn), V(nn, 0.0);
+    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, dia = 0;
+        for (int p = 0; p < n; p++) {
+            dia += A[(size_t)p * n + p] * A[(size_t)p * n + p];
+            for (int q = p + 1; q < n; q++) off += A[(size_t)p * n + q] * A[(size_t)p * n + q];
+        }
+        if (off <= 1e-28 * dia) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = A[(size_t)p * n + q];
+                if (apq == 0) continue;
+                double theta = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2 * apq);
+                double t = 1 / (fabs(theta) + sqrt(theta * theta + 1));
+                if (theta < 0) t = -t;
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * akp - s * akq;
+                    A[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * apk - s * aqk;
+                    A[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = c * vkp - s * vkq;
+                    V[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double lmax = 0;
+    for (int i = 0; i < n; i++) lmax = std::max(lmax, fabs(A[(size_t)i * n + i]));
+    double thr = 1e-10 * lmax;
+    for (int r = 0; r < n; r++)
+        for (int c = r; c < n; c++) {
+            double s = 0;
+            for (int i = 0; i < n; i++) {
+                double l = A[(size_t)i * n + i];
+                if (fabs(l) > thr) s += V[(size_t)r * n + i] * V[(size_t)c * n + i] / l;
+            }
+            P[(size_t)r * n + c] = s;
+            P[(size_t)c * n + r] = s;
+        }
+}
 
 void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     fem = fem_;
@@ -71,10 +185,28 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         }
         nlev++;
         if (!ok_alloc) { alloc_failed = true; break; }
+        if (l.nn <= MG_DIRECT_NN) break;
         bool ok = nlev < 16;
         for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
         if (!ok) break;
         for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
+    }
+    if (!alloc_failed && L[nlev - 1].nn <= MG_DIRECT_NN) {
+        const MGLevel<double> &c = L[nlev - 1];
+        int n = c.nn;
+        std::vector<double> A((size_t)n * n), Pm((size_t)n * n);
+        coarse_operator(fem, c.d, prd, A.data());
+        sym_pinv(n, A.data(), Pm.data());
+        bool ok = hipMalloc(&d_pinv, sizeof(double) * n * n) == hipSuccess &&
+                  hipMemcpy(d_pinv, Pm.data(), sizeof(double) * n * n, hipMemcpyHostToDevice) == hipSuccess;
+        bytes += sizeof(double) * (size_t)n * n;
+        if (ok && mixed) {
+            std::vector<float> Pf(Pm.begin(), Pm.end());
+            ok = hipMalloc(&f_pinv, sizeof(float) * n * n) == hipSuccess &&
+                 hipMemcpy(f_pinv, Pf.data(), sizeof(float) * n * n, hipMemcpyHostToDevice) == hipSuccess;
+            bytes += sizeof(float) * (size_t)n * n;
+        }
+        if (!ok) alloc_failed = true;
     }
     // the tail starts at the first level small enough for one workgroup,
     // with all its levels in LDS when they fit
@@ -125,6 +257,9 @@ void MGSolver::release() {
         }
     }
     nlev = 0;
+    if (d_pinv) hipFree(d_pinv);
+    if (f_pinv) hipFree(f_pinv);
+    d_pinv = nullptr; f_pinv = nullptr;
     if (d_max) hipFree(d_max);
     if (h_max) hipHostFree(h_max);
     d_max = nullptr; h_max = nullptr;
@@ -174,7 +309,7 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
 
 void MGSolver::vcycle64(hipStream_t s, int lev) {
     if (lev >= tail_lev) {
-        launch_mg_tail(s, fem, L + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds64);
+        launch_mg_tail(s, fem, L + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds64, (const double *)d_pinv);
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
@@ -188,7 +323,7 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
 
 void MGSolver::vcycle32(hipStream_t s, int lev) {
     if (lev >= tail_lev) {
-        launch_mg_tail(s, fem, Lf + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds32);
+        launch_mg_tail(s, fem, Lf + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds32, (const float *)f_pinv);
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;

@@ -61,6 +61,8 @@ struct MGSolver {
     double last_res = 0;
     KProf *prof = nullptr;     // times the finest-level smoother launches
     KProf *vprof = nullptr;    // times whole V-cycles (one defect-correction step when mixed)
+    double *d_pinv = nullptr;              // pseudo-inverse of the coarsest operator (or null)
+    float *f_pinv = nullptr;               // its FP32 copy (mixed)
     unsigned long long *d_max = nullptr;
     size_t bytes = 0;                      // device bytes of the level arrays
     bool alloc_failed = false;

@@ -185,6 +185,8 @@ struct FemColour {
     int nks, ks0, ks1;        // boundary k values of the colour
     int njs, js0, js1;        // boundary j values of the colour
     int NI, NA, NB;
+    // float reciprocals of the divisors of the work-item -> node map
+    float rD0, rw, rnjI, rhyc, rnkI;
 };
 
 template <typename T>
@@ -192,18 +194,18 @@ __device__ __forceinline__ bool fem_colour_node(const MGLevel<T> &L, const FemCo
                                                 int &k) {
     int D0 = L.d[0];
     if (F.plain) {
-        int row = t / D0;
+        int row = qdiv(t, D0, F.rD0);
         i = t - row * D0;
-        int b = row / F.hyc;
+        int b = qdiv(row, F.hyc, F.rhyc);
         j = jc + 2 * (row - b * F.hyc);
         k = F.kc + 2 * b;
         return false;
     }
     if (t < F.NI) {
         int w = D0 - 2;
-        int row = t / w;
+        int row = qdiv(t, w, F.rw);
         i = 1 + (t - row * w);
-        int b = row / F.njI;
+        int b = qdiv(row, F.njI, F.rnjI);
         j = F.jI0 + 2 * (row - b * F.njI);
         k = F.kI0 + 2 * b;
         return true;
@@ -212,22 +214,22 @@ __device__ __forceinline__ bool fem_colour_node(const MGLevel<T> &L, const FemCo
     if (t < F.NA) {
         int row = t >> 1;
         i = (t & 1) ? D0 - 1 : 0;
-        int b = row / F.njI;
+        int b = qdiv(row, F.njI, F.rnjI);
         j = F.jI0 + 2 * (row - b * F.njI);
         k = F.kI0 + 2 * b;
         return false;
     }
     t -= F.NA;
-    int rb = t / D0;
+    int rb = qdiv(t, D0, F.rD0);
     i = t - rb * D0;
     int n1 = F.nks * F.hyc;
     if (rb < n1) {
-        int a = rb / F.hyc;
+        int a = qdiv(rb, F.hyc, F.rhyc);
         k = a ? F.ks1 : F.ks0;
         j = jc + 2 * (rb - a * F.hyc);
     } else {
         rb -= n1;
-        int a = rb / F.nkI;
+        int a = qdiv(rb, F.nkI, F.rnkI);
         j = a ? F.js1 : F.js0;
         k = F.kI0 + 2 * (rb - a * F.nkI);
     }
@@ -262,6 +264,11 @@ static __host__ __device__ FemColour fem_colour(const int d[3], int color) {
     F.njI = (F.jI0 <= D1 - 2) ? (D1 - 2 - F.jI0) / 2 + 1 : 0;
     F.kI0 = kc ? 1 : 2;
     F.nkI = (F.kI0 <= D2 - 2) ? (D2 - 2 - F.kI0) / 2 + 1 : 0;
+    F.rD0 = 1.0f / D0;
+    F.rw = 1.0f / (D0 > 2 ? D0 - 2 : 1);
+    F.rnjI = 1.0f / (F.njI > 0 ? F.njI : 1);
+    F.rhyc = 1.0f / (F.hyc > 0 ? F.hyc : 1);
+    F.rnkI = 1.0f / (F.nkI > 0 ? F.nkI : 1);
     if (D0 < 3 || F.njI == 0 || F.nkI == 0) {
         F.plain = 1;
         F.NB = D0 * F.hyc * hzc;
@@ -722,12 +729,29 @@ __global__ void k_copy(const T *a, T *b, int n) {
 // place of kernel boundaries, so the result is identical and the launch cost
 // of the small levels is paid once per V-cycle.
 
+// Per tail level, computed on the host: the FEM colour layouts, the float
+// reciprocals of the index maps and the level's place in LDS, so that a
+// phase starts from kernel-argument constants instead of recomputing them.
+struct TailLevel {
+    FemColour F[4];
+    float rd0, rd1;           // 1 / D0, 1 / D1
+    int hx;                   // MAC: nodes of one colour per x-row (rounded up)
+    float rhx;
+    int off;                  // element offset of the level's x, b, r in LDS
+};
 template <typename T>
-__device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
+struct TailArgs {
+    MGTail<T> t;
+    TailLevel I[MG_TAIL_MAXLEV];
+    const T *pinv;            // pseudo-inverse of the coarsest operator, or null
+};
+
+template <typename T>
+__device__ void wg_smooth(const MGLevel<T> &L, const TailLevel &I, const Prd &P, int fem, int col) {
     int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2];
     if (fem) {
         // the colour's interior nodes first, as in k_fem_smooth
-        FemColour F = fem_colour(L.d, col);
+        const FemColour &F = I.F[col];
         int n = F.NI + F.NA + F.NB;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             int i, j, k;
@@ -741,11 +765,10 @@ __device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
             }
         }
     } else {
-        int hx = (D0 + 1) / 2, n = hx * D1 * D2;
-        float ra = 1.0f / hx, rb = 1.0f / D1;
+        int hx = I.hx, n = hx * D1 * D2;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             int ih, j, k;
-            unflat(t, hx, D1, ra, rb, ih, j, k);
+            unflat(t, hx, D1, I.rhx, I.rd1, ih, j, k);
             int i = 2 * ih + ((j + k + col) & 1);
             if (i < D0) mac_smooth_at(L, P, i, j, k);
         }
@@ -754,44 +777,33 @@ __device__ void wg_smooth(const MGLevel<T> &L, const Prd &P, int fem, int col) {
 }
 
 // IN_LDS: every tail level lives in LDS (x, b, r of each level packed in
-// that order, at offsets kept in LDS); the finest tail level's x and b are
-// loaded from HBM on entry and its x stored back on exit.  The level views
-// are rebuilt from the shared base at each use, so the compiler sees LDS
-// pointers and issues LDS instructions.  Otherwise the levels stay in HBM.
+// that order at the offset the host computed); the finest tail level's x and
+// b are loaded from HBM on entry and its x stored back on exit.  The level
+// views are rebuilt from the shared base at each use, so the compiler sees
+// LDS pointers and issues LDS instructions.  Otherwise the levels stay in
+// HBM.
 template <typename T, bool IN_LDS>
-__device__ __forceinline__ MGLevel<T> tail_level(const MGTail<T> &t, const int (*dims)[4], const int *offs,
-                                                 unsigned char *smem, int l) {
+__device__ __forceinline__ MGLevel<T> tail_level(const TailArgs<T> &A, unsigned char *smem, int l) {
     MGLevel<T> L;
-    L.d[0] = dims[l][0]; L.d[1] = dims[l][1]; L.d[2] = dims[l][2]; L.nn = dims[l][3];
+    L.d[0] = A.t.L[l].d[0]; L.d[1] = A.t.L[l].d[1]; L.d[2] = A.t.L[l].d[2]; L.nn = A.t.L[l].nn;
     if (IN_LDS) {
-        T *base = (T *)smem + offs[l];
+        T *base = (T *)smem + A.I[l].off;
         L.x = base; L.b = base + L.nn; L.r = base + 2 * L.nn;
     } else {
-        L.x = t.L[l].x; L.b = t.L[l].b; L.r = t.L[l].r;
+        L.x = A.t.L[l].x; L.b = A.t.L[l].b; L.r = A.t.L[l].r;
     }
     return L;
 }
 
 template <typename T, bool IN_LDS>
-__global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int vdown, int vup, int csweeps) {
+__global__ __launch_bounds__(1024) void k_tail(TailArgs<T> A, Prd P, int fem, int vdown, int vup, int csweeps) {
     extern __shared__ unsigned char smem[];
-    __shared__ int dims[MG_TAIL_MAXLEV][4];
-    __shared__ int offs[MG_TAIL_MAXLEV];
-    int ncol = fem ? 4 : 2;
-    int last = t.n - 1;
-    if (threadIdx.x == 0) {
-        int o = 0;
-        for (int l = 0; l < t.n; l++) {
-            for (int a = 0; a < 3; a++) dims[l][a] = t.L[l].d[a];
-            dims[l][3] = t.L[l].nn;
-            offs[l] = o;
-            o += 3 * t.L[l].nn;
-        }
-    }
-    __syncthreads();
+    const int ncol = fem ? 4 : 2;
+    const int last = A.t.n - 1;
     if (IN_LDS) {
         // several loads in flight per work-item before the LDS stores
-        MGLevel<T> L0 = tail_level<T, true>(t, dims, offs, smem, 0);
+        MGLevel<T> L0 = tail_level<T, true>(A, smem, 0);
+        const T *gx = A.t.L[0].x, *gb = A.t.L[0].b;
         const int stride = blockDim.x;
         for (int e0 = threadIdx.x; e0 < L0.nn; e0 += SWEEP_BATCH * stride) {
             T xv[SWEEP_BATCH], bv[SWEEP_BATCH];
@@ -799,8 +811,8 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
             for (int u = 0; u < SWEEP_BATCH; u++) {
                 int e = e0 + u * stride;
                 if (e < L0.nn) {
-                    xv[u] = t.L[0].x[e];
-                    bv[u] = t.L[0].b[e];
+                    xv[u] = gx[e];
+                    bv[u] = gb[e];
                 }
             }
 #pragma unroll
@@ -815,13 +827,14 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
         __syncthreads();
     }
     for (int l = 0; l < last; l++) {
-        const MGLevel<T> L = tail_level<T, IN_LDS>(t, dims, offs, smem, l);
-        const MGLevel<T> C = tail_level<T, IN_LDS>(t, dims, offs, smem, l + 1);
+        const MGLevel<T> L = tail_level<T, IN_LDS>(A, smem, l);
+        const MGLevel<T> C = tail_level<T, IN_LDS>(A, smem, l + 1);
+        const TailLevel &I = A.I[l];
         for (int sw = 0; sw < vdown; sw++)
-            for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
+            for (int c = 0; c < ncol; c++) wg_smooth(L, I, P, fem, c);
         if (fem) {
             for (int col = 0; col < 4; col++) {
-                FemColour F = fem_colour(L.d, col);
+                const FemColour &F = I.F[col];
                 int n = F.NI + F.NA + F.NB;
                 for (int e = threadIdx.x; e < n; e += blockDim.x) {
                     int i, j, k;
@@ -831,30 +844,43 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
                 }
             }
         } else {
-            float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
             for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
                 int i, j, k;
-                unflat(e, L.d[0], L.d[1], ra, rb, i, j, k);
+                unflat(e, L.d[0], L.d[1], I.rd0, I.rd1, i, j, k);
                 resid_at(L, P, 0, i, j, k);
             }
         }
         __syncthreads();
-        float rc0 = 1.0f / C.d[0], rc1 = 1.0f / C.d[1];
+        const TailLevel &IC = A.I[l + 1];
         for (int e = threadIdx.x; e < C.nn; e += blockDim.x) {
-            int I, J, K;
-            unflat(e, C.d[0], C.d[1], rc0, rc1, I, J, K);
-            restrict_at(L, C, P, fem, I, J, K);
+            int I_, J, K;
+            unflat(e, C.d[0], C.d[1], IC.rd0, IC.rd1, I_, J, K);
+            restrict_at(L, C, P, fem, I_, J, K);
         }
         __syncthreads();
     }
-    {
-        const MGLevel<T> W = tail_level<T, IN_LDS>(t, dims, offs, smem, last);
-        float ra = 1.0f / W.d[0], rb = 1.0f / W.d[1];
+    if (A.pinv) {
+        // exact coarsest solve x = A^+ b: one work-item per row, the row
+        // summed in column order (A^+ is symmetric, so column c of row r
+        // is read at c n + r and a wavefront's loads are contiguous)
+        const MGLevel<T> W = tail_level<T, IN_LDS>(A, smem, last);
+        const int n = W.nn;
+        const T *Pm = A.pinv;
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            T s = 0;
+#pragma unroll 8
+            for (int c = 0; c < n; c++) s += Pm[c * n + r] * W.b[c];
+            W.x[r] = s;
+        }
+        __syncthreads();
+    } else {
+        const MGLevel<T> W = tail_level<T, IN_LDS>(A, smem, last);
+        const TailLevel &I = A.I[last];
         for (int sw = 0; sw < csweeps; sw++)
             for (int c = 0; c < ncol; c++) {
                 for (int e = threadIdx.x; e < W.nn; e += blockDim.x) {
                     int i, j, k;
-                    unflat(e, W.d[0], W.d[1], ra, rb, i, j, k);
+                    unflat(e, W.d[0], W.d[1], I.rd0, I.rd1, i, j, k);
                     int my = fem ? ((j & 1) + 2 * (k & 1)) : ((i + j + k) & 1);
                     if (my != c) continue;
                     if (fem) fem_smooth_at(W, P, i, j, k);
@@ -864,21 +890,22 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail<T> t, Prd P, int fem, int 
             }
     }
     for (int l = last - 1; l >= 0; l--) {
-        const MGLevel<T> L = tail_level<T, IN_LDS>(t, dims, offs, smem, l);
-        const MGLevel<T> C = tail_level<T, IN_LDS>(t, dims, offs, smem, l + 1);
-        float ra = 1.0f / L.d[0], rb = 1.0f / L.d[1];
+        const MGLevel<T> L = tail_level<T, IN_LDS>(A, smem, l);
+        const MGLevel<T> C = tail_level<T, IN_LDS>(A, smem, l + 1);
+        const TailLevel &I = A.I[l];
         for (int e = threadIdx.x; e < L.nn; e += blockDim.x) {
             int i, j, k;
-            unflat(e, L.d[0], L.d[1], ra, rb, i, j, k);
+            unflat(e, L.d[0], L.d[1], I.rd0, I.rd1, i, j, k);
             prolong_at(L, C, P, fem, i, j, k);
         }
         __syncthreads();
         for (int sw = 0; sw < vup; sw++)
-            for (int c = 0; c < ncol; c++) wg_smooth(L, P, fem, c);
+            for (int c = 0; c < ncol; c++) wg_smooth(L, I, P, fem, c);
     }
     if (IN_LDS) {
-        MGLevel<T> L0 = tail_level<T, true>(t, dims, offs, smem, 0);
-        for (int e = threadIdx.x; e < L0.nn; e += blockDim.x) t.L[0].x[e] = L0.x[e];
+        MGLevel<T> L0 = tail_level<T, true>(A, smem, 0);
+        T *gx = A.t.L[0].x;
+        for (int e = threadIdx.x; e < L0.nn; e += blockDim.x) gx[e] = L0.x[e];
     }
 }
 
@@ -1012,7 +1039,7 @@ int mg_tail_enable_lds(size_t bytes) {
     int dev = 0, maxb = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
-    if (bytes + sizeof(int) * 5 * MG_TAIL_MAXLEV > (size_t)maxb) return 0;
+    if (bytes > (size_t)maxb) return 0;
     // raising the dynamic limit is required on some runtimes and a no-op on others
     (void)hipFuncSetAttribute((const void *)k_tail<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     (void)hipGetLastError();
@@ -1021,15 +1048,27 @@ int mg_tail_enable_lds(size_t bytes) {
 
 template <typename T>
 void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const int prd[3], int vdown, int vup,
-                    int csweeps, int lds) {
-    MGTail<T> t;
-    t.n = n;
-    for (int l = 0; l < n; l++) t.L[l] = L[l];
+                    int csweeps, int lds, const T *pinv) {
+    TailArgs<T> A{};
+    A.t.n = n;
+    A.pinv = pinv;
+    int off = 0;
+    for (int l = 0; l < n; l++) {
+        A.t.L[l] = L[l];
+        TailLevel &I = A.I[l];
+        for (int c = 0; c < 4; c++) I.F[c] = fem_colour(L[l].d, c);
+        I.rd0 = 1.0f / L[l].d[0];
+        I.rd1 = 1.0f / L[l].d[1];
+        I.hx = (L[l].d[0] + 1) / 2;
+        I.rhx = 1.0f / I.hx;
+        I.off = off;
+        off += 3 * L[l].nn;
+    }
     if (lds)
-        hipLaunchKernelGGL((k_tail<T, true>), dim3(1), dim3(1024), mg_tail_lds_bytes(L, n), s, t, mkprd(prd), fem, vdown,
+        hipLaunchKernelGGL((k_tail<T, true>), dim3(1), dim3(1024), mg_tail_lds_bytes(L, n), s, A, mkprd(prd), fem, vdown,
                            vup, csweeps);
     else
-        hipLaunchKernelGGL((k_tail<T, false>), dim3(1), dim3(1024), 0, s, t, mkprd(prd), fem, vdown, vup, csweeps);
+        hipLaunchKernelGGL((k_tail<T, false>), dim3(1), dim3(1024), 0, s, A, mkprd(prd), fem, vdown, vup, csweeps);
 }
 void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf) {
     NodeMap M = node_map(L.d, fem);
@@ -1054,7 +1093,8 @@ void launch_zero_f(hipStream_t s, float *a, int n) {
     template void launch_mg_resid<T>(hipStream_t, int, const MGLevel<T> &, const int *);                    \
     template void launch_mg_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *); \
     template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *);  \
-    template void launch_mg_tail<T>(hipStream_t, int, const MGLevel<T> *, int, const int *, int, int, int, int); \
+    template void launch_mg_tail<T>(hipStream_t, int, const MGLevel<T> *, int, const int *, int, int, int, int,  \
+                                    const T *);                                                              \
     template size_t mg_tail_lds_bytes<T>(const MGLevel<T> *, int);                                          \
     template int mg_tail_enable_lds<T>(size_t);
 INST(double)

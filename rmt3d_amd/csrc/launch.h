@@ -58,8 +58,12 @@ struct MGLevel {
     T *x, *b, *r;
 };
 
+// a level with at most this many unknowns ends the hierarchy and is solved
+// exactly with the pseudo-inverse of its operator
+constexpr int MG_DIRECT_NN = 160;
+
 // the levels below a size threshold, processed by one workgroup
-constexpr int MG_TAIL_MAXLEV = 12;
+constexpr int MG_TAIL_MAXLEV = 8;       // a level of <= 6144 unknowns coarsens at most 4 more times
 constexpr int MG_TAIL_NN = 6144;       // largest level handed to the tail kernel
 constexpr size_t MG_TAIL_LDS_MAX = 160 * 1024 - 1024;   // dynamic LDS of the tail (gfx950: 160 KiB per workgroup)
 template <typename T>
@@ -101,10 +105,12 @@ template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
 template <typename T>
 void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
-// the rest of a V-cycle from L[0] (finest of the tail) to L[n-1] (coarsest)
+// the rest of a V-cycle from L[0] (finest of the tail) to L[n-1] (coarsest);
+// pinv (n_c x n_c, symmetric) solves the coarsest level exactly, or null for
+// csweeps smoothing sweeps there
 template <typename T>
 void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const int prd[3], int vdown, int vup,
-                    int csweeps, int lds);
+                    int csweeps, int lds, const T *pinv);
 // LDS bytes the tail needs for levels L[0..n-1]; enable returns 1 when the
 // kernel may use that much dynamic LDS on the current device
 template <typename T>

@@ -365,6 +365,57 @@ def test_mg_solution_matches_dense_solve(which, dims, prd):
     assert np.linalg.norm(dx) <= 1e-9 * np.linalg.norm(xd)
 
 
+# ------------------------------------------------ exact coarsest-level solve
+@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("dims,prd", [((4, 4, 4), (0, 0, 0)), ((4, 4, 4), (1, 1, 1)), ((6, 4, 4), (1, 0, 1)),
+                                      ((2, 2, 2), (0, 0, 0)), ((5, 3, 2), (0, 0, 0))])
+def test_pinv_of_operator_is_moore_penrose(which, dims, prd):
+    # the four Moore-Penrose conditions and numpy's SVD pseudo-inverse, on the
+    # (singular: constant null space) Neumann / periodic operators
+    from oracle.oracle import sym_pinv
+    cfg, objs = _mg_cfg(dims, prd)
+    S = OracleSim(cfg, objs)
+    d, A = _dense(S, which)
+    P = sym_pinv(A)
+    scale = np.abs(A).max()
+    assert np.array_equal(P, P.T)
+    assert np.abs(A @ P @ A - A).max() <= 1e-12 * scale
+    assert np.abs(P @ A @ P - P).max() <= 1e-12 * np.abs(P).max()
+    assert np.abs((A @ P).T - A @ P).max() <= 1e-12
+    assert np.abs(P @ np.ones(len(A))).max() <= 1e-12 * np.abs(P).max()     # minimum norm: P 1 = 0
+    assert np.allclose(P, np.linalg.pinv(A), rtol=0, atol=1e-11 * np.abs(P).max())
+
+
+def test_pinv_of_spd_matrix_is_inverse():
+    from oracle.oracle import sym_pinv
+    rng = np.random.default_rng(7)
+    M = rng.standard_normal((40, 40))
+    A = M @ M.T + 40 * np.eye(40)
+    P = sym_pinv(A)
+    assert np.abs(P @ A - np.eye(40)).max() <= 1e-13
+
+
+@pytest.mark.parametrize("which,dims,prd,nlev", [(0, (4, 4, 4), (0, 0, 0), 1), (1, (4, 4, 4), (0, 0, 0), 1),
+                                                 (0, (16, 16, 16), (0, 0, 0), 3), (1, (16, 16, 16), (0, 0, 0), 3),
+                                                 (0, (8, 8, 8), (1, 1, 1), 2)])
+def test_hierarchy_ends_at_direct_level(which, dims, prd, nlev):
+    # coarsening stops at the first level of at most 160 unknowns, which the
+    # pseudo-inverse solves exactly; a one-level hierarchy converges in the
+    # first cycle
+    cfg, objs = _mg_cfg(dims, prd)
+    S = OracleSim(cfg, objs)
+    levels = S.mg_levels(which)
+    assert len(levels) == nlev
+    assert int(np.prod(levels[-1])) <= 160 < (int(np.prod(levels[-2])) if nlev > 1 else 161)
+    assert S.mg_direct(which)
+    if nlev == 1:
+        d, A = _dense(S, which)
+        b = np.random.default_rng(1).standard_normal(len(A))
+        b -= b.mean()
+        x, ncyc, res = S.mg_solve(which, b, np.zeros(len(A)))
+        assert np.abs(A @ x - b).max() <= 1e-12
+
+
 # ---------------------------------------------------------------- band layers
 def _bfs_layers(rm_c, oid, pad_dims, nlayers):
     """6-neighbour BFS distance from the object's primary cells over the
