@@ -939,16 +939,20 @@ static inline Prd mkprd(const int p[3]) { Prd P; P.p[0] = p[0]; P.p[1] = p[1]; P
 template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color, hipEvent_t start,
                       hipEvent_t stop) {
+    // workgroup sizes measured on the 128^3 / 129^3 levels: 1024 for the
+    // red-black pass (7.8 us against 8.1 at 256, FP32; 8.8 against 10.0,
+    // FP64); 64 for the FEM colour pass, whose wall-row work-items cost
+    // several times an interior one and finish last, so small workgroups
+    // spread them over more CUs (6.9 us against 8.9 at 256, FP32)
     Prd P = mkprd(prd);
     if (!fem) {
         int hx = (L.d[0] + 1) / 2, n = hx * L.d[1] * L.d[2];
-        hipExtLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, start, stop, 0, L, P, color,
-                              hx, 1.0f / hx, 1.0f / L.d[1]);
+        hipExtLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 1023) / 1024), dim3(1024), 0, s, start, stop, 0, L, P,
+                              color, hx, 1.0f / hx, 1.0f / L.d[1]);
     } else {
         FemColour F = fem_colour(L.d, color);
         int n = F.NI + F.NA + F.NB;
-        hipExtLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 255) / 256), dim3(256), 0, s, start, stop, 0, L, P, color,
-                              F);
+        hipExtLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 63) / 64), dim3(64), 0, s, start, stop, 0, L, P, color, F);
     }
 }
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
