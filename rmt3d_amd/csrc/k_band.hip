@@ -237,14 +237,18 @@ __device__ __forceinline__ double nb_weight(const Geo &g, const Scratch &W, int 
 // reference's loops, and park each point's terms in LDS; then one lane per
 // sum accumulates its sum over the points in that order.  Every sum is thus
 // the same sequence of additions as the serial loop of extrap_helper
-// (bit-identical), while the stencil gathers of a cell run 64-wide.
+// (bit-identical), while the stencil gathers of a cell run 64-wide.  A point
+// that is not a data point parks +0 terms: the running sums start at +0 and
+// are never -0, so adding them leaves the sums bit-identical and the summing
+// loops are branch-free.  Terms are stored point-major ([point][quantity]),
+// so the summing lanes read adjacent words and their loads can be issued
+// ahead of the dependent additions.
 constexpr int LSQ_T1 = 7;    // ravg[3], davg[3], wsum
 constexpr int LSQ_T2 = 15;   // am[6], rhs[9]
 
 __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt,
                                                    LsqArgs A, int *status) {
-    __shared__ double term[LSQ_T2][64];
-    __shared__ int valid[64];
+    __shared__ double term[64][LSQ_T2 + 1];
     __shared__ double bc[LSQ_T2];
     const int lane = threadIdx.x;
     int n = *cnt;
@@ -272,23 +276,24 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
                     w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
                 }
                 bool v = w >= 0;
-                valid[lane] = v;
                 if (v) {
                     int nb = ind + p + g.m4 * q + g.mn4 * r;
-                    term[0][lane] = W.val[0][nb] * w;
-                    term[1][lane] = W.val[1][nb] * w;
-                    term[2][lane] = W.val[2][nb] * w;
-                    term[3][lane] = g.lx[ii + p] * w;
-                    term[4][lane] = g.ly[jj + q] * w;
-                    term[5][lane] = g.lz[kk + r] * w;
-                    term[6][lane] = w;
+                    term[lane][0] = W.val[0][nb] * w;
+                    term[lane][1] = W.val[1][nb] * w;
+                    term[lane][2] = W.val[2][nb] * w;
+                    term[lane][3] = g.lx[ii + p] * w;
+                    term[lane][4] = g.ly[jj + q] * w;
+                    term[lane][5] = g.lz[kk + r] * w;
+                    term[lane][6] = w;
+                } else {
+                    for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
                 }
                 npt += __popcll(__ballot(v));
                 __syncthreads();
                 if (lane < LSQ_T1) {
                     int ne = min(64, nc - k0);
-                    for (int e = 0; e < ne; e++)
-                        if (valid[e]) acc += term[lane][e];
+#pragma unroll 8
+                    for (int e = 0; e < ne; e++) acc += term[e][lane];
                 }
                 __syncthreads();
             }
@@ -311,28 +316,29 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
                     Q = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
                 }
                 bool v = Q >= 0;
-                valid[lane] = v;
                 if (v) {
                     int nb = ind + p + g.m4 * q + g.mn4 * r;
                     double dp0 = g.lx[ii + p], dp1 = g.ly[jj + q], dp2 = g.lz[kk + r];
-                    term[0][lane] = Q * (dp0 - davg[0]) * (dp0 - davg[0]);
-                    term[1][lane] = Q * (dp1 - davg[1]) * (dp1 - davg[1]);
-                    term[2][lane] = Q * (dp2 - davg[2]) * (dp2 - davg[2]);
-                    term[3][lane] = Q * (dp0 - davg[0]) * (dp1 - davg[1]);
-                    term[4][lane] = Q * (dp0 - davg[0]) * (dp2 - davg[2]);
-                    term[5][lane] = Q * (dp1 - davg[1]) * (dp2 - davg[2]);
+                    term[lane][0] = Q * (dp0 - davg[0]) * (dp0 - davg[0]);
+                    term[lane][1] = Q * (dp1 - davg[1]) * (dp1 - davg[1]);
+                    term[lane][2] = Q * (dp2 - davg[2]) * (dp2 - davg[2]);
+                    term[lane][3] = Q * (dp0 - davg[0]) * (dp1 - davg[1]);
+                    term[lane][4] = Q * (dp0 - davg[0]) * (dp2 - davg[2]);
+                    term[lane][5] = Q * (dp1 - davg[1]) * (dp2 - davg[2]);
                     double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
                     double dd[3] = {dp0 - davg[0], dp1 - davg[1], dp2 - davg[2]};
                     for (int i = 0; i < 3; i++) {
                         double lr = Q * (rv[i] - ravg[i]);
-                        for (int j = 0; j < 3; j++) term[6 + 3 * i + j][lane] = dd[j] * lr;
+                        for (int j = 0; j < 3; j++) term[lane][6 + 3 * i + j] = dd[j] * lr;
                     }
+                } else {
+                    for (int a = 0; a < LSQ_T2; a++) term[lane][a] = 0.0;
                 }
                 __syncthreads();
                 if (lane < LSQ_T2) {
                     int ne = min(64, nc - k0);
-                    for (int e = 0; e < ne; e++)
-                        if (valid[e]) acc += term[lane][e];
+#pragma unroll 8
+                    for (int e = 0; e < ne; e++) acc += term[e][lane];
                 }
                 __syncthreads();
             }

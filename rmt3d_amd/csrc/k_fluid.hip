@@ -300,9 +300,11 @@ __global__ __launch_bounds__(1024) void k_stress_fluid(Geo g, Phys P, Fluid U, M
         }
 }
 
-// one thread per solid cell, one launch per face direction
+// one thread per solid cell, one launch per face direction; launched in
+// 64-work-item workgroups, which the bound tells the compiler so that the
+// per-cell state fits in registers instead of spilling to scratch
 template <int F>
-__global__ void k_stress_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
+__global__ __launch_bounds__(64) void k_stress_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
     int n = *count;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
         stress_face<F>(g, P, O, U, M, list[t]);
@@ -519,7 +521,7 @@ __global__ void k_advect(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
 // (normal_derivatives_refmap + compute_half_time_edge_refmap), which read
 // only the cell-centred velocity
 template <bool FORCING>
-__global__ void k_advect_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count,
+__global__ __launch_bounds__(64) void k_advect_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count,
                                int *status) {
     int n = *count;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
@@ -581,7 +583,7 @@ __device__ __forceinline__ bool upwind_map(const Geo &g, const Fluid &U, const M
     return true;
 }
 
-__global__ void k_upwind(Geo g, Fluid U, Maps M, int *status) {
+__global__ __launch_bounds__(64) void k_upwind(Geo g, Fluid U, Maps M, int *status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y + 1, k = blockIdx.z + 1;
     if (i > g.m + 1) return;
@@ -839,7 +841,7 @@ __global__ void k_ustar(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
 }
 
 template <bool FORCING>
-__global__ void k_ustar_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
+__global__ __launch_bounds__(64) void k_ustar_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
     int n = *count;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         int c = list[t];
