@@ -876,11 +876,24 @@ struct GradPhi { double v[8][3]; };
 
 struct Faces6 { FaceC f[6]; };
 
-// one thread per node (flat index over fm x fn x fo)
-__global__ void k_pp_rhs(Geo g, Fluid U, GradPhi gp, Faces6 faces, int nwall, double *r) {
+// quotient of non-negative integers from a float reciprocal, corrected by
+// the exact remainder
+__device__ __forceinline__ int fdivq(int x, int d, float rd) {
+    int q = (int)((float)x * rd);
+    int r = x - q * d;
+    while (r < 0) { q--; r += d; }
+    while (r >= d) { q++; r -= d; }
+    return q;
+}
+
+// one thread per node (flat index over fm x fn x fo); rfm, rfn: 1/fm, 1/fn
+__global__ void k_pp_rhs(Geo g, Fluid U, GradPhi gp, Faces6 faces, int nwall, double *r, float rfm, float rfn) {
     int tt = blockIdx.x * blockDim.x + threadIdx.x;
     if (tt >= g.fm * g.fn * g.fo) return;
-    int a = tt % g.fm, b = (tt / g.fm) % g.fn, cz = tt / (g.fm * g.fn);
+    int row = fdivq(tt, g.fm, rfm);
+    int a = tt - row * g.fm;
+    int cz = fdivq(row, g.fn, rfn);
+    int b = row - cz * g.fn;
     double acc = 0;
     // elements in (k, j, i) ascending order: corner n = 7..0
 #pragma unroll
@@ -896,8 +909,10 @@ __global__ void k_pp_rhs(Geo g, Fluid U, GradPhi gp, Faces6 faces, int nwall, do
     // boundary flux of the wall faces (fluid_3d.cc:2591-2645): node N on face s
     // receives from the elements E = N - corner whose normal coordinate is the
     // face's boundary cell, in the reference's element order (higher
-    // tangential dimension outer)
-    for (int w = 0; w < nwall; w++) {
+    // tangential dimension outer).  A wall face holds only nodes of index 0
+    // or the last in its normal dimension.
+    bool edge = a == 0 || a == g.fm - 1 || b == 0 || b == g.fn - 1 || cz == 0 || cz == g.fo - 1;
+    for (int w = 0; w < (edge ? nwall : 0); w++) {
         const FaceC &F = faces.f[w];
         int s = F.f, nd = F.n;
         int N[3] = {a, b, cz};
@@ -1188,7 +1203,8 @@ void launch_pp_rhs(hipStream_t s, const Geo &g, const Fluid &U, const double gra
     memset(&fc, 0, sizeof fc);
     for (int w = 0; w < nwall && w < 6; w++) fc.f[w] = make_face(g, faces[w]);
     int nn = g.fm * g.fn * g.fo;
-    hipLaunchKernelGGL(k_pp_rhs, dim3((nn + 255) / 256), dim3(256), 0, s, g, U, gp, fc, nwall, r);
+    hipLaunchKernelGGL(k_pp_rhs, dim3((nn + 255) / 256), dim3(256), 0, s, g, U, gp, fc, nwall, r, 1.0f / g.fm,
+                       1.0f / g.fn);
 }
 size_t face_struct_size() { return sizeof(FaceC) * 6; }
 void launch_pp_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x, const double *mean) {
