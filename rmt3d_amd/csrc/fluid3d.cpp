@@ -241,6 +241,7 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     log_tol = log10(tol);
     hipMalloc(&d_max, sizeof(unsigned long long));
     hipHostMalloc((void **)&h_max, sizeof(unsigned long long), hipHostMallocDefault);
+    hipEventCreateWithFlags(&ev_check, hipEventDisableTiming);
 }
 
 void MGSolver::release() {
@@ -262,7 +263,22 @@ void MGSolver::release() {
     d_pinv = nullptr; f_pinv = nullptr;
     if (d_max) hipFree(d_max);
     if (h_max) hipHostFree(h_max);
-    d_max = nullptr; h_max = nullptr;
+    if (ev_check) hipEventDestroy(ev_check);
+    d_max = nullptr; h_max = nullptr; ev_check = nullptr;
+}
+
+void MGSolver::check_queue(hipStream_t s) {
+    launch_zero_u64(s, d_max, 1);
+    launch_mg_resid64_max(s, fem, L[0], prd, d_max);
+    hipMemcpyAsync(h_max, d_max, sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+    hipEventRecord(ev_check, s);
+}
+
+double MGSolver::check_wait() {
+    hipEventSynchronize(ev_check);
+    double v;
+    memcpy(&v, h_max, sizeof v);
+    return v;
 }
 
 double MGSolver::resid_max(hipStream_t s) {
@@ -400,21 +416,33 @@ void MGSolver::timed_cycle(hipStream_t s) {
 }
 
 // mg_prob::run (Utilities/mg_probs.cc:3-83), profile = false
+// The residual before the first cycle is read only in the reference's
+// profiling mode (mg_probs.cc:9-13), so it is not taken.  Every later
+// residual check is followed by at least one more V-cycle: the first of the
+// next extra batch, or the first insurance cycle.  That cycle is queued
+// right behind the check and its read-back, and the host waits on the
+// read-back's event alone, so the GPU runs the cycle while the host decides.
+// The V-cycles and checks run in the reference's order on the same iterates.
 double MGSolver::run(hipStream_t s) {
     const int hard_cap = 20;
     double thr = tol_override > 0 ? tol_override : max_norm_tol;
     int cycles = 0;
-    double res = resid_max(s);
     int pre = iV / iP;
     for (int i = 0; i < pre; i++) { timed_cycle(s); cycles++; }
-    res = resid_max(s);
+    check_queue(s);
+    timed_cycle(s); cycles++;             // queued: needed whatever the check says
+    double res = check_wait();
     int extra = 0;
     while (res > thr && extra < hard_cap) {
         extra++;
-        for (int n = 0; n < extra; n++) { timed_cycle(s); cycles++; }
-        res = resid_max(s);
+        // the batch's first cycle is already queued
+        for (int n = 1; n < extra; n++) { timed_cycle(s); cycles++; }
+        check_queue(s);
+        timed_cycle(s); cycles++;
+        res = check_wait();
     }
-    for (int i = 0; i < 2; i++) { timed_cycle(s); cycles++; }
+    // the first insurance cycle is the one queued behind the last check
+    timed_cycle(s); cycles++;
     // mg_pred::update(int) (mg_probs.hh:51-66)
     double dV = iV - V_avg;
     int exreduce = (int)(dV / sqrt(V_var));

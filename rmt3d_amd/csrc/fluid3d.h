@@ -72,6 +72,11 @@ struct MGSolver {
     void setup(int fem_, const int d[3], const int prd_[3], bool mixed_);
     void release();
     double resid_max(hipStream_t s);
+    // residual check in two halves: queue the max-norm and its read-back,
+    // then wait for that read-back only
+    void check_queue(hipStream_t s);
+    double check_wait();
+    hipEvent_t ev_check = nullptr;
     void vcycle64(hipStream_t s, int lev);
     void vcycle32(hipStream_t s, int lev);
     void cycle(hipStream_t s);              // one V-cycle (or one defect-correction step)
