@@ -264,10 +264,14 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
         double davg[3], ravg[3], coeff[9];
         for (int sr = 2; !ok && sr <= A.max_rs; sr++) {
             const int D = 2 * sr + 1, nc = D * D * D;
+            // the 5^3 stencil is two chunks: each lane keeps its two points'
+            // weights and data in registers for the second pass
+            const bool keep = nc <= 2 * 64;
+            double kw[2] = {-1, -1}, krv[2][3], kdp[2][3];
             // pass 1: weighted sums (extrap_helper.cc:49-62)
             double acc = 0;
             int npt = 0;
-            for (int k0 = 0; k0 < nc; k0 += 64) {
+            for (int k0 = 0, ch = 0; k0 < nc; k0 += 64, ch++) {
                 int k = k0 + lane;
                 double w = -1;
                 int p = 0, q = 0, r = 0;
@@ -278,13 +282,26 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
                 bool v = w >= 0;
                 if (v) {
                     int nb = ind + p + g.m4 * q + g.mn4 * r;
-                    term[lane][0] = W.val[0][nb] * w;
-                    term[lane][1] = W.val[1][nb] * w;
-                    term[lane][2] = W.val[2][nb] * w;
-                    term[lane][3] = g.lx[ii + p] * w;
-                    term[lane][4] = g.ly[jj + q] * w;
-                    term[lane][5] = g.lz[kk + r] * w;
+                    double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
+                    double dp[3] = {g.lx[ii + p], g.ly[jj + q], g.lz[kk + r]};
+                    term[lane][0] = rv[0] * w;
+                    term[lane][1] = rv[1] * w;
+                    term[lane][2] = rv[2] * w;
+                    term[lane][3] = dp[0] * w;
+                    term[lane][4] = dp[1] * w;
+                    term[lane][5] = dp[2] * w;
                     term[lane][6] = w;
+                    if (keep) {
+                        // ch is 0 or 1 here; selects keep the arrays in registers
+                        for (int a = 0; a < 3; a++) {
+                            krv[0][a] = ch ? krv[0][a] : rv[a];
+                            kdp[0][a] = ch ? kdp[0][a] : dp[a];
+                            krv[1][a] = ch ? rv[a] : krv[1][a];
+                            kdp[1][a] = ch ? dp[a] : kdp[1][a];
+                        }
+                        kw[0] = ch ? kw[0] : w;
+                        kw[1] = ch ? w : kw[1];
+                    }
                 } else {
                     for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
                 }
@@ -307,25 +324,34 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
             for (int a = 0; a < 3; a++) davg[a] /= wsum;
             // pass 2: centred moments (extrap_helper.cc:68-75, 129-146)
             acc = 0;
-            for (int k0 = 0; k0 < nc; k0 += 64) {
+            for (int k0 = 0, ch = 0; k0 < nc; k0 += 64, ch++) {
                 int k = k0 + lane;
                 double Q = -1;
-                int p = 0, q = 0, r = 0;
-                if (k < nc) {
-                    p = k % D - sr; q = (k / D) % D - sr; r = k / (D * D) - sr;
+                double rv[3] = {0, 0, 0}, dpv[3] = {0, 0, 0};
+                if (keep) {
+                    Q = ch ? kw[1] : kw[0];
+                    for (int a = 0; a < 3; a++) {
+                        rv[a] = ch ? krv[1][a] : krv[0][a];
+                        dpv[a] = ch ? kdp[1][a] : kdp[0][a];
+                    }
+                } else if (k < nc) {
+                    int p = k % D - sr, q = (k / D) % D - sr, r = k / (D * D) - sr;
                     Q = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
+                    if (Q >= 0) {
+                        int nb = ind + p + g.m4 * q + g.mn4 * r;
+                        for (int a = 0; a < 3; a++) rv[a] = W.val[a][nb];
+                        dpv[0] = g.lx[ii + p]; dpv[1] = g.ly[jj + q]; dpv[2] = g.lz[kk + r];
+                    }
                 }
                 bool v = Q >= 0;
                 if (v) {
-                    int nb = ind + p + g.m4 * q + g.mn4 * r;
-                    double dp0 = g.lx[ii + p], dp1 = g.ly[jj + q], dp2 = g.lz[kk + r];
+                    double dp0 = dpv[0], dp1 = dpv[1], dp2 = dpv[2];
                     term[lane][0] = Q * (dp0 - davg[0]) * (dp0 - davg[0]);
                     term[lane][1] = Q * (dp1 - davg[1]) * (dp1 - davg[1]);
                     term[lane][2] = Q * (dp2 - davg[2]) * (dp2 - davg[2]);
                     term[lane][3] = Q * (dp0 - davg[0]) * (dp1 - davg[1]);
                     term[lane][4] = Q * (dp0 - davg[0]) * (dp2 - davg[2]);
                     term[lane][5] = Q * (dp1 - davg[1]) * (dp2 - davg[2]);
-                    double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
                     double dd[3] = {dp0 - davg[0], dp1 - davg[1], dp2 - davg[2]};
                     for (int i = 0; i < 3; i++) {
                         double lr = Q * (rv[i] - ravg[i]);
