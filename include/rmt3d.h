@@ -91,6 +91,17 @@ typedef struct rmt3d_config {
     double vel_profile[2 * RMT3D_MAX_VEL_PROF];
     int    dump_code, ntracers, debug_flag, chk_num, chk_step;
     int    n_obj;
+    /* output and restart keys (Sim/sim_params.hh:20-84): carried through to
+     * recover.cfg and the slice writer; defaults as sim_params.cc:12-20 */
+    int    suppress, display, chkpt_freq, omp_num_thr, debug_obj_id;
+    int    obj_body, data_fmt, out_flag, output_dim, output_ind;
+    double current_time;         /* cur_time (restart time)                 */
+    char   dirname[256];         /* output directory: the .cfg path with
+                                    ".cfg" replaced by ".out"               */
+    char   recover_dirname[256]; /* where chk.%05d directories are sought   */
+    char   chk_dirname[256];     /* recover_dirname/chk.<chk_num>, set by the
+                                    parser when its recover.cfg exists; ""
+                                    otherwise (sim_params.cc:374-398)      */
 } rmt3d_config;
 
 typedef struct rmt3d_ctx rmt3d_ctx;
@@ -134,7 +145,14 @@ void rmt3d_destroy(rmt3d_ctx *ctx);
 const char *rmt3d_last_error(const rmt3d_ctx *ctx);
 
 /* fluid_3d::initialize (fluid_3d.cc:192-233): returns init_err (1 = from
- * scratch) or a negative library error. */
+ * scratch) or a negative library error.  With cfg.chk_num > 0 and a
+ * non-empty cfg.chk_dirname the state is read from the checkpoint files
+ * there (fluid_3d::initialize_from_chk_point, fluid_3d_chk_pt.cc:74-163):
+ * 0 = resumed at cfg.current_time; 1 = a file is missing, 2 = a file is
+ * empty, and in both cases the simulation is initialised from scratch with
+ * chk_num reset to 0, as the reference does.  Files are read uncompressed:
+ * where only "<name>.xz" exists the call fails with RMT3D_ERR_ARG (the
+ * reference runs `xz -d` on it first). */
 int rmt3d_initialize(rmt3d_ctx *ctx);
 /* The run_sim.cc:98-106 sequence between initialize() and run(): clamp
  * spars.dt to dt_reg, build the stepper's dt (stepper.hh:211-225), then
@@ -157,6 +175,33 @@ int rmt3d_upload_state(rmt3d_ctx *ctx, const rmt3d_state *st);
 int rmt3d_download_state(rmt3d_ctx *ctx, rmt3d_state *st);
 /* Number of band entries currently held (to size rmt3d_state arrays). */
 int rmt3d_band_count(rmt3d_ctx *ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Checkpoint and slice files (the reference's wire formats). */
+
+/* fluid_3d::write_chk_pt (fluid_3d_chk_pt.cc:3-72): writes the eight files
+ * vx vy vz p X Y Z OID (write_params::default_names 0,1,2,7,9,10,11,12) of
+ * the current state into the existing directory chk_dirname, each o raw
+ * FP64 z-slices of (m+1)(n+1) values in the reference's slice buffer layout,
+ * then recover.cfg with chk_num = fnum, chk_step = snum, current_time = the
+ * context's time.  Files are left uncompressed (the reference pipes each
+ * through `xz`).  Also records fnum / snum / time in the context's config. */
+int rmt3d_write_checkpoint(rmt3d_ctx *ctx, int snum, int fnum, const char *chk_dirname);
+/* sim_params::write_params (Sim/sim_params.cc:489-606): writes
+ * <dir>/recover.cfg from cfg and its n_obj objects. */
+int rmt3d_write_params(const rmt3d_config *cfg, const rmt3d_object *objs, const char *dir);
+/* fluid_3d::slice (fluid_3d_io.cc:98-418) on one GPU: the global slice at
+ * index `point` (0 <= point < cells along dim) normal to dim (0 x, 1 y,
+ * 2 z) of output type o_type (0..21, IO/write_params.hh:6-27; obj_id -1
+ * picks each point's object by minimum phi).  out receives d1*d2 values,
+ * d1 fastest, where (d1, d2) are the two other dimensions in increasing
+ * order (FEM node counts for the corner fields 3,4,5,7,18); *len is the
+ * capacity on entry and the count on exit. */
+int rmt3d_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len);
+/* fluid_3d::write_slice (fluid_3d_io.cc:3-65): format 0 matrix text
+ * (%17.14g), 1 coordinate text, 2 gnuplot binary (float). */
+int rmt3d_write_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, int format,
+                      const char *filename);
 
 /* Multigrid solve behind mg_prob::run (Utilities/mg_probs.cc:3-83):
  * which = 0 MAC (cell-centred 7-point, simp_mac, mg_probs.hh:497-530),

@@ -179,6 +179,18 @@ double orc_phi(const orc_sim *S, int obj_index, const double xi[3]);
 double orc_heaviside(const orc_sim *S, double phi);
 void orc_compute_stress(orc_sim *S);
 
+void orc_init_extrapolate(orc_sim *S);
+void orc_fill_pp_rhs(orc_sim *S, double *rp);
+
+/* orc_io.c: checkpoint and slice formats */
+int orc_init_from_chk_point(orc_sim *S);
+double orc_slice_point(orc_sim *S, const double *r0, int o_type, int obj_id, int i, int j, int k);
+int orc_slice(orc_sim *S, int dim, int point, int o_type, int obj_id, double *g_val);
+void orc_chk_images(orc_sim *S, double *img);
+void orc_chk_restore(orc_sim *S, const double *img);
+int orc_chk_read(const char *dir, int o, size_t L, double *img);
+int orc_write_chk_pt(orc_sim *S, const char *dir);
+
 /* orc_extrap.c */
 void orc_band_init(orc_band *B, int Ng);
 void orc_band_free(orc_band *B);

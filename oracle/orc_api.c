@@ -110,6 +110,41 @@ int oracle_upload(orc_sim *S, const rmt3d_state *st)
     return 0;
 }
 
+/* Replaces the band by n entries (sorted by cell; entries of one cell in
+ * multimap order), for format tests on hand-made states. */
+int oracle_set_band(orc_sim *S, int n, const int *cell, const uint32_t *c, const double *x, const double *xpred)
+{
+    orc_band *B = &S->ext;
+    orc_band_reset(B, S->Ng);
+    if (n > B->cap) {
+        while (B->cap < n) B->cap *= 2;
+        B->e = (orc_bentry *)realloc(B->e, sizeof(orc_bentry) * (size_t)B->cap);
+    }
+    for (int a = 0; a < n; a++) {
+        if (cell[a] < 0 || cell[a] >= S->Ng || (a > 0 && cell[a] < cell[a - 1])) return -1;
+        orc_bentry *e = &B->e[a];
+        memset(e, 0, sizeof *e);
+        e->cell = cell[a];
+        e->r.c = c[a];
+        for (int b = 0; b < 3; b++) { e->r.x[b] = x[(size_t)b * n + a]; e->r.xpred[b] = xpred[(size_t)b * n + a]; }
+        for (int f = 0; f < 3; f++)
+            for (int b = 0; b < 3; b++) e->r.gphi[f][b] = -1.e200;
+        if (B->cnt[cell[a]] == 0) B->start[cell[a]] = a;
+        B->cnt[cell[a]]++;
+    }
+    B->n = n;
+    return 0;
+}
+
+/* checkpoint and slice formats (orc_io.c) */
+void oracle_chk_images(orc_sim *S, double *img) { orc_chk_images(S, img); }
+void oracle_chk_restore(orc_sim *S, const double *img) { orc_chk_restore(S, img); }
+int oracle_chk_read(const char *dir, int o, size_t L, double *img) { return orc_chk_read(dir, o, L, img); }
+int oracle_write_chk_pt(orc_sim *S, const char *dir) { return orc_write_chk_pt(S, dir); }
+int oracle_slice(orc_sim *S, int dim, int point, int t, int obj_id, double *g) { return orc_slice(S, dim, point, t, obj_id, g); }
+int oracle_get_nt(orc_sim *S) { return S->nt; }
+int oracle_chk_num(orc_sim *S) { return S->cfg.chk_num; }
+
 /* Multigrid solve on a stand-alone problem: which 0 MAC, 1 FEM.  Uses the
  * simulation's solver objects (initial guess = x on entry). */
 int oracle_mg_solve(orc_sim *S, int which, const double *rhs, double *x, double tol, int *ncyc, double *res)

@@ -467,9 +467,23 @@ static void compute_grad_basis(orc_sim *S)
     }
 }
 
-/* fluid_3d::initialize (fluid_3d.cc:192-233), from scratch */
+static void fill_pp_rhs(orc_sim *S, double *rp);
+
+/* fluid_3d::initialize (fluid_3d.cc:192-233): from the checkpoint named by
+ * cfg.chk_dirname when chk_num > 0, else (or when that fails) from scratch */
 int orc_initialize(orc_sim *S)
 {
+    int init_err = 1;
+    if (S->cfg.chk_num > 0 && S->cfg.chk_dirname[0]) {
+        init_err = orc_init_from_chk_point(S);
+        if (init_err < 0) return init_err;
+        if (init_err == 0) {
+            orc_set_bc_cc(S);
+            compute_grad_basis(S);
+            return 0;
+        }
+    }
+    S->cfg.chk_num = 0;
     S->time = 0;
     init_fluid(S);
     init_pres(S);
@@ -477,8 +491,11 @@ int orc_initialize(orc_sim *S)
     init_extrapolate(S);
     orc_set_bc_cc(S);
     compute_grad_basis(S);
-    return 1;
+    return init_err;
 }
+
+void orc_init_extrapolate(orc_sim *S) { init_extrapolate(S); }
+void orc_fill_pp_rhs(orc_sim *S, double *rp) { fill_pp_rhs(S, rp); }
 
 /* ------------------------------------------------------------------ */
 /* advection (a)                                                        */
@@ -1396,7 +1413,10 @@ int orc_prepare_run(orc_sim *S, int init_err, double *dt_out, int *total)
     int steps = (int)(df / sdt);
     double rem = df - steps * sdt;
     if (rem > 0) sdt -= (sdt - rem) / (++steps);
-    int tot = (S->cfg.frames - 1) * steps;
+    /* a resumed run has its first chk_num frames behind it (stepper.hh:
+     * 214-216, 358-361) */
+    int tot = (S->cfg.frames - 1 - S->cfg.chk_num) * steps;
+    if (tot < 0) tot = 0;
     S->dt = sdt;
     S->cfg.dt = sdt;
     int e = orc_init_iter(S, init_err);

@@ -63,6 +63,20 @@ class Config(C.Structure):
         ("chk_num", C.c_int),
         ("chk_step", C.c_int),
         ("n_obj", C.c_int),
+        ("suppress", C.c_int),
+        ("display", C.c_int),
+        ("chkpt_freq", C.c_int),
+        ("omp_num_thr", C.c_int),
+        ("debug_obj_id", C.c_int),
+        ("obj_body", C.c_int),
+        ("data_fmt", C.c_int),
+        ("out_flag", C.c_int),
+        ("output_dim", C.c_int),
+        ("output_ind", C.c_int),
+        ("current_time", C.c_double),
+        ("dirname", C.c_char * 256),
+        ("recover_dirname", C.c_char * 256),
+        ("chk_dirname", C.c_char * 256),
     ]
 
 
@@ -110,6 +124,8 @@ def default_config():
     c.sim_type = SIM_FTEST
     c.dump_code, c.ntracers = 3, 1000
     c.n_obj = 0
+    c.chkpt_freq, c.omp_num_thr, c.debug_obj_id, c.obj_body = 1, 1, -1, -1
+    c.out_flag, c.output_dim, c.output_ind = 1 | 2 | 4 | 128, 2, -1
     return c
 
 
@@ -129,6 +145,8 @@ def finalize_config(c):
     for d in range(3):
         if not c.prd[d]:
             c.walls[2 * d] = c.walls[2 * d + 1] = 1
+    if c.output_ind == -1 and 0 <= c.output_dim <= 2:
+        c.output_ind = c.sys_size[c.output_dim] // 2
     dh = min((c.xyz_bounds[2 * d + 1] - c.xyz_bounds[2 * d]) / c.sys_size[d] for d in range(3))
     c.wall_dist = max(dh, c.wall_dist)
     return c

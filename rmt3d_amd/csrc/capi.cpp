@@ -78,6 +78,16 @@ int rmt3d_download_state(rmt3d_ctx *ctx, rmt3d_state *st) {
     GUARD(ctx->sim.download(st));
 }
 int rmt3d_band_count(rmt3d_ctx *ctx) { GUARD(ctx->sim.band_count()); }
+int rmt3d_write_checkpoint(rmt3d_ctx *ctx, int snum, int fnum, const char *chk_dirname) {
+    if (!chk_dirname) return RMT3D_ERR_ARG;
+    GUARD(ctx->sim.write_checkpoint(snum, fnum, chk_dirname));
+}
+int rmt3d_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len) {
+    GUARD(ctx->sim.slice(dim, point, o_type, obj_id, out, len));
+}
+int rmt3d_write_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, int format, const char *filename) {
+    GUARD(ctx->sim.write_slice(dim, point, o_type, obj_id, format, filename));
+}
 int rmt3d_mg_solve(rmt3d_ctx *ctx, int which, const double *rhs, double *x, double tol, int *ncycles, double *resid) {
     if (!rhs || !x || (which != 0 && which != 1)) return RMT3D_ERR_ARG;
     GUARD(ctx->sim.mg_solve(which, rhs, x, tol, ncycles, resid));

@@ -937,7 +937,53 @@ void Sim::compute_grad_basis() {
     }
 }
 
+// fluid_3d::initialize_from_chk_point (fluid_3d_chk_pt.cc:74-163): the eight
+// files are read on the host, scattered into the constructor-state fields on
+// the device, their ghosts filled, and the band rebuilt by a fresh
+// extrapolation.  Returns 0, or 1 / 2 (file missing / empty) with the state
+// untouched, or a library error.
+int Sim::init_from_checkpoint() {
+    printf("fluid_3d:: initializing from files in %s/\n", cfg.chk_dirname);
+    size_t L = (size_t)(g.m + 1) * (g.n + 1), n8 = 8 * (size_t)g.o * L;
+    std::vector<double> img(n8);
+    int e = chk_read_files(cfg.chk_dirname, img.data(), g.o, L, err);
+    if (e) return e;
+    time = cfg.current_time;
+    nt = cfg.chk_step;
+    double *d = nullptr;
+    HCHK(hipMalloc(&d, sizeof(double) * n8));
+    hipError_t he = hipMemcpyAsync(d, img.data(), sizeof(double) * n8, hipMemcpyHostToDevice, stream);
+    if (he == hipSuccess) {
+        launch_zero_i32(stream, d_status, 1);
+        launch_chk_reset(stream, g, U, R);
+        launch_chk_unpack(stream, g, U, R, d);
+        he = hipStreamSynchronize(stream);
+    }
+    hipFree(d);
+    HCHK(he);
+    comm_fluid(ORC_VEL);
+    comm_fluid(ORC_PRES);
+    comm_rm(ORC_C);
+    comm_rm(ORC_X | ORC_XPRED);
+    launch_zero_i32(stream, ext.cnt, g.Ng);
+    launch_zero_i32(stream, ext.n, 1);
+    init_extrapolate();
+    return 0;
+}
+
 int Sim::initialize() {
+    int init_err = 1;
+    if (cfg.chk_num > 0 && cfg.chk_dirname[0]) {
+        init_err = init_from_checkpoint();
+        if (init_err < 0) return init_err;
+        if (init_err == 0) {
+            fill_bc_cc();
+            compute_grad_basis();
+            return check_status("initialize");
+        }
+    }
+    if (cfg.chk_num > 0) printf("# Initializing from scratch, setting chk_num to 0.\n");
+    cfg.chk_num = 0;
     time = 0;
     launch_zero_i32(stream, d_status, 1);
     init_fluid();
@@ -950,7 +996,7 @@ int Sim::initialize() {
     compute_grad_basis();
     int e = check_status("initialize");
     if (e) return e;
-    return 1;
+    return init_err;
 }
 
 // ---------------------------------------------------------------------------
@@ -1129,7 +1175,10 @@ int Sim::prepare_run(int init_err, double *dt_out, int *total) {
     int steps = (int)(df / sdt);
     double rem = df - steps * sdt;
     if (rem > 0) sdt -= (sdt - rem) / (++steps);
-    int tot = (cfg.frames - 1) * steps;
+    // a resumed run has its first chk_num frames behind it (stepper.hh:
+    // 214-216, 358-361); stop_short counts from the resumed step
+    int tot = (cfg.frames - 1 - cfg.chk_num) * steps;
+    if (tot < 0) tot = 0;
     dt = sdt;
     cfg.dt = sdt;
     int e = init_iter(init_err);
@@ -1201,6 +1250,103 @@ int Sim::download(rmt3d_state *st) {
     st->nband = nb;
     HCHK(hipStreamSynchronize(stream));
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint and slice output (fluid_3d_chk_pt.cc:3-72, fluid_3d_io.cc:3-418)
+// ---------------------------------------------------------------------------
+int Sim::write_checkpoint(int snum, int fnum, const char *dir) {
+    size_t L = (size_t)(g.m + 1) * (g.n + 1), n8 = 8 * (size_t)g.o * L;
+    std::vector<double> img(n8);
+    double *d = nullptr;
+    HCHK(hipMalloc(&d, sizeof(double) * n8));
+    launch_chk_pack(stream, g, d_obj, n_obj, U, maps(), d);
+    hipError_t he = hipMemcpyAsync(img.data(), d, sizeof(double) * n8, hipMemcpyDeviceToHost, stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(stream);
+    hipFree(d);
+    HCHK(he);
+    int e = chk_write_files(dir, img.data(), g.o, L, err);
+    if (e) return e;
+    cfg.chk_num = fnum;
+    cfg.chk_step = snum;
+    cfg.current_time = time;
+    e = rmt3d_write_params(&cfg, objs_in.data(), dir);
+    if (e) err = std::string("cannot write ") + dir + "/recover.cfg";
+    return e;
+}
+
+// write_params::corner_field (IO/write_params.hh:66-69)
+static bool corner_field(int t) { return (t >= 3 && t < 6) || t == 7 || t == 18; }
+
+int Sim::slice_dims(int dim, int t, int &d1, int &d2) const {
+    if (dim < 0 || dim > 2 || t < 0 || t > 21) return RMT3D_ERR_ARG;
+    int gd[3] = {g.m, g.n, g.o};
+    if (corner_field(t)) { gd[0] = g.fm; gd[1] = g.fn; gd[2] = g.fo; }
+    gd[dim] = 1;
+    static const int it1[3] = {1, 0, 0}, it2[3] = {2, 2, 1};
+    d1 = gd[it1[dim]];
+    d2 = gd[it2[dim]];
+    return 0;
+}
+
+int Sim::slice(int dim, int point, int t, int obj_id, double *out, int *len) {
+    int d1, d2;
+    if (slice_dims(dim, t, d1, d2)) { err = "slice: bad dimension or output type"; return RMT3D_ERR_ARG; }
+    size_t n = (size_t)d1 * d2;
+    if (!out || !len || (size_t)*len < n) {
+        if (len) *len = (int)n;
+        err = "slice: output array too small";
+        return RMT3D_ERR_ARG;
+    }
+    *len = (int)n;
+    int cells[3] = {g.m, g.n, g.o};
+    if (point < 0 || point >= cells[dim]) {
+        // copy_slice_to_buf refuses the point and the zeroed slice is kept
+        printf("copy_slice_to_buf(): Error! Index out of bound. Nothing has been done.\n");
+        for (size_t a = 0; a < n; a++) out[a] = 0.;
+        return 0;
+    }
+    double *d = nullptr;
+    HCHK(hipMalloc(&d, sizeof(double) * n));
+    const double *r0 = nullptr;
+    if (t == 18) {
+        launch_pp_rhs(stream, g, U, gradphi, wall_faces.data(), (int)wall_faces.size(), d_faces, d_rhs_tmp);
+        r0 = d_rhs_tmp;
+    }
+    launch_slice(stream, g, d_obj, n_obj, U, maps(), r0, t, obj_id, dim, point, d1, d2, d);
+    hipError_t he = hipMemcpyAsync(out, d, sizeof(double) * n, hipMemcpyDeviceToHost, stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(stream);
+    hipFree(d);
+    HCHK(he);
+    return 0;
+}
+
+int Sim::write_slice(int dim, int point, int t, int obj_id, int format, const char *filename) {
+    int d1, d2;
+    if (slice_dims(dim, t, d1, d2) || format < 0 || format > 2 || !filename) {
+        err = "write_slice: bad arguments";
+        return RMT3D_ERR_ARG;
+    }
+    std::vector<double> v((size_t)d1 * d2);
+    int len = (int)v.size();
+    int e = slice(dim, point, t, obj_id, v.data(), &len);
+    if (e) return e;
+    // cell-centre coordinates gx, gy, gz (fluid_3d.cc:132-134), continued
+    // past the last cell for the node counts of the corner fields
+    int cells[3] = {g.m, g.n, g.o};
+    double h[3] = {g.dx, g.dy, g.dz};
+    std::vector<double> crd[3];
+    for (int a = 0; a < 3; a++) {
+        int nn = cells[a] + 1;
+        crd[a].resize(nn);
+        for (int i = 0; i < nn; i++) crd[a][i] = cfg.xyz_bounds[2 * a] + (i + 0.5) * h[a];
+    }
+    const double *a1 = crd[0].data(), *a2 = crd[1].data();
+    if (dim == 0) { a1 = crd[1].data(); a2 = crd[2].data(); }
+    else if (dim == 1) a2 = crd[2].data();
+    e = slice_write_file(filename, format, v.data(), d1, d2, a1, a2, g.m);
+    if (e) err = std::string("cannot write ") + filename;
+    return e;
 }
 
 int Sim::mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res) {

@@ -156,6 +156,10 @@ class Sim {
     int download(rmt3d_state *st);
     int band_count();
     int mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res);
+    int write_checkpoint(int snum, int fnum, const char *dir);
+    int slice_dims(int dim, int t, int &d1, int &d2) const;
+    int slice(int dim, int point, int t, int obj_id, double *out, int *len);
+    int write_slice(int dim, int point, int t, int obj_id, int format, const char *filename);
 
   private:
     void *dalloc(size_t bytes);
@@ -165,6 +169,7 @@ class Sim {
     void comm_fluid(int flags);
     void comm_rm(int flags);
     void init_fluid();
+    int init_from_checkpoint();
     void init_pres();
     int init_refmap();
     void init_extrapolate();

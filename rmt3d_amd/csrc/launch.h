@@ -1,6 +1,7 @@
 // Host-side declarations of the kernel launchers (one translation unit per
 // stage group: k_fluid.hip, k_mg.hip, k_band.hip).
 #pragma once
+#include <string>
 #include "dev.h"
 
 namespace rmt {
@@ -163,5 +164,27 @@ void launch_redraw(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RM
                    int *status);
 void launch_xpred(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U, RMap R);
 void launch_band_x_eq_xpred(hipStream_t s, Band B);
+
+// ---- k_io.hip ----
+// slice of output type t (write_params numbering) normal to dim at point;
+// out holds d1*d2 values; r0 is the pressure-Poisson rhs (type 18 only)
+void launch_slice(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
+                  const double *r0, int t, int obj_id, int dim, int point, int d1, int d2, double *out);
+// the eight checkpoint files as 8 x o x (m+1)(n+1) doubles
+void launch_chk_pack(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
+                     double *out);
+void launch_chk_unpack(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R, const double *in);
+void launch_chk_reset(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R);
+
+// ---- chk_pt.cpp (host) ----
+extern const char *const CHK_NAMES[8];
+// writes the eight files from their images (8 x o x L doubles)
+int chk_write_files(const char *dir, const double *img, int o, size_t L, std::string &err);
+// reads them back as fluid_3d::initialize_from_chk_point does: 0, 1 (a file
+// cannot be opened), 2 (a slice read returned nothing), <0 library error
+int chk_read_files(const char *dir, double *img, int o, size_t L, std::string &err);
+// save_matrix / save_text / save_gnuplot (fluid_3d_io.cc:57-90)
+int slice_write_file(const char *filename, int format, const double *v, int d1, int d2, const double *a1,
+                     const double *a2, int m);
 
 }  // namespace rmt

@@ -25,6 +25,9 @@ extern "C" void rmt3d_default_config(rmt3d_config *c) {
     c->sim_type = RMT3D_SIM_FTEST;
     c->dump_code = 1 | 2; c->ntracers = 1000; c->debug_flag = 0;
     c->n_obj = 0;
+    c->chkpt_freq = 1; c->omp_num_thr = 1; c->debug_obj_id = -1; c->obj_body = -1;
+    c->data_fmt = 0; c->out_flag = 1 | 2 | 4 | 128; c->output_dim = 2; c->output_ind = -1;
+    c->current_time = 0;
 }
 
 namespace {
@@ -48,6 +51,16 @@ extern "C" int rmt3d_parse_cfg(const char *path, rmt3d_config *c, rmt3d_object *
         fprintf(stderr, "sim_params::sim_params(const char*): Filename must end in '.cfg'\n");
         return RMT3D_ERR_CFG;
     }
+    // the .cfg root names the output directory, which is also where
+    // checkpoints are sought unless recover_dirname says otherwise
+    // (sim_params.cc:53-61)
+    if (len - 4 + 4 >= sizeof c->dirname) {
+        fprintf(stderr, "sim_params: configuration path too long\n");
+        return RMT3D_ERR_CFG;
+    }
+    memcpy(c->dirname, path, len - 3);
+    memcpy(c->dirname + len - 3, "out", 4);
+    memcpy(c->recover_dirname, c->dirname, len + 1);
     FILE *f = fopen(path, "r");
     if (!f) return RMT3D_ERR_CFG;
     char buf[1024];
@@ -65,10 +78,15 @@ extern "C" int rmt3d_parse_cfg(const char *path, rmt3d_config *c, rmt3d_object *
         Tok T;
         T.ln = ln;
         std::string k(kw);
-        if (k == "suppress" || k == "display" || k == "omp_num_thr" || k == "debug_obj_id" || k == "out_flag" ||
-            k == "obj_body" || k == "data_fmt" || k == "chkpt_freq") {
-            T.i();
-        } else if (k == "implicit") c->impl = T.i();
+        if (k == "suppress") c->suppress = T.i();
+        else if (k == "display") c->display = T.i();
+        else if (k == "omp_num_thr") c->omp_num_thr = T.i();
+        else if (k == "debug_obj_id") c->debug_obj_id = T.i();
+        else if (k == "out_flag") c->out_flag = T.i();
+        else if (k == "obj_body") c->obj_body = T.i();
+        else if (k == "data_fmt") c->data_fmt = T.i();
+        else if (k == "chkpt_freq") c->chkpt_freq = T.i();
+        else if (k == "implicit") c->impl = T.i();
         else if (k == "godunov") c->godunov = T.i();
         else if (k == "x_prd") c->prd[0] = T.i();
         else if (k == "y_prd") c->prd[1] = T.i();
@@ -77,7 +95,7 @@ extern "C" int rmt3d_parse_cfg(const char *path, rmt3d_config *c, rmt3d_object *
         else if (k == "walls") { for (int i = 0; i < 6; i++) c->walls[i] = T.i(); }
         else if (k == "wall_pos") { for (int i = 0; i < 6; i++) c->wall_pos[i] = T.d(); }
         else if (k == "wall_dist") c->wall_dist = T.d();
-        else if (k == "recover_dirname") T.next();
+        else if (k == "recover_dirname") snprintf(c->recover_dirname, sizeof c->recover_dirname, "%s", T.next());
         else if (k == "chk_num") c->chk_num = T.i();
         else if (k == "chk_step") c->chk_step = T.i();
         else if (k == "sys_size") {
@@ -85,13 +103,21 @@ extern "C" int rmt3d_parse_cfg(const char *path, rmt3d_config *c, rmt3d_object *
         } else if (k == "frames") c->frames = T.i();
         else if (k == "stop_short") c->stop_short = T.i();
         else if (k == "debug_flag") c->debug_flag = T.i();
-        else if (k == "output_dim" || k == "output_ind") T.i();
+        else if (k == "output_dim") {
+            c->output_dim = T.i();
+            if (c->output_dim < 0 || c->output_dim > 2) c->output_dim = -1;
+        } else if (k == "output_ind") {
+            // compared with sys_size as parsed so far (sim_params.cc:145-147)
+            c->output_ind = T.i();
+            if (c->output_ind < 0 || c->output_dim < 0 || c->output_ind >= c->sys_size[c->output_dim])
+                c->output_ind = -1;
+        }
         else if (k == "dump_code") c->dump_code = (int)strtoul(T.next(), nullptr, 10);
         else if (k == "ntracers") c->ntracers = T.i();
         else if (k == "num_iters") c->num_iters = T.i();
         else if (k == "dt") c->dt = T.d();
         else if (k == "T") { double t = T.d(); if (t > 0) c->T = t; }
-        else if (k == "current_time") T.d();
+        else if (k == "current_time") { double t = T.d(); if (t > 0) c->current_time = t; }
         else if (k == "xyz_bounds") { for (int i = 0; i < 6; i++) c->xyz_bounds[i] = T.d(); }
         else if (k == "fmu") c->fmu = T.d();
         else if (k == "fdt_pad") c->fdt_pad = T.d();
@@ -163,6 +189,7 @@ extern "C" int rmt3d_parse_cfg(const char *path, rmt3d_config *c, rmt3d_object *
     fclose(f);
     if (rc) return rc;
     // consolidation (sim_params.cc:326-372)
+    if (c->output_ind == -1 && c->output_dim >= 0) c->output_ind = c->sys_size[c->output_dim] / 2;
     for (int d = 0; d < 3; d++) {
         int s = c->bct[0][2 * d] + c->bct[1][2 * d] + c->bct[2][2 * d];
         if (s >= 0 && s < 3) {
@@ -179,6 +206,23 @@ extern "C" int rmt3d_parse_cfg(const char *path, rmt3d_config *c, rmt3d_object *
     double mdh = dxs[0] < dxs[1] ? dxs[0] : dxs[1];
     mdh = mdh < dxs[2] ? mdh : dxs[2];
     c->wall_dist = mdh > c->wall_dist ? mdh : c->wall_dist;
+    // a checkpoint is used only when its recover.cfg can be opened
+    // (sim_params.cc:374-398)
+    c->chk_dirname[0] = 0;
+    if (c->chk_num > 0) {
+        char probe[600];
+        snprintf(c->chk_dirname, sizeof c->chk_dirname, "%s/chk.%05d", c->recover_dirname, c->chk_num);
+        snprintf(probe, sizeof probe, "%s/recover.cfg", c->chk_dirname);
+        FILE *fh = fopen(probe, "rb");
+        if (!fh) {
+            printf("sim_params::Cannot find checkpt directory %s. Reset to none.\n", c->chk_dirname);
+            c->chk_num = 0;
+            c->chk_step = 0;
+            c->chk_dirname[0] = 0;
+        } else {
+            fclose(fh);
+        }
+    }
     if (sim_type == "ftest") { c->sim_type = RMT3D_SIM_FTEST; n_obj = 0; }
     else if (sim_type == "stest") { c->sim_type = RMT3D_SIM_STEST; n_obj = 0; }
     else if (sim_type == "objects") c->sim_type = RMT3D_SIM_OBJECTS;

@@ -23,7 +23,8 @@ EXPORTS = [
     "rmt3d_stream", "rmt3d_profile", "rmt3d_profile_read", "rmt3d_device_synchronize",
     "rmt3d_estimate_bytes", "rmt3d_server_create", "rmt3d_server_submit", "rmt3d_server_wait",
     "rmt3d_server_info", "rmt3d_server_error", "rmt3d_server_result", "rmt3d_server_release",
-    "rmt3d_server_stats", "rmt3d_server_destroy",
+    "rmt3d_server_stats", "rmt3d_server_destroy", "rmt3d_write_checkpoint", "rmt3d_write_params",
+    "rmt3d_slice", "rmt3d_write_slice",
 ]
 
 
@@ -81,6 +82,10 @@ def lib():
         L.rmt3d_server_stats.argtypes = [P, dp, ip, dp]
         L.rmt3d_server_destroy.argtypes = [P]
         L.rmt3d_server_destroy.restype = None
+        L.rmt3d_write_checkpoint.argtypes = [P, C.c_int, C.c_int, C.c_char_p]
+        L.rmt3d_write_params.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_char_p]
+        L.rmt3d_slice.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, ip]
+        L.rmt3d_write_slice.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p]
         _lib = L
     return _lib
 
@@ -92,6 +97,45 @@ def parse_cfg(path, max_obj=256):
     if rc:
         raise Rmt3dError("rmt3d_parse_cfg(%s) failed with %d" % (path, rc))
     return cfg, [objs[i] for i in range(cfg.n_obj)]
+
+
+def write_params(cfg, objs, dirname):
+    """sim_params::write_params: <dirname>/recover.cfg."""
+    n = max(1, len(objs))
+    arr = (_abi.Object * n)(*objs) if objs else (_abi.Object * 1)()
+    rc = lib().rmt3d_write_params(C.byref(cfg), arr, os.fsencode(dirname))
+    if rc:
+        raise Rmt3dError("rmt3d_write_params(%s) failed with %d" % (dirname, rc))
+
+
+CHK_FILES = ("vx", "vy", "vz", "p", "X", "Y", "Z", "OID")
+
+
+def compress_checkpoint(dirname):
+    """Replace the eight checkpoint files by .xz files, as the reference's
+    `xz` step after write_chk_pt does (fluid_3d_chk_pt.cc:54-60)."""
+    import lzma
+    for name in CHK_FILES:
+        path = os.path.join(dirname, name)
+        with open(path, "rb") as f:
+            data = f.read()
+        with lzma.open(path + ".xz", "wb", format=lzma.FORMAT_XZ, check=lzma.CHECK_CRC64) as f:
+            f.write(data)
+        os.remove(path)
+
+
+def decompress_checkpoint(dirname):
+    """The reference's `xz -d` before reading (fluid_3d_chk_pt.cc:109-114):
+    every <name>.xz present is expanded to <name> and removed."""
+    import lzma
+    for name in CHK_FILES:
+        path = os.path.join(dirname, name)
+        if os.path.exists(path + ".xz"):
+            with lzma.open(path + ".xz", "rb") as f:
+                data = f.read()
+            with open(path, "wb") as f:
+                f.write(data)
+            os.remove(path + ".xz")
 
 
 def _dp(a):
@@ -208,6 +252,28 @@ class Simulation:
                 setattr(st, k, _dp(a))
             keep.append(a)
         self._check(lib().rmt3d_upload_state(self.h, C.byref(st)), "upload")
+
+    # ---- checkpoint and slice formats ----
+    def write_checkpoint(self, snum, fnum, dirname):
+        """fluid_3d::write_chk_pt into the existing directory dirname."""
+        self._check(lib().rmt3d_write_checkpoint(self.h, snum, fnum, os.fsencode(dirname)), "write_checkpoint")
+
+    def slice(self, dim, point, o_type, obj_id=-1):
+        """fluid_3d::slice: the global slice as a (d2, d1) array."""
+        m, n, o = self.cfg.sys_size
+        cap = (max(m, n, o) + 1) ** 2
+        out = np.zeros(cap)
+        ln = C.c_int(cap)
+        self._check(lib().rmt3d_slice(self.h, dim, point, o_type, obj_id, _dp(out), C.byref(ln)), "slice")
+        corner = 3 <= o_type < 6 or o_type in (7, 18)
+        g = [self.cfg.sys_size[d] + (1 if corner and not self.cfg.prd[d] else 0) for d in range(3)]
+        rest = [d for d in range(3) if d != dim]
+        return out[:ln.value].reshape(g[rest[1]], g[rest[0]])
+
+    def write_slice(self, dim, point, o_type, filename, obj_id=-1, fmt=0):
+        """fluid_3d::write_slice (fmt 0 matrix, 1 text, 2 gnuplot binary)."""
+        self._check(lib().rmt3d_write_slice(self.h, dim, point, o_type, obj_id, fmt, os.fsencode(filename)),
+                    "write_slice")
 
     def mg_dims(self, which):
         d = (C.c_int * 3)()

@@ -12,9 +12,14 @@ CFG_DIR = os.path.join(os.path.dirname(HERE), "rmt3d_amd", "sim_configs")
 REF_CFG = "/root/reference/sim_configs"
 
 
+PATHS = ("dirname", "recover_dirname", "chk_dirname")
+
+
 def _fields(s):
     out = {}
     for name, _ in s._fields_:
+        if name in PATHS:
+            continue
         v = getattr(s, name)
         if hasattr(v, "__len__"):
             v = [list(x) if hasattr(x, "__len__") else x for x in v]
@@ -28,9 +33,12 @@ def _fields(s):
     ("stretched_sphere_128", lambda: configs.stretched_sphere(128)),
 ])
 def test_workload_cfg_matches_builder(name, builder):
-    cfg, objs = lib.parse_cfg(os.path.join(CFG_DIR, name + ".cfg"))
+    path = os.path.join(CFG_DIR, name + ".cfg")
+    cfg, objs = lib.parse_cfg(path)
     cb, ob = builder()
     assert _fields(cfg) == _fields(cb)
+    # the output directory is the .cfg root (sim_params.cc:53-61)
+    assert cfg.dirname == path[:-4].encode() + b".out" == cfg.recover_dirname
     assert len(objs) == len(ob)
     for a, b in zip(objs, ob):
         assert _fields(a) == _fields(b)
