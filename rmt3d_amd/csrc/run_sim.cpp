@@ -1,17 +1,48 @@
 // Command-line driver over the C ABI, following the reference's run_sim
-// sequence (Execs/run_sim.cc:39-165): read a .cfg, create the simulation,
-// initialize, set up the stepper (dt clamp, step count, init_iter warm-up),
-// then step frame by frame, printing per-frame diagnostics and the total
-// wall time.  Usage:
+// sequence (Execs/run_sim.cc:39-165) and the stepper's frame loop
+// (Sim/stepper.hh:211-376, 580-610): read a .cfg, create the simulation,
+// initialize (from a checkpoint when chk_num names one), set up the stepper
+// (dt clamp, frame step count, init_iter warm-up), then step frame by frame.
+// After each frame (and once before the first) it writes what dump_code
+// asks for: slices (bit 4, one per out_flag bit, write_params naming
+// "<dirname>/<field>.<frame>") and checkpoints (bit 8, every chkpt_freq
+// frames, "<dirname>/chk.<frame>").  Usage:
 //   run_sim <file.cfg> [--mixed] [--steps N] [--quiet]
-// --steps N overrides the step count of the stepper (stop_short / T / dt).
+// --steps N stops after N steps (as stop_short does).
+#include <sys/stat.h>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "rmt3d.h"
+
+// write_params::default_names (IO/write_params.cc:3)
+static const char *const FIELD_NAMES[22] = {"vx", "vy", "vz", "wx",  "wy",     "wz",   "w",    "p",
+                                            "phi", "X", "Y",  "Z",   "OID",    "LID",  "pr_phi", "pr_X",
+                                            "pr_Y", "pr_Z", "div_u", "div_eu", "rho", "J"};
+
+// stepper::write_files without tracers and contours (stepper.hh:580-610)
+static int write_files(rmt3d_ctx *ctx, const rmt3d_config &cfg, int snum, int fnum) {
+    char buf[1024];
+    if (cfg.dump_code & 4) {
+        for (int t = 0; t < 22; t++) {
+            if (!((1 << t) & cfg.out_flag)) continue;
+            snprintf(buf, sizeof buf, "%s/%s.%05d", cfg.dirname, FIELD_NAMES[t], fnum);
+            int rc = rmt3d_write_slice(ctx, cfg.output_dim, cfg.output_ind, t, cfg.obj_body, cfg.data_fmt, buf);
+            if (rc) return rc;
+        }
+    }
+    if ((cfg.dump_code & 8) && cfg.chkpt_freq > 0 && fnum % cfg.chkpt_freq == 0) {
+        snprintf(buf, sizeof buf, "%s/chk.%05d", cfg.dirname, fnum);
+        mkdir(buf, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+        int rc = rmt3d_write_checkpoint(ctx, snum, fnum, buf);
+        if (rc) return rc;
+    }
+    return 0;
+}
 
 int main(int argc, char **argv) {
     if (argc < 2) {
@@ -56,34 +87,57 @@ int main(int argc, char **argv) {
         rmt3d_destroy(ctx);
         return 1;
     }
-    if (steps_override >= 0) total = steps_override;
-    int frames = cfg.frames > 0 ? cfg.frames : 1;
-    int per_frame = total / frames > 0 ? total / frames : 1;
+    // the stepper's frame bookkeeping (stepper.hh:211-235): steps per frame,
+    // the first frame and step of a resumed run, and the stop_short bound
+    // counted from that step
+    int init_frame = init_err > 0 ? 0 : cfg.chk_num, init_step = cfg.chk_step;
+    double df = cfg.T / (cfg.frames - 1);
+    int steps = (int)llround(df / dt);
+    if (steps < 1) steps = 1;
+    long long stop = (cfg.stop_short <= 0) ? (long long)(cfg.frames - 1) * steps + 1 : cfg.stop_short + init_step;
+    if (steps_override >= 0) stop = (long long)init_step + steps_override;
     long long cells = (long long)cfg.sys_size[0] * cfg.sys_size[1] * cfg.sys_size[2];
-    printf("# %s: %dx%dx%d, %d object(s), %s, dt %.6g, %d steps\n", argv[1], cfg.sys_size[0], cfg.sys_size[1],
-           cfg.sys_size[2], cfg.n_obj, precision == RMT3D_PREC_MIXED ? "mixed" : "fp64", dt, total);
+    printf("# %s: %dx%dx%d, %d object(s), %s, dt %.6g, %d steps per frame, %d step(s) to run\n", argv[1],
+           cfg.sys_size[0], cfg.sys_size[1], cfg.sys_size[2], cfg.n_obj,
+           precision == RMT3D_PREC_MIXED ? "mixed" : "fp64", dt, steps, total);
+    if (cfg.dump_code & 12) mkdir(cfg.dirname, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    if ((rc = write_files(ctx, cfg, init_step, init_frame))) {
+        fprintf(stderr, "run_sim: output failed: %s\n", rmt3d_last_error(ctx));
+        rmt3d_destroy(ctx);
+        return 1;
+    }
     rmt3d_device_synchronize(ctx);
     auto t1 = std::chrono::steady_clock::now();
     int status = 0, done = 0;
-    for (int s = 0; s < total; s++) {
-        status = rmt3d_step(ctx, 0);
+    long long tot_step = init_step;
+    for (int frame = init_frame + 1; frame <= cfg.frames - 1 && tot_step < stop && status == 0; frame++) {
+        for (int s = 0; s < steps && tot_step < stop; s++) {
+            tot_step++;
+            status = rmt3d_step(ctx, 0);
+            if (status) break;
+            done++;
+        }
         if (status) break;
-        done++;
-        if (!quiet && (done % per_frame == 0 || done == total)) {
+        if (!quiet) {
             double divu = 0, nmac = 0, nfem = 0;
             rmt3d_diag(ctx, 0, &divu);
             rmt3d_diag(ctx, 1, &nmac);
             rmt3d_diag(ctx, 2, &nfem);
-            printf("step %6d  time %.6f  div_u %.3e  V-cycles MAC %2.0f FEM %2.0f\n", done, rmt3d_get_time(ctx), divu,
-                   nmac, nfem);
+            printf("frame %5d step %6lld  time %.6f  div_u %.3e  V-cycles MAC %2.0f FEM %2.0f\n", frame, tot_step,
+                   rmt3d_get_time(ctx), divu, nmac, nfem);
             fflush(stdout);
+        }
+        if ((rc = write_files(ctx, cfg, (int)tot_step, frame))) {
+            fprintf(stderr, "run_sim: output failed: %s\n", rmt3d_last_error(ctx));
+            status = rc;
+            break;
         }
     }
     rmt3d_device_synchronize(ctx);
     auto t2 = std::chrono::steady_clock::now();
     double init_s = std::chrono::duration<double>(t1 - t0).count();
     double run_s = std::chrono::duration<double>(t2 - t1).count();
-    if (status) printf("# stopped at step %d with code %d (%s)\n", done + 1, status, rmt3d_last_error(ctx));
+    if (status) printf("# stopped at step %lld with code %d (%s)\n", tot_step, status, rmt3d_last_error(ctx));
     printf("# initialization %.3f s, %d steps %.3f s, %.3f ms/step, %.4g cell-steps/s\n", init_s, done, run_s,
            done ? 1e3 * run_s / done : 0.0, done ? cells * done / run_s : 0.0);
     rmt3d_destroy(ctx);
