@@ -64,6 +64,10 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
+    ap.add_argument("--io", action="store_true",
+                    help="instead of the step: time the checkpoint write (device gather + files) and the "
+                         "resume (files + device scatter) of the FP64 state after --warmup steps")
+    ap.add_argument("--io-reps", type=int, default=5)
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=16)
     return ap.parse_args()
@@ -204,6 +208,114 @@ def run_cpu(args):
                       % (args.cpu_steps, args.n, threads, multiprocessing.cpu_count())}
 
 
+def run_io(args):
+    """Checkpoint files of the --n^3 stretched-sphere state (fluid_3d::
+    write_chk_pt / initialize_from_chk_point).  One JSON line: value is the
+    checkpoint bytes written per second of rmt3d_write_checkpoint (device
+    gather, copy to the host, eight files into a scratch directory);
+    roofline is the gather kernel k_chk_pack against HBM; cpu_baseline is the
+    oracle's gather of the same state (the reference's slice loop restated)."""
+    import shutil
+    import tempfile
+    import numpy as np
+    from oracle.oracle import OracleSim
+    from rmt3d_amd import _abi, configs
+    from rmt3d_amd import lib as rlib
+    from rmt3d_amd.lib import Simulation
+    cfg, objs = configs.stretched_sphere(args.n, steps=args.warmup + 10)
+    work = tempfile.mkdtemp(prefix="rmt3d_io_")
+    try:
+        out = os.path.join(work, "run.out")
+        chk = os.path.join(out, "chk.00001")
+        os.makedirs(chk)
+        cfg.dirname = out.encode()
+        sim = Simulation(cfg, objs, _abi.PREC_FP64)
+        sim.prepare_run(sim.initialize())
+        for _ in range(args.warmup):
+            if sim.step():
+                raise RuntimeError("step failed")
+        sim.synchronize()
+        sim.write_checkpoint(args.warmup, 1, chk)       # first launch loads the kernels
+        sim.profile(1)
+        t0 = time.perf_counter()
+        for _ in range(args.io_reps):
+            sim.write_checkpoint(args.warmup, 1, chk)
+        t1 = time.perf_counter()
+        pack_ms, pack_n = sim.profile_read(6)
+        st = sim.download()
+        sim.profile(0)
+        m, n, o = cfg.sys_size
+        L = (m + 1) * (n + 1)
+        fm, fn = m + 1, n + 1
+        N = m * n * o
+        nband = len(st["band_cell"])
+        n_solid = int(np.count_nonzero(st["rm_c"]))
+        file_bytes = 8 * 8 * o * L
+        # gather: 8 words written per slot; per cell vel 3 words, c and cnt
+        # 4 B each; per node p 1 word; x 3 words of each primary solid cell;
+        # start 4 B, c 4 B and x 3 words of each band entry
+        pack_bytes = file_bytes + N * (24 + 4 + 4) + o * fm * fn * 8 + n_solid * 24 + nband * 36
+        pack_s = pack_ms / max(pack_n, 1) * 1e-3
+        achieved = pack_bytes / pack_s / 1e9
+        # resume from those files
+        c2, o2 = rlib.parse_cfg(os.path.join(chk, "recover.cfg"))
+        h = Simulation(c2, o2, _abi.PREC_FP64)
+        h.profile(1)
+        t2 = time.perf_counter()
+        e = h.initialize()
+        h.synchronize()
+        t3 = time.perf_counter()
+        unpack_ms, unpack_n = h.profile_read(7)
+        h.close()
+        if e != 0:
+            raise RuntimeError("resume returned %d" % e)
+        # scatter: per cell 7 words read, vel 3, x 3, xpred 3 words and c
+        # written; per node 1 word read and written
+        unpack_bytes = N * (56 + 72 + 4) + o * fm * fn * 16
+        # the oracle gathers the same state: files must match bit for bit
+        orc = OracleSim(cfg, objs)
+        orc.upload(**{k: st[k] for k in ("u", "v", "w", "p", "rm_c", "rm_x", "rm_xpred")})
+        orc.set_band(st["band_cell"], st["band_c"], st["band_x"].reshape(3, -1), st["band_xpred"].reshape(3, -1))
+        t4 = time.perf_counter()
+        img = orc.chk_images()
+        t5 = time.perf_counter()
+        orc.close()
+        names = ("vx", "vy", "vz", "p", "X", "Y", "Z", "OID")
+        match = all(np.array_equal(np.fromfile(os.path.join(chk, nm)), img[f].ravel()) for f, nm in enumerate(names))
+        sim.close()
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    wall = (t1 - t0) / args.io_reps
+    line = {
+        "metric": "checkpoint write bytes/s at %d^3 (write_chk_pt files)" % args.n,
+        "value": round(file_bytes / wall, 1),
+        "unit": "B/s",
+        "n_gpus": 1,
+        "steps": args.io_reps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "stretched_sphere_%d checkpoint" % args.n, "file_bytes": file_bytes,
+                   "band_entries": nband, "solid_cells": n_solid, "files_match_oracle": bool(match),
+                   "resume_ms": round(1e3 * (t3 - t2), 3),
+                   "scatter": {"kernel": "k_chk_unpack", "avg_launch_us": round(unpack_ms / max(unpack_n, 1) * 1e3, 3),
+                               "bytes_per_launch": unpack_bytes,
+                               "achieved_gbs": round(unpack_bytes / (unpack_ms / max(unpack_n, 1) * 1e-3) / 1e9, 1)
+                               if unpack_ms > 0 else 0.0}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_chk_pack",
+                     "bytes_per_launch": pack_bytes, "avg_launch_us": round(pack_s * 1e6, 3), "launches": pack_n},
+        "cpu_baseline": {"value": round(file_bytes / (t5 - t4), 1), "unit": "B/s", "cores": 1, "kind": "port",
+                         "sample": "one gather of the same %d^3 state by the oracle's restatement of "
+                                   "write_chk_pt's slice loop (files not written)" % args.n},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def _rel(a, b):
     import numpy as np
     d = np.linalg.norm(a - b)
@@ -229,6 +341,9 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.io:
+        run_io(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -243,7 +243,9 @@ void *rmt3d_stream(rmt3d_ctx *ctx);
  * which: 0 finest-level FEM smoother colour pass,
  * 1 finest-level MAC smoother colour pass, 2 compute_stress, 3 advection,
  * 4 one FEM V-cycle, 5 one MAC V-cycle (with the mixed path, one FP64
- * defect-correction step around an FP32 V-cycle). */
+ * defect-correction step around an FP32 V-cycle), 6 the checkpoint gather
+ * of rmt3d_write_checkpoint, 7 the checkpoint scatter of a resuming
+ * rmt3d_initialize (6 and 7 are timed from level 1). */
 int rmt3d_profile(rmt3d_ctx *ctx, int enable);
 int rmt3d_profile_read(rmt3d_ctx *ctx, int which, double *ms, int *launches);
 /* hipDeviceSynchronize on the context's device. */

@@ -956,7 +956,9 @@ int Sim::init_from_checkpoint() {
     if (he == hipSuccess) {
         launch_zero_i32(stream, d_status, 1);
         launch_chk_reset(stream, g, U, R);
-        launch_chk_unpack(stream, g, U, R, d);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        kprof[7].pair(e0, e1);
+        launch_chk_unpack(stream, g, U, R, d, e0, e1);
         he = hipStreamSynchronize(stream);
     }
     hipFree(d);
@@ -1260,7 +1262,9 @@ int Sim::write_checkpoint(int snum, int fnum, const char *dir) {
     std::vector<double> img(n8);
     double *d = nullptr;
     HCHK(hipMalloc(&d, sizeof(double) * n8));
-    launch_chk_pack(stream, g, d_obj, n_obj, U, maps(), d);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    kprof[6].pair(e0, e1);
+    launch_chk_pack(stream, g, d_obj, n_obj, U, maps(), d, e0, e1);
     hipError_t he = hipMemcpyAsync(img.data(), d, sizeof(double) * n8, hipMemcpyDeviceToHost, stream);
     if (he == hipSuccess) he = hipStreamSynchronize(stream);
     hipFree(d);

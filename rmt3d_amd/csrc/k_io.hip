@@ -5,6 +5,7 @@
 // 303-352) scattered into the padded fields.  One thread per output point;
 // every access is a plain coalesced load or store (the work is HBM-bound
 // byte movement).
+#include <hip/hip_ext.h>
 #include "launch.h"
 
 namespace rmt {
@@ -54,8 +55,8 @@ __device__ __forceinline__ int pick_map(const Maps &M, int c, int obj_id) {
 // Value of output type t at local cell/node (i,j,k) (padded index c), as
 // copy_slice_to_buf computes it.  r0 is the pressure-Poisson right-hand
 // side (type 18 only).
-__device__ double slice_value(const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M, const double *r0,
-                              int t, int obj_id, int i, int j, int k) {
+__device__ __forceinline__ double slice_value(const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
+                                              const double *r0, int t, int obj_id, int i, int j, int k) {
     int c = IXp(g, i + 2, j + 2, k + 2);
     int sm4 = g.m4, lo = c - g.mn4;
     if (t < 3) return U.vel[t][c];
@@ -139,16 +140,27 @@ __global__ void k_slice(Geo g, const ObjC *O, int nobj, Fluid U, Maps M, const d
 // fill e < m*n (row stride m), the pressure fills e < fm*fn (row stride fm).
 // So a cell field written before the pressure carries zeros past m*n, and
 // one written after it carries the pressure's last slice (k = o-1) there.
-__global__ void k_chk_pack(Geo g, const ObjC *O, int nobj, Fluid U, Maps M, size_t L, double *out) {
+__global__ void __launch_bounds__(256) k_chk_pack(Geo g, const ObjC *O, int nobj, Fluid U, Maps M, size_t L,
+                                                   double *out) {
     int e = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (e >= (int)L) return;
     size_t fs = (size_t)g.o * L, base = (size_t)k * L + e;
     int mn = g.m * g.n, fmn = g.fm * g.fn;
     double v[8];
     if (e < mn) {
-        int i = e % g.m, j = e / g.m;
-        for (int t = 0; t < 3; t++) v[t] = slice_value(g, O, nobj, U, M, nullptr, t, -1, i, j, k);
-        for (int t = 9; t < 13; t++) v[t - 5] = slice_value(g, O, nobj, U, M, nullptr, t, -1, i, j, k);
+        int c = IXp(g, e % g.m + 2, e / g.m + 2, k + 2);
+        for (int a = 0; a < 3; a++) v[a] = U.vel[a][c];
+        // X, Y, Z and OID of the object with the least phi (min_phi_id, then
+        // the solid branch of copy_slice_to_buf), selected once per cell
+        int id = min_phi_id(O, nobj, M, c);
+        int ref = id < 0 ? -1 : pick_map(M, c, id);
+        if (ref < 0) {
+            v[4] = v[5] = v[6] = 100.;
+            v[7] = 0.;
+        } else {
+            for (int a = 0; a < 3; a++) v[4 + a] = M.x(ref, a);
+            v[7] = ref < M.Ng ? (double)(M.r.c[c] & OMASK) : 0.;
+        }
     } else {
         v[0] = v[1] = v[2] = 0.;
         double tail = 0.;
@@ -210,17 +222,18 @@ void launch_slice(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fl
 }
 
 void launch_chk_pack(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
-                     double *out) {
+                     double *out, hipEvent_t start, hipEvent_t stop) {
     size_t L = (size_t)(g.m + 1) * (g.n + 1);
     dim3 grid((unsigned)((L + 255) / 256), g.o);
-    hipLaunchKernelGGL(k_chk_pack, grid, dim3(256), 0, s, g, O, nobj, U, M, L, out);
+    hipExtLaunchKernelGGL(k_chk_pack, grid, dim3(256), 0, s, start, stop, 0, g, O, nobj, U, M, L, out);
 }
 
-void launch_chk_unpack(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R, const double *in) {
+void launch_chk_unpack(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R, const double *in,
+                       hipEvent_t start, hipEvent_t stop) {
     size_t L = (size_t)(g.m + 1) * (g.n + 1);
     int fmn = g.fm * g.fn;
     dim3 grid((fmn + 255) / 256, g.o);
-    hipLaunchKernelGGL(k_chk_unpack, grid, dim3(256), 0, s, g, U, R, L, in);
+    hipExtLaunchKernelGGL(k_chk_unpack, grid, dim3(256), 0, s, start, stop, 0, g, U, R, L, in);
 }
 
 void launch_chk_reset(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R) {

@@ -170,10 +170,12 @@ void launch_band_x_eq_xpred(hipStream_t s, Band B);
 // out holds d1*d2 values; r0 is the pressure-Poisson rhs (type 18 only)
 void launch_slice(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
                   const double *r0, int t, int obj_id, int dim, int point, int d1, int d2, double *out);
-// the eight checkpoint files as 8 x o x (m+1)(n+1) doubles
+// the eight checkpoint files as 8 x o x (m+1)(n+1) doubles; start/stop
+// (may be null) are recorded at the kernel's own start and end
 void launch_chk_pack(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
-                     double *out);
-void launch_chk_unpack(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R, const double *in);
+                     double *out, hipEvent_t start, hipEvent_t stop);
+void launch_chk_unpack(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R, const double *in,
+                       hipEvent_t start, hipEvent_t stop);
 void launch_chk_reset(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R);
 
 // ---- chk_pt.cpp (host) ----
