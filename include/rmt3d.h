@@ -226,6 +226,27 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * 16/17 -> colour passes per smoothing launch on the finest level. */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
+/* The macro.dat quantities of stepper::collect_macro_data
+ * (Sim/stepper.hh:378-440), reduced on the device.  Sums run in a fixed
+ * blocked order (256 blocks of consecutive cells, a tree within each block,
+ * the block partials added in order), not the reference's serial order, so
+ * they agree with it to rounding. */
+typedef struct rmt3d_macro {
+    double divu_max, divu_min;   /* MAC divergence extremes (fluid_3d_utils.cc:572-594)  */
+    double divu_l2;              /* the FEM right-hand-side norm (:507-560)              */
+    double momentum[3];          /* total_momentum (:783-800)                            */
+    double energy, kinetic, elastic, potential, power; /* total_energy (:806-870)        */
+    double max_dev_detF, dev_detF_l2;                   /* total_dev_in_detF (:1032-1085) */
+} rmt3d_macro;
+int rmt3d_macro_data(rmt3d_ctx *ctx, rmt3d_macro *out);
+/* fluid_3d::compute_v_centroid (fluid_3d_utils.cc:1290-1346) followed by
+ * sim_manager::compute_avgs (sim_manager.hh:223-246): for each object,
+ * out[6i..6i+5] = centroid x, y, z and average velocity x, y, z, weighted by
+ * the Heaviside solid fraction of the primary cells.  As in the reference,
+ * the averages are kept: the wall repulsion of later steps reads the
+ * average velocity (sim_manager.cc:212-216).  out may be NULL. */
+int rmt3d_centroids(rmt3d_ctx *ctx, double *out);
+
 /* Per-stage device time of the steps run while rmt3d_profile is enabled
  * (ms, from event pairs on the context stream), in the order of the
  * reference's timer buckets "Extrapolation, Pressure Poisson, Communication,

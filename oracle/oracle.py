@@ -13,7 +13,7 @@ from rmt3d_amd import _abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_io.c", "orc_api.c"]
+SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_io.c", "orc_diag.c", "orc_api.c"]
 
 _lib = None
 
@@ -127,6 +127,10 @@ def lib():
         L.oracle_heaviside.argtypes = [P, D]
         L.oracle_heaviside.restype = D
         L.oracle_lsq.argtypes = [C.c_int, dp, dp, dp, D, dp, dp, dp, dp]
+        L.oracle_det_log.argtypes = [D]
+        L.oracle_det_log.restype = D
+        L.oracle_macro.argtypes = [P, dp]
+        L.oracle_centroids.argtypes = [P, dp]
         L.oracle_set_band.argtypes = [P, C.c_int, ip, C.POINTER(C.c_uint32), dp, dp]
         L.oracle_chk_images.argtypes = [P, dp]
         L.oracle_chk_restore.argtypes = [P, dp]
@@ -320,6 +324,19 @@ class OracleSim:
         g = np.zeros(d1 * d2)
         lib().oracle_slice(self.h, dim, point, o_type, obj_id, _dp(g))
         return g.reshape(d2, d1)
+
+    def macro_data(self):
+        """The macro.dat quantities (rmt3d_macro field order) as a dict."""
+        out = np.zeros(13)
+        lib().oracle_macro(self.h, _dp(out))
+        return dict(zip(_abi.MACRO_FIELDS, out.tolist()))
+
+    def centroids(self):
+        """compute_v_centroid: (n_obj, 6) centroid x y z, velocity x y z."""
+        k = max(1, self.cfg.n_obj)
+        out = np.zeros(6 * k)
+        lib().oracle_centroids(self.h, _dp(out))
+        return out.reshape(k, 6)[:self.cfg.n_obj]
 
     @property
     def nt(self):

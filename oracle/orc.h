@@ -191,6 +191,10 @@ void orc_chk_restore(orc_sim *S, const double *img);
 int orc_chk_read(const char *dir, int o, size_t L, double *img);
 int orc_write_chk_pt(orc_sim *S, const char *dir);
 
+/* orc_diag.c: macro.dat / centroid.dat reductions */
+void orc_macro(orc_sim *S, double *out);
+void orc_centroids(orc_sim *S, double *out);
+
 /* orc_extrap.c */
 void orc_band_init(orc_band *B, int Ng);
 void orc_band_free(orc_band *B);

@@ -210,6 +210,9 @@ int oracle_mg_direct(orc_sim *S, int which) { return (which ? S->fem.pinv : S->m
 /* the deterministic sine/cosine shared with the HIP path */
 double oracle_det_sin(double x) { return det_sin(x); }
 double oracle_det_cos(double x) { return det_cos(x); }
+double oracle_det_log(double x) { return det_log(x); }
+void oracle_macro(orc_sim *S, double *out) { orc_macro(S, out); }
+void oracle_centroids(orc_sim *S, double *out) { orc_centroids(S, out); }
 
 /* threads the oracle's parallel loops use (OpenMP; 1 without it) */
 int oracle_threads(void)

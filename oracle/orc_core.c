@@ -1161,7 +1161,7 @@ static void solid_stress(orc_sim *S, int eid, int F, double ss[3], double *sfrac
         double sg[6], cg[6];
         orc_mat_aat(Fd, sg);
         orc_mat_ata(Fd, cg);
-        rf->energy[F] = 0.5 * G * ((cg[0] + cg[3] + cg[5]) - 3 - 2 * log(rf->detF[F]));
+        rf->energy[F] = 0.5 * G * ((cg[0] + cg[3] + cg[5]) - 3 - 2 * det_log(rf->detF[F]));
         double savg = (sg[0] + sg[3] + sg[5]) / 3.0;
         for (int a = 0; a < 3; a++) sg[symind(a, a)] -= savg;
         double exmu = M->ex_mu * tf * (1 + M->ev_trans_mult * tderiv_func_in(S, phiv));

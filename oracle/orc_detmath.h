@@ -32,4 +32,29 @@ DET_FN double det_sincos(double x, int cosine)
 }
 DET_FN double det_sin(double x) { return det_sincos(x, 0); }
 DET_FN double det_cos(double x) { return det_sincos(x, 1); }
+
+// Deterministic natural logarithm, same operations on host and device:
+// x = 2^e f with f in [sqrt(1/2), sqrt(2)), log f = 2 atanh(s) with
+// s = (f-1)/(f+1), |s| < 0.172, from the Taylor series of atanh through
+// s^23 (the next term is below 1e-20), plus e ln2 with ln2 split so that
+// e * DET_LN2_HI is exact.  Error against libm below 2 ulp.
+#define DET_LN2_HI 0.6931471803691238
+#define DET_LN2_LO 1.9082149292705877e-10
+DET_FN double det_log(double x)
+{
+    if (!(x > 0)) return x == 0 ? -INFINITY : NAN;
+    if (isinf(x)) return x;
+    int e;
+    double f = frexp(x, &e);
+    if (f < 0.7071067811865476) {
+        f *= 2;
+        e -= 1;
+    }
+    double s = (f - 1) / (f + 1);
+    double s2 = s * s;
+    double q = 1.0 / 3 + s2 * (1.0 / 5 + s2 * (1.0 / 7 + s2 * (1.0 / 9 + s2 * (1.0 / 11 + s2 * (1.0 / 13 +
+               s2 * (1.0 / 15 + s2 * (1.0 / 17 + s2 * (1.0 / 19 + s2 * (1.0 / 21 + s2 * (1.0 / 23))))))))));
+    double lf = 2 * s + 2 * s * (s2 * q);
+    return e * DET_LN2_HI + (e * DET_LN2_LO + lf);
+}
 #endif

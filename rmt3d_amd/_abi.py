@@ -80,6 +80,15 @@ class Config(C.Structure):
     ]
 
 
+MACRO_FIELDS = ("divu_max", "divu_min", "divu_l2", "momentum_x", "momentum_y", "momentum_z", "energy", "kinetic",
+                "elastic", "potential", "power", "max_dev_detF", "dev_detF_l2")
+
+
+class Macro(C.Structure):
+    """rmt3d_macro (include/rmt3d.h): the macro.dat quantities."""
+    _fields_ = [(k, C.c_double) for k in MACRO_FIELDS]
+
+
 class State(C.Structure):
     _fields_ = [
         ("u", C.POINTER(C.c_double)),

@@ -82,6 +82,11 @@ int rmt3d_write_checkpoint(rmt3d_ctx *ctx, int snum, int fnum, const char *chk_d
     if (!chk_dirname) return RMT3D_ERR_ARG;
     GUARD(ctx->sim.write_checkpoint(snum, fnum, chk_dirname));
 }
+int rmt3d_macro_data(rmt3d_ctx *ctx, rmt3d_macro *out) {
+    if (!out) return RMT3D_ERR_ARG;
+    GUARD(ctx->sim.macro_data(out));
+}
+int rmt3d_centroids(rmt3d_ctx *ctx, double *out) { GUARD(ctx->sim.centroids(out)); }
 int rmt3d_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len) {
     GUARD(ctx->sim.slice(dim, point, o_type, obj_id, out, len));
 }

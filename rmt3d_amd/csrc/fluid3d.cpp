@@ -471,11 +471,12 @@ double Sim::estimate_bytes(const rmt3d_config &c, const rmt3d_object *o, int pre
         cap += 2.0 * (4.0 / 3.0) * M_PI * R0 * R0 * R0 / (dh * dh * dh);
     }
     cap = std::min(cap, Ng * std::max(1, n_obj));
-    double per_entry = 2 * (8.0 + 8.0 * 39) + 60;
+    double per_entry = 2 * (8.0 + 8.0 * 39) + 60 + 48;
     double fm = c.prd[0] ? m : m + 1, fn = c.prd[1] ? n : n + 1, fo = c.prd[2] ? k : k + 1;
     double mg_words = (m * n * k + fm * fn * fo) * 8.0 / 7.0;
     double mg = 3 * 8 * mg_words + (prec == RMT3D_PREC_MIXED ? 3 * 4 * mg_words : 0);
-    return Ng * per_cell + cap * per_entry + mg + 16.0 * fm * fn * fo + (1 << 20);
+    double diag = 8.0 * (7 * 256 * std::max(1, n_obj) + 4096);
+    return Ng * per_cell + cap * per_entry + mg + 16.0 * fm * fn * fo + diag + (1 << 20);
 }
 
 // ---------------------------------------------------------------------------
@@ -764,6 +765,8 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     for (int a = 0; a < 3; a++) {
         W.st_x[a] = (double *)dalloc(sizeof(double) * cap);
         W.st_xpred[a] = (double *)dalloc(sizeof(double) * cap);
+        W.st_en[a] = (double *)dalloc(sizeof(double) * cap);
+        W.st_detF[a] = (double *)dalloc(sizeof(double) * cap);
     }
     W.st_n = (int *)dalloc(sizeof(int));
     W.perm = (int *)dalloc(sizeof(int) * cap);
@@ -775,6 +778,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     d_solid_n = (int *)dalloc(sizeof(int));
     d_u64 = (unsigned long long *)dalloc(sizeof(unsigned long long) * 4);
     d_partial = (double *)dalloc(sizeof(double) * 1024);
+    d_diag = (double *)dalloc(sizeof(double) * diag_scratch_doubles(n_obj));
     d_mean = (double *)dalloc(sizeof(double) * 4);
     d_faces = dalloc(face_struct_size());
     d_rhs_tmp = (double *)dalloc(sizeof(double) * (size_t)g.fm * g.fn * g.fo * 2);
@@ -1116,7 +1120,7 @@ int Sim::step(int debug) {
     mark(0);
     launch_full_update(stream, g, R, ext);
     launch_redraw(stream, g, P, d_obj, R, ext, W, cfg.nlayers, d_status);
-    band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status);
+    band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status, 1);
     mark_end(0);
     mark(2);
     fill_bc_cc();
@@ -1250,6 +1254,63 @@ int Sim::download(rmt3d_state *st) {
         }
     }
     st->nband = nb;
+    HCHK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// macro diagnostics (stepper.hh:378-475, fluid_3d_utils.cc:507-1346)
+// ---------------------------------------------------------------------------
+int Sim::macro_data(rmt3d_macro *m) {
+    double l2 = div_u();
+    size_t nd = diag_scratch_doubles(n_obj);
+    double *out = d_diag + nd - 16;
+    launch_macro(stream, g, U, maps(), cfg.gravity, cfg.xyz_bounds[4], d_diag, out);
+    double r[10];
+    HCHK(hipMemcpyAsync(h_dbl, out, sizeof r, hipMemcpyDeviceToHost, stream));
+    HCHK(hipStreamSynchronize(stream));
+    memcpy(r, h_dbl, sizeof r);
+    double dv = g.dx * g.dy * g.dz;
+    m->divu_l2 = l2;
+    m->divu_max = r[7];
+    m->divu_min = -r[8];
+    // total_momentum: sum times dx*dy*dz, left to right (fluid_3d_utils.cc:799)
+    for (int a = 0; a < 3; a++) m->momentum[a] = r[a] * g.dx * g.dy * g.dz;
+    // total_energy (fluid_3d_utils.cc:858-869)
+    m->elastic = r[5] * dv;
+    m->kinetic = r[3] * dv;
+    m->potential = r[4] * dv;
+    m->power = 0. * dv;
+    m->energy = m->elastic + m->kinetic + m->potential;
+    // total_dev_in_detF (fluid_3d_utils.cc:1068-1084)
+    m->dev_detF_l2 = sqrt(r[6] * dv);
+    double mx = r[9];
+    if (mx == -1.0) mx = 0;
+    m->max_dev_detF = sqrt(mx);
+    return 0;
+}
+
+int Sim::centroids(double *out) {
+    if (n_obj <= 0) return 0;
+    size_t nd = diag_scratch_doubles(n_obj);
+    double *res = d_diag + nd - 16 - 7 * (size_t)n_obj;
+    launch_centroids(stream, g, P, d_obj, U, R, n_obj, d_diag, res);
+    std::vector<double> r(7 * (size_t)n_obj);
+    HCHK(hipMemcpyAsync(r.data(), res, sizeof(double) * r.size(), hipMemcpyDeviceToHost, stream));
+    HCHK(hipStreamSynchronize(stream));
+    // sim_manager::compute_avgs (sim_manager.hh:223-246); the average
+    // velocity feeds solid_acceleration's wall repulsion from now on
+    for (int i = 0; i < n_obj; i++) {
+        const double *q = &r[7 * (size_t)i];
+        double N = q[0];
+        ObjC &o = h_obj[i];
+        for (int a = 0; a < 3; a++) o.avg_vel[a] = (N != 0.) ? q[1 + a] / N : 0.;
+        if (out) {
+            for (int a = 0; a < 3; a++) out[6 * i + a] = (N != 0.) ? q[4 + a] / N : 0.;
+            for (int a = 0; a < 3; a++) out[6 * i + 3 + a] = o.avg_vel[a];
+        }
+    }
+    HCHK(hipMemcpyAsync(d_obj, h_obj.data(), sizeof(ObjC) * n_obj, hipMemcpyHostToDevice, stream));
     HCHK(hipStreamSynchronize(stream));
     return 0;
 }

@@ -128,6 +128,7 @@ class Sim {
     double *d_partial = nullptr, *d_mean = nullptr;
     void *d_faces = nullptr;
     double *d_rhs_tmp = nullptr;       // FEM rhs for div_u
+    double *d_diag = nullptr;          // diagnostics partials and results
     MGSolver mac, fem;
     std::vector<void *> allocs;
     // 0 FEM smoother colour pass (finest), 1 MAC smoother colour pass
@@ -158,6 +159,8 @@ class Sim {
     int band_count();
     int mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res);
     int write_checkpoint(int snum, int fnum, const char *dir);
+    int macro_data(rmt3d_macro *m);
+    int centroids(double *out);
     int slice_dims(int dim, int t, int &d1, int &d2) const;
     int slice(int dim, int point, int t, int obj_id, double *out, int *len);
     int write_slice(int dim, int point, int t, int obj_id, int format, const char *filename);

@@ -521,7 +521,7 @@ __global__ void k_sort_cells(Geo g, Band B, Scratch W, int nlayers) {
     }
 }
 
-__global__ void k_gather_entries(Band B, Scratch W) {
+__global__ void k_gather_entries(Band B, Scratch W, int carry) {
     int n = *B.n;
     if (n > B.cap) n = B.cap;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
@@ -530,7 +530,10 @@ __global__ void k_gather_entries(Band B, Scratch W) {
         B.c[e] = W.st_c[v];
         for (int a = 0; a < 3; a++) { B.x[a][e] = W.st_x[a][v]; B.xpred[a][e] = W.st_xpred[a][v]; }
         for (int a = 0; a < 9; a++) B.gphi[a][e] = -1.e200;
-        for (int a = 0; a < 3; a++) { B.detF[a][e] = -1.e200; B.en[a][e] = -1.e200; }
+        if (carry)
+            for (int a = 0; a < 3; a++) { B.detF[a][e] = W.st_detF[a][v]; B.en[a][e] = W.st_en[a][v]; }
+        else
+            for (int a = 0; a < 3; a++) { B.detF[a][e] = -1.e200; B.en[a][e] = -1.e200; }
         for (int a = 0; a < 18; a++) B.fx[a][e] = 0;
     }
 }
@@ -557,7 +560,7 @@ void launch_zero_u64(hipStream_t s, unsigned long long *p, int n) {
 static inline dim3 g1(int n) { return dim3((n + 255) / 256); }
 static const int GRID_STRIDE_BLOCKS = 2048;
 
-void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status) {
+void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status, int carry) {
     launch_zero_i32(s, B.cnt, g.Ng);
     launch_zero_i32(s, W.fill, g.Ng);
     hipLaunchKernelGGL(k_count_cells, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
@@ -568,7 +571,7 @@ void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, in
     hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(1), 0, s, B, status);
     hipLaunchKernelGGL(k_place, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
     hipLaunchKernelGGL(k_sort_cells, g1(g.Ng), dim3(256), 0, s, g, B, W, nlayers);
-    hipLaunchKernelGGL(k_gather_entries, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B, W);
+    hipLaunchKernelGGL(k_gather_entries, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B, W, carry);
 }
 
 // fluid_3d::extrapolate (fluid_3d_ex.cc:31-47); the caller has swapped the
@@ -720,23 +723,36 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
     if (t >= g.Ng) return;
     uint32_t lc[RD_MAX];
     double lx[RD_MAX][3], lp[RD_MAX][3];
+    // where each listed map came from: its band entry, or -1 for this
+    // cell's primary; the whole record moves in the reference
+    // (fluid_3d_ex.cc:345-389), so face energies and det F go with it
+    int lsrc[RD_MAX];
     int nl = 0;
     int s = B.start[t], n = B.cnt[t];
     for (int e = s; e < s + n && nl < RD_MAX - 1; e++, nl++) {
         lc[nl] = B.c[e];
+        lsrc[nl] = e;
         for (int a = 0; a < 3; a++) { lx[nl][a] = B.x[a][e]; lp[nl][a] = B.xpred[a][e]; }
     }
+    double pen[3], pdet[3];
     uint32_t lflag = (uint32_t)(nlayers + 1) << LDIG;
     uint32_t pc = R.c[t];
     double px[3] = {R.x[0][t], R.x[1][t], R.x[2][t]};
     double pp[3] = {R.xpred[0][t], R.xpred[1][t], R.xpred[2][t]};
     bool occupied = false, changed = false, cleared = false;
+    int take = -1;   // band entry that became the primary
     int oidp = (int)(pc & OMASK);
     if (oidp > 0) {
         double pphi = obj_phi(O, P.n_obj, oidp - 1, px[0], px[1], px[2]);
         if (pphi > 0) {
             lc[nl] = lflag | pc;
-            for (int a = 0; a < 3; a++) { lx[nl][a] = px[a]; lp[nl][a] = pp[a]; }
+            lsrc[nl] = -1;
+            for (int a = 0; a < 3; a++) {
+                lx[nl][a] = px[a];
+                lp[nl][a] = pp[a];
+                pen[a] = R.en[a][t];
+                pdet[a] = R.detF[a][t];
+            }
             nl++;
             pc = 0;
             for (int a = 0; a < 3; a++) { px[a] = 0; pp[a] = 0; }
@@ -759,6 +775,7 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             if (pphi < -g.dx) { atomicOr(status, ST_FATAL_REDRAW); return; }
             pc = oid;
             for (int a = 0; a < 3; a++) { px[a] = lx[l][a]; pp[a] = lp[l][a]; }
+            take = lsrc[l];
             occupied = true;
             changed = true;
             cleared = false;
@@ -768,6 +785,7 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             if (j0 >= 0) {
                 for (int b = j0; b < nl - 1; b++) {
                     lc[b] = lc[b + 1];
+                    lsrc[b] = lsrc[b + 1];
                     for (int a = 0; a < 3; a++) { lx[b][a] = lx[b + 1][a]; lp[b][a] = lp[b + 1][a]; }
                 }
                 nl--;
@@ -781,6 +799,8 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             for (int a = 0; a < 18; a++) R.fx[a][t] = 0;
             for (int a = 0; a < 9; a++) R.gphi[a][t] = 0;
             for (int a = 0; a < 3; a++) { R.detF[a][t] = 0; R.en[a][t] = 0; }
+        } else if (take >= 0) {
+            for (int a = 0; a < 3; a++) { R.detF[a][t] = B.detF[a][take]; R.en[a][t] = B.en[a][take]; }
         }
     }
     if (nl == 0) return;
@@ -791,6 +811,11 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
         W.st_cell[p] = t;
         W.st_c[p] = lc[e];
         for (int a = 0; a < 3; a++) { W.st_x[a][p] = lx[e][a]; W.st_xpred[a][p] = lp[e][a]; }
+        int src = lsrc[e];
+        for (int a = 0; a < 3; a++) {
+            W.st_en[a][p] = src >= 0 ? B.en[a][src] : pen[a];
+            W.st_detF[a][p] = src >= 0 ? B.detF[a][src] : pdet[a];
+        }
     }
 }
 void launch_redraw(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W, int nlayers,

@@ -130,7 +130,7 @@ __device__ __forceinline__ void solid_stress(const Geo &g, const Phys &P, const 
                 sg[symind(i, j)] = a;
                 cg[symind(i, j)] = b;
             }
-        M.enr(rf, F) = 0.5 * G * ((cg[0] + cg[3] + cg[5]) - 3 - 2 * log(detF));
+        M.enr(rf, F) = 0.5 * G * ((cg[0] + cg[3] + cg[5]) - 3 - 2 * det_log(detF));
         double savg = (sg[0] + sg[3] + sg[5]) / 3.0;
         for (int a = 0; a < 3; a++) sg[symind(a, a)] -= savg;
         double exmu = oc.ex_mu * tf * (1 + oc.ev_trans_mult * tderiv_func_in(P, phiv));

@@ -145,6 +145,7 @@ struct Scratch {
     int *st_cell;
     uint32_t *st_c;
     double *st_x[3], *st_xpred[3];
+    double *st_en[3], *st_detF[3];   // face energies and det F carried by a redraw
     int *st_n;
     int st_cap;
     int *perm;
@@ -156,7 +157,11 @@ struct Scratch {
 
 void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
                       int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj);
-void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status);
+// carry: take each entry's face energies and det F from the staging (a
+// redraw moves existing maps) instead of marking them not computed (fresh
+// extrapolated maps)
+void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status,
+                             int carry = 0);
 void launch_corrector(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Band T, int *err);
 void launch_revert(hipStream_t s, const Geo &g, RMap R, Band B, Band T);
 void launch_full_update(hipStream_t s, const Geo &g, RMap R, Band B);
@@ -177,6 +182,17 @@ void launch_chk_pack(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const
 void launch_chk_unpack(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R, const double *in,
                        hipEvent_t start, hipEvent_t stop);
 void launch_chk_reset(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R);
+
+// ---- k_diag.hip ----
+// macro.dat sums into out[0..6] (momentum 3, kinetic, potential, elastic,
+// squared det F deviation) and maxima out[7..9] (div max, -div min, max
+// squared det F deviation, -1 when no solid cell)
+void launch_macro(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, double gravity, double az,
+                  double *scratch, double *out);
+// per object 7 sums (H, u H, v H, w H, x H, y H, z H) into out[7*o + q]
+void launch_centroids(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const RMap &R,
+                      int nobj, double *scratch, double *out);
+size_t diag_scratch_doubles(int nobj);
 
 // ---- chk_pt.cpp (host) ----
 extern const char *const CHK_NAMES[8];

@@ -5,11 +5,13 @@
 // (dt clamp, frame step count, init_iter warm-up), then step frame by frame.
 // After each frame (and once before the first) it writes what dump_code
 // asks for: slices (bit 4, one per out_flag bit, write_params naming
-// "<dirname>/<field>.<frame>") and checkpoints (bit 8, every chkpt_freq
-// frames, "<dirname>/chk.<frame>").  Usage:
+// "<dirname>/<field>.<frame>"), checkpoints (bit 8, every chkpt_freq
+// frames, "<dirname>/chk.<frame>") and the macro.dat / centroid.dat records
+// (bit 32).  Usage:
 //   run_sim <file.cfg> [--mixed] [--steps N] [--quiet]
 // --steps N stops after N steps (as stop_short does).
 #include <sys/stat.h>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -40,6 +42,71 @@ static int write_files(rmt3d_ctx *ctx, const rmt3d_config &cfg, int snum, int fn
         mkdir(buf, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
         int rc = rmt3d_write_checkpoint(ctx, snum, fnum, buf);
         if (rc) return rc;
+    }
+    return 0;
+}
+
+static bool file_exists(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) return false;
+    fclose(f);
+    return true;
+}
+
+// stepper::collect_macro_data (stepper.hh:378-475): one macro.dat record
+// per call (the reference ends it without a newline), and with objects one
+// centroid.dat line; the headers are rewritten at the first call of a run
+static int collect_macro_data(rmt3d_ctx *ctx, const rmt3d_config &cfg, int frame, long long tot_step,
+                              long long init_step) {
+    if (!(cfg.dump_code & 32)) return 0;
+    char fn[1024];
+    snprintf(fn, sizeof fn, "%s/macro.dat", cfg.dirname);
+    if (!file_exists(fn) || tot_step == init_step) {
+        FILE *fh = fopen(fn, "w");
+        if (!fh) return RMT3D_ERR_ARG;
+        fprintf(fh,
+                "%10s %6s %6s %10s %6s %10s %6s %10s %10s %10s %10s %10s %10s %10s %10s %10s %10s %10s %16s %16s "
+                "%10s\n",
+                "#Time", "frame", "steps", "div_u_max", "rank", "div_u_min", "rank", "divu_L2", "tot_mmt_x",
+                "tot_mmt_y", "tot_mmt_z", "tot_energy", "kinetic", "elastic", "potential", "power", "max_dev_detF",
+                "dev_detF_L2", "sing. obj. cent.", "volume", "change in u");
+        fclose(fh);
+    }
+    rmt3d_macro m;
+    int rc = rmt3d_macro_data(ctx, &m);
+    if (rc) return rc;
+    const int digs = DBL_DIG;
+    double t = rmt3d_get_time(ctx);
+    FILE *fh = fopen(fn, "a");
+    if (!fh) return RMT3D_ERR_ARG;
+    fprintf(fh, "%.*e %06d %06lld %.*e %06d %.*e %06d ", digs, t, frame, tot_step, digs, m.divu_max, 0, digs,
+            m.divu_min, 0);
+    fprintf(fh, "%.*e ", digs, m.divu_l2);
+    fprintf(fh, "%.*e %.*e %.*e %.*e ", digs, m.momentum[0], digs, m.momentum[1], digs, m.momentum[2], digs,
+            m.energy);
+    fprintf(fh, "%.*e %.*e %.*e %.*e ", digs, m.kinetic, digs, m.elastic, digs, m.potential, digs, m.power);
+    fprintf(fh, "%.*e %.*e ", digs, m.max_dev_detF, digs, m.dev_detF_l2);
+    fclose(fh);
+    if (cfg.n_obj > 0 && cfg.sim_type != RMT3D_SIM_FTEST) {
+        std::vector<double> c(6 * (size_t)cfg.n_obj);
+        if ((rc = rmt3d_centroids(ctx, c.data()))) return rc;
+        snprintf(fn, sizeof fn, "%s/centroid.dat", cfg.dirname);
+        if (!file_exists(fn) || tot_step == init_step) {
+            fh = fopen(fn, "w");
+            if (!fh) return RMT3D_ERR_ARG;
+            fprintf(fh, "%10s %6s %6s %6s %6s %6s %6s %6s %6s->\n", "#Time", "frame", "steps", "x", "y", "z", "vx",
+                    "vy", "vz");
+            fclose(fh);
+        }
+        fh = fopen(fn, "a");
+        if (!fh) return RMT3D_ERR_ARG;
+        fprintf(fh, "%.*e %d %lld ", digs, t, frame, tot_step);
+        for (int i = 0; i < cfg.n_obj; i++) {
+            fprintf(fh, "%.*e %.*e %.*e ", digs, c[6 * i], digs, c[6 * i + 1], digs, c[6 * i + 2]);
+            fprintf(fh, "%.*e %.*e %.*e ", digs, c[6 * i + 3], digs, c[6 * i + 4], digs, c[6 * i + 5]);
+        }
+        fprintf(fh, "\n");
+        fclose(fh);
     }
     return 0;
 }
@@ -100,8 +167,11 @@ int main(int argc, char **argv) {
     printf("# %s: %dx%dx%d, %d object(s), %s, dt %.6g, %d steps per frame, %d step(s) to run\n", argv[1],
            cfg.sys_size[0], cfg.sys_size[1], cfg.sys_size[2], cfg.n_obj,
            precision == RMT3D_PREC_MIXED ? "mixed" : "fp64", dt, steps, total);
-    if (cfg.dump_code & 12) mkdir(cfg.dirname, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
-    if ((rc = write_files(ctx, cfg, init_step, init_frame))) {
+    if (cfg.dump_code & (8 | 4 | 32)) mkdir(cfg.dirname, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    // the first macro record is taken with the frame counter still at 0
+    // (stepper.hh:216, 341-344)
+    if ((rc = write_files(ctx, cfg, init_step, init_frame)) ||
+        (rc = collect_macro_data(ctx, cfg, 0, init_step, init_step))) {
         fprintf(stderr, "run_sim: output failed: %s\n", rmt3d_last_error(ctx));
         rmt3d_destroy(ctx);
         return 1;
@@ -127,7 +197,8 @@ int main(int argc, char **argv) {
                    rmt3d_get_time(ctx), divu, nmac, nfem);
             fflush(stdout);
         }
-        if ((rc = write_files(ctx, cfg, (int)tot_step, frame))) {
+        if ((rc = write_files(ctx, cfg, (int)tot_step, frame)) ||
+            (rc = collect_macro_data(ctx, cfg, frame, tot_step, init_step))) {
             fprintf(stderr, "run_sim: output failed: %s\n", rmt3d_last_error(ctx));
             status = rc;
             break;

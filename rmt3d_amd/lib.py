@@ -24,7 +24,7 @@ EXPORTS = [
     "rmt3d_estimate_bytes", "rmt3d_server_create", "rmt3d_server_submit", "rmt3d_server_wait",
     "rmt3d_server_info", "rmt3d_server_error", "rmt3d_server_result", "rmt3d_server_release",
     "rmt3d_server_stats", "rmt3d_server_destroy", "rmt3d_write_checkpoint", "rmt3d_write_params",
-    "rmt3d_slice", "rmt3d_write_slice",
+    "rmt3d_slice", "rmt3d_write_slice", "rmt3d_macro_data", "rmt3d_centroids",
 ]
 
 
@@ -82,6 +82,8 @@ def lib():
         L.rmt3d_server_stats.argtypes = [P, dp, ip, dp]
         L.rmt3d_server_destroy.argtypes = [P]
         L.rmt3d_server_destroy.restype = None
+        L.rmt3d_macro_data.argtypes = [P, C.POINTER(_abi.Macro)]
+        L.rmt3d_centroids.argtypes = [P, dp]
         L.rmt3d_write_checkpoint.argtypes = [P, C.c_int, C.c_int, C.c_char_p]
         L.rmt3d_write_params.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_char_p]
         L.rmt3d_slice.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, ip]
@@ -252,6 +254,20 @@ class Simulation:
                 setattr(st, k, _dp(a))
             keep.append(a)
         self._check(lib().rmt3d_upload_state(self.h, C.byref(st)), "upload")
+
+    # ---- diagnostics (stepper::collect_macro_data) ----
+    def macro_data(self):
+        m = _abi.Macro()
+        self._check(lib().rmt3d_macro_data(self.h, C.byref(m)), "macro_data")
+        return {k: getattr(m, k) for k in _abi.MACRO_FIELDS}
+
+    def centroids(self):
+        """compute_v_centroid: (n_obj, 6) centroid x y z, velocity x y z;
+        the average velocities are kept for the wall repulsion."""
+        k = max(1, self.cfg.n_obj)
+        out = np.zeros(6 * k)
+        self._check(lib().rmt3d_centroids(self.h, _dp(out)), "centroids")
+        return out.reshape(k, 6)[:self.cfg.n_obj]
 
     # ---- checkpoint and slice formats ----
     def write_checkpoint(self, snum, fnum, dirname):

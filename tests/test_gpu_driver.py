@@ -82,3 +82,21 @@ def test_run_sim_outputs_and_resume(tmp_path):
         assert (res / name).read_bytes() == (ref2 / name).read_bytes(), name
     text = (res / "recover.cfg").read_text()
     assert "chk_step                  %d#" % (10 + spf) in text
+
+
+def test_run_sim_macro_files(tmp_path):
+    # dump_code 32: one macro.dat record per frame plus the initial one (the
+    # reference writes them without line breaks) and one centroid.dat line
+    # per record
+    cfg, objs = configs.cavity_sphere(24)
+    cfg.dump_code = 32
+    path = str(tmp_path / "m.cfg")
+    configs.write_cfg(cfg, objs, path)
+    r = subprocess.run([DRIVER, path, "--steps", "20", "--quiet"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    lines = (tmp_path / "m.out" / "macro.dat").read_text().split("\n")
+    assert lines[0].split()[0] == "#Time" and len(lines) == 2
+    assert len(lines[1].split()) == 3 * 18
+    cent = (tmp_path / "m.out" / "centroid.dat").read_text().strip().split("\n")
+    assert len(cent) == 4 and all(len(l.split()) == 9 for l in cent[1:])
