@@ -242,6 +242,16 @@ def run_io(args):
             sim.write_checkpoint(args.warmup, 1, chk)
         t1 = time.perf_counter()
         pack_ms, pack_n = sim.profile_read(6)
+        # macro.dat / centroid.dat reductions of the same state
+        sim.macro_data()
+        sim.centroids()
+        tm0 = time.perf_counter()
+        for _ in range(args.io_reps):
+            sim.macro_data()
+            sim.centroids()
+        tm1 = time.perf_counter()
+        mac_ms, mac_n = sim.profile_read(8)
+        cen_ms, cen_n = sim.profile_read(9)
         st = sim.download()
         sim.profile(0)
         m, n, o = cfg.sys_size
@@ -257,6 +267,12 @@ def run_io(args):
         pack_bytes = file_bytes + N * (24 + 4 + 4) + o * fm * fn * 8 + n_solid * 24 + nband * 36
         pack_s = pack_ms / max(pack_n, 1) * 1e-3
         achieved = pack_bytes / pack_s / 1e9
+        # macro partials: per cell vel 3 words, six face velocities, c 4 B;
+        # per solid cell en 3 + det F 3 words of its own map and of its three
+        # upper neighbours' maps, and the band count of each of those cells
+        macro_bytes = N * (24 + 48 + 4) + n_solid * (48 + 3 * (48 + 4 + 4))
+        # centroid partials: c of every cell; x 3 and vel 3 words per solid cell
+        cen_bytes = N * 4 + n_solid * 48
         # resume from those files
         c2, o2 = rlib.parse_cfg(os.path.join(chk, "recover.cfg"))
         h = Simulation(c2, o2, _abi.PREC_FP64)
@@ -279,6 +295,9 @@ def run_io(args):
         t4 = time.perf_counter()
         img = orc.chk_images()
         t5 = time.perf_counter()
+        orc.macro_data()
+        orc.centroids()
+        t6 = time.perf_counter()
         orc.close()
         names = ("vx", "vy", "vz", "p", "X", "Y", "Z", "OID")
         match = all(np.array_equal(np.fromfile(os.path.join(chk, nm)), img[f].ravel()) for f, nm in enumerate(names))
@@ -305,7 +324,18 @@ def run_io(args):
                    "scatter": {"kernel": "k_chk_unpack", "avg_launch_us": round(unpack_ms / max(unpack_n, 1) * 1e3, 3),
                                "bytes_per_launch": unpack_bytes,
                                "achieved_gbs": round(unpack_bytes / (unpack_ms / max(unpack_n, 1) * 1e-3) / 1e9, 1)
-                               if unpack_ms > 0 else 0.0}},
+                               if unpack_ms > 0 else 0.0},
+                   "macro": {"kernel": "k_macro_partials", "avg_launch_us": round(mac_ms / max(mac_n, 1) * 1e3, 3),
+                             "bytes_per_launch": macro_bytes,
+                             "achieved_gbs": round(macro_bytes / (mac_ms / max(mac_n, 1) * 1e-3) / 1e9, 1)
+                             if mac_ms > 0 else 0.0,
+                             "ms_per_record": round(1e3 * (tm1 - tm0) / args.io_reps, 3),
+                             "oracle_ms_per_record": round(1e3 * (t6 - t5), 3)},
+                   "centroids": {"kernel": "k_centroid_partials",
+                                 "avg_launch_us": round(cen_ms / max(cen_n, 1) * 1e3, 3),
+                                 "bytes_per_launch": cen_bytes,
+                                 "achieved_gbs": round(cen_bytes / (cen_ms / max(cen_n, 1) * 1e-3) / 1e9, 1)
+                                 if cen_ms > 0 else 0.0}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_chk_pack",
                      "bytes_per_launch": pack_bytes, "avg_launch_us": round(pack_s * 1e6, 3), "launches": pack_n},

@@ -266,7 +266,8 @@ void *rmt3d_stream(rmt3d_ctx *ctx);
  * 4 one FEM V-cycle, 5 one MAC V-cycle (with the mixed path, one FP64
  * defect-correction step around an FP32 V-cycle), 6 the checkpoint gather
  * of rmt3d_write_checkpoint, 7 the checkpoint scatter of a resuming
- * rmt3d_initialize (6 and 7 are timed from level 1). */
+ * rmt3d_initialize, 8 the partial sums of rmt3d_macro_data, 9 those of
+ * rmt3d_centroids (6-9 are timed from level 1). */
 int rmt3d_profile(rmt3d_ctx *ctx, int enable);
 int rmt3d_profile_read(rmt3d_ctx *ctx, int which, double *ms, int *launches);
 /* hipDeviceSynchronize on the context's device. */

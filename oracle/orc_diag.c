@@ -7,8 +7,8 @@
  *   total_dev_in_detF, compute_v_centroid + sim_manager::compute_avgs
  *   (fluid_3d_utils.cc:507-1346, sim_manager.hh:223-246).
  * Each sum is formed as the reference forms its per-cell contribution, then
- * added over the interior cells in i-fastest order with orc_blocked_sum (the
- * fixed blocked order the HIP path uses), not serially.
+ * added over the interior cells in i-fastest order with diag_sum (the fixed
+ * blocked order the HIP path uses), not serially.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -17,6 +17,37 @@
 #include "orc.h"
 
 static inline int IXd(const orc_sim *S, int i, int j, int k) { return i + S->m4 * (j + S->n4 * k); }
+
+/* The fixed summation order of the HIP path's diagnostics: 2048 blocks of
+ * ceil(n/2048) consecutive terms, 256 strided partial sums per block and a
+ * binary tree; then 256 sums of 8 consecutive block partials, and a tree. */
+static double diag_sum(const double *a, int n)
+{
+    enum { NB = 2048, NT = 256, NF = NB / NT };
+    double part[NB];
+    int per = (n + NB - 1) / NB;
+    for (int b = 0; b < NB; b++) {
+        double sh[NT];
+        int s0 = b * per, e0 = s0 + per < n ? s0 + per : n;
+        for (int t = 0; t < NT; t++) {
+            double v = 0;
+            for (int e = s0 + t; e < e0; e += NT) v += a[e];
+            sh[t] = v;
+        }
+        for (int off = NT / 2; off > 0; off >>= 1)
+            for (int t = 0; t < off; t++) sh[t] += sh[t + off];
+        part[b] = sh[0];
+    }
+    double sh[NT];
+    for (int t = 0; t < NT; t++) {
+        double v = part[NF * t];
+        for (int i = 1; i < NF; i++) v += part[NF * t + i];
+        sh[t] = v;
+    }
+    for (int off = NT / 2; off > 0; off >>= 1)
+        for (int t = 0; t < off; t++) sh[t] += sh[t + off];
+    return sh[0];
+}
 
 /* out: divu_max, divu_min, divu_l2, momentum[3], energy, kinetic, elastic,
  * potential, power, max_dev_detF, dev_detF_l2 (the rmt3d_macro order) */
@@ -69,7 +100,7 @@ void orc_macro(orc_sim *S, double *out)
                 if (div < dmin) dmin = div;
             }
     double r[7];
-    for (int q = 0; q < 7; q++) r[q] = orc_blocked_sum(T + (size_t)q * N, N, 1.0);
+    for (int q = 0; q < 7; q++) r[q] = diag_sum(T + (size_t)q * N, N);
     free(T);
     double dv = S->dx * S->dy * S->dz;
     out[0] = dmax;
@@ -115,7 +146,7 @@ void orc_centroids(orc_sim *S, double *out)
                     T[(size_t)6 * N + e] = S->lz[kk + 2] * sfrac;
                 }
         double r[7];
-        for (int q = 0; q < 7; q++) r[q] = orc_blocked_sum(T + (size_t)q * N, N, 1.0);
+        for (int q = 0; q < 7; q++) r[q] = diag_sum(T + (size_t)q * N, N);
         double *av = S->avg_vel + 3 * ob;
         for (int a = 0; a < 3; a++) av[a] = (r[0] != 0.) ? r[1 + a] / r[0] : 0.;
         if (out) {

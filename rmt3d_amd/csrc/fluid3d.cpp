@@ -1265,7 +1265,9 @@ int Sim::macro_data(rmt3d_macro *m) {
     double l2 = div_u();
     size_t nd = diag_scratch_doubles(n_obj);
     double *out = d_diag + nd - 16;
-    launch_macro(stream, g, U, maps(), cfg.gravity, cfg.xyz_bounds[4], d_diag, out);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    kprof[8].pair(e0, e1);
+    launch_macro(stream, g, U, maps(), cfg.gravity, cfg.xyz_bounds[4], d_diag, out, e0, e1);
     double r[10];
     HCHK(hipMemcpyAsync(h_dbl, out, sizeof r, hipMemcpyDeviceToHost, stream));
     HCHK(hipStreamSynchronize(stream));
@@ -1294,7 +1296,9 @@ int Sim::centroids(double *out) {
     if (n_obj <= 0) return 0;
     size_t nd = diag_scratch_doubles(n_obj);
     double *res = d_diag + nd - 16 - 7 * (size_t)n_obj;
-    launch_centroids(stream, g, P, d_obj, U, R, n_obj, d_diag, res);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    kprof[9].pair(e0, e1);
+    launch_centroids(stream, g, P, d_obj, U, R, n_obj, d_diag, res, e0, e1);
     std::vector<double> r(7 * (size_t)n_obj);
     HCHK(hipMemcpyAsync(r.data(), res, sizeof(double) * r.size(), hipMemcpyDeviceToHost, stream));
     HCHK(hipStreamSynchronize(stream));

@@ -133,11 +133,11 @@ class Sim {
     std::vector<void *> allocs;
     // 0 FEM smoother colour pass (finest), 1 MAC smoother colour pass
     // (finest), 2 stress, 3 advection, 4 FEM V-cycle, 5 MAC V-cycle,
-    // 6 checkpoint gather, 7 checkpoint scatter, then the six stage buckets
-    // (extrapolation, pressure Poisson, communication, stresses, MAC,
-    // advection)
-    static const int NPROF_KERNEL = 8;
-    static const int NPROF = 14;
+    // 6 checkpoint gather, 7 checkpoint scatter, 8 macro.dat partial sums,
+    // 9 centroid partial sums, then the six stage buckets (extrapolation,
+    // pressure Poisson, communication, stresses, MAC, advection)
+    static const int NPROF_KERNEL = 10;
+    static const int NPROF = 16;
     KProf kprof[NPROF];
     int profile(int enable);
     int profile_read(int which, double *ms, int *launches);

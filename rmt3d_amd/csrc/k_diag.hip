@@ -2,19 +2,21 @@
 // fluid_3d::total_momentum, total_energy, total_dev_in_detF, the MAC part of
 // div_u and compute_v_centroid (fluid_3d_utils.cc:507-1346).  Each quantity
 // is a per-cell term over the interior cells in the reference's loop order
-// (i fastest), summed in a fixed blocked order: 256 blocks of consecutive
+// (i fastest), summed in a fixed blocked order: 2048 blocks of consecutive
 // cells, 256 work-items per block striding the block's range, a tree
-// within the block, then the 256 block partials added in order by one
-// work-item.  The same order is the oracle's, so FP64 results agree bit for
-// bit; against the reference's serial sum they agree to rounding.  Maxima
-// and minima are exact in any order.
+// within the block; then 256 work-items each add 8 consecutive block
+// partials in order, and a tree adds those.  The same order is the
+// oracle's, so FP64 results agree bit for bit; against the reference's
+// serial sum they agree to rounding.  Maxima and minima are exact in any
+// order.
+#include <hip/hip_ext.h>
 #include "launch.h"
 
 namespace rmt {
 
 namespace {
 
-constexpr int NBLK = 256, NTHR = 256;
+constexpr int NBLK = 2048, NTHR = 256, NFIN = NBLK / NTHR;
 
 __device__ __forceinline__ int IXd(const Geo &g, int i, int j, int k) { return i + g.m4 * (j + g.n4 * k); }
 
@@ -140,37 +142,42 @@ __global__ void __launch_bounds__(NTHR) k_centroid_partials(Geo g, Phys P, const
     if (t < 7) part[((size_t)o * 7 + t) * NBLK + b] = sh[t][0];
 }
 
-// The block partials of nq quantities added in block order; maxima (if
-// any) reduced the same way.
-__global__ void k_diag_finish(const double *part, int nq, const double *maxes, int nm, double *out) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nq) {
-        double s = 0;
-        for (int b = 0; b < NBLK; b++) s += part[(size_t)q * NBLK + b];
-        out[q] = s;
-    } else if (q < nq + nm) {
-        int a = q - nq;
-        double s = maxes[a * NBLK];
-        for (int b = 1; b < NBLK; b++) s = fmax(s, maxes[a * NBLK + b]);
-        out[q] = s;
+// The block partials of one quantity per workgroup (blockIdx.x): work-item
+// t adds partials 8t..8t+7 in order, then a tree; the maxima (after the nq
+// sums) reduce the same way.
+__global__ void __launch_bounds__(NTHR) k_diag_finish(const double *part, int nq, const double *maxes, double *out) {
+    __shared__ double sh[NTHR];
+    int q = blockIdx.x, t = threadIdx.x;
+    bool mx = q >= nq;
+    const double *p = mx ? maxes + (size_t)(q - nq) * NBLK : part + (size_t)q * NBLK;
+    double s = p[NFIN * t];
+    for (int i = 1; i < NFIN; i++) s = mx ? fmax(s, p[NFIN * t + i]) : s + p[NFIN * t + i];
+    sh[t] = s;
+    __syncthreads();
+    for (int off = NTHR / 2; off > 0; off >>= 1) {
+        if (t < off) sh[t] = mx ? fmax(sh[t], sh[t + off]) : sh[t] + sh[t + off];
+        __syncthreads();
     }
+    if (t == 0) out[q] = sh[0];
 }
 
 void launch_macro(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, double gravity, double az,
-                  double *scratch, double *out) {
+                  double *scratch, double *out, hipEvent_t start, hipEvent_t stop) {
     double *part = scratch, *maxes = scratch + 7 * NBLK;
-    hipLaunchKernelGGL(k_macro_partials, dim3(NBLK), dim3(NTHR), 0, s, g, U, M, gravity, az, part, maxes);
-    hipLaunchKernelGGL(k_diag_finish, dim3(1), dim3(64), 0, s, (const double *)part, 7, (const double *)maxes, 3,
+    hipExtLaunchKernelGGL(k_macro_partials, dim3(NBLK), dim3(NTHR), 0, s, start, stop, 0, g, U, M, gravity, az, part,
+                          maxes);
+    hipLaunchKernelGGL(k_diag_finish, dim3(10), dim3(NTHR), 0, s, (const double *)part, 7, (const double *)maxes,
                        out);
 }
 
 void launch_centroids(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const RMap &R,
-                      int nobj, double *scratch, double *out) {
+                      int nobj, double *scratch, double *out, hipEvent_t start, hipEvent_t stop) {
     if (nobj <= 0) return;
-    hipLaunchKernelGGL(k_centroid_partials, dim3(NBLK, nobj), dim3(NTHR), 0, s, g, P, O, U, R, scratch);
+    hipExtLaunchKernelGGL(k_centroid_partials, dim3(NBLK, nobj), dim3(NTHR), 0, s, start, stop, 0, g, P, O, U, R,
+                          scratch);
     int nq = 7 * nobj;
-    hipLaunchKernelGGL(k_diag_finish, dim3((nq + 255) / 256), dim3(256), 0, s, (const double *)scratch, nq,
-                       (const double *)nullptr, 0, out);
+    hipLaunchKernelGGL(k_diag_finish, dim3(nq), dim3(NTHR), 0, s, (const double *)scratch, nq,
+                       (const double *)nullptr, out);
 }
 
 size_t diag_scratch_doubles(int nobj) {

@@ -188,10 +188,10 @@ void launch_chk_reset(hipStream_t s, const Geo &g, const Fluid &U, const RMap &R
 // squared det F deviation) and maxima out[7..9] (div max, -div min, max
 // squared det F deviation, -1 when no solid cell)
 void launch_macro(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, double gravity, double az,
-                  double *scratch, double *out);
+                  double *scratch, double *out, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // per object 7 sums (H, u H, v H, w H, x H, y H, z H) into out[7*o + q]
 void launch_centroids(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const RMap &R,
-                      int nobj, double *scratch, double *out);
+                      int nobj, double *scratch, double *out, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 size_t diag_scratch_doubles(int nobj);
 
 // ---- chk_pt.cpp (host) ----
