@@ -226,6 +226,19 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * 16/17 -> colour passes per smoothing launch on the finest level. */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
+/* Tracers (Utilities/tracers.{hh,cc}): with dump_code & 1 and ntracers > 0
+ * rmt3d_initialize places cfg.ntracers tracers (uniform random positions
+ * from GSL's taus generator seeded with 1, or those of tr.<chk_num> in the
+ * checkpoint directory) and every rmt3d_step advects them with the
+ * tricubic interpolant of the velocity (improved Euler, slowed near no-slip
+ * walls).  rmt3d_write_tracers is tracers::print (tracers.cc:110-139):
+ * binary = an int count then 3 floats per tracer, else "%g %g %g" lines;
+ * a tracer that has left the box is written as 0 0 0.
+ * rmt3d_tracer_positions gives the same floats (*n: capacity in, count
+ * out; xyz NULL queries the count). */
+int rmt3d_write_tracers(rmt3d_ctx *ctx, const char *filename, int binary);
+int rmt3d_tracer_positions(rmt3d_ctx *ctx, float *xyz, int *n);
+
 /* The macro.dat quantities of stepper::collect_macro_data
  * (Sim/stepper.hh:378-440), reduced on the device.  Sums run in a fixed
  * blocked order (256 blocks of consecutive cells, a tree within each block,

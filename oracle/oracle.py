@@ -13,7 +13,7 @@ from rmt3d_amd import _abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_io.c", "orc_diag.c", "orc_api.c"]
+SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_io.c", "orc_diag.c", "orc_tracers.c", "orc_api.c"]
 
 _lib = None
 
@@ -129,7 +129,15 @@ def lib():
         L.oracle_lsq.argtypes = [C.c_int, dp, dp, dp, D, dp, dp, dp, dp]
         L.oracle_det_log.argtypes = [D]
         L.oracle_det_log.restype = D
+        L.oracle_det_exp.argtypes = [D]
+        L.oracle_det_exp.restype = D
         L.oracle_macro.argtypes = [P, dp]
+        L.oracle_tracer_count.argtypes = [P]
+        L.oracle_tracer_positions.argtypes = [P, C.POINTER(C.c_float)]
+        L.oracle_taus.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_uint32)]
+        L.oracle_tracer_interp.argtypes = [P, ip, dp, dp]
+        L.oracle_tracers_init.argtypes = [P]
+        L.oracle_tracers_move.argtypes = [P]
         L.oracle_centroids.argtypes = [P, dp]
         L.oracle_set_band.argtypes = [P, C.c_int, ip, C.POINTER(C.c_uint32), dp, dp]
         L.oracle_chk_images.argtypes = [P, dp]
@@ -160,6 +168,14 @@ def chk_read(dirname, dims):
     img = np.zeros((8, o, L))
     rc = lib().oracle_chk_read(os.fsencode(dirname), o, L, _dp(img))
     return rc, img
+
+
+def taus(seed, n):
+    """The first n outputs of gsl_rng_taus seeded with seed (the oracle's
+    restatement)."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_taus(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
 
 
 def slice_dims(cfg, dim, o_type):
@@ -337,6 +353,28 @@ class OracleSim:
         out = np.zeros(6 * k)
         lib().oracle_centroids(self.h, _dp(out))
         return out.reshape(k, 6)[:self.cfg.n_obj]
+
+    def tracer_interp(self, p, r):
+        """The tricubic velocity interpolant of the cube at padded node p."""
+        pa = (C.c_int * 3)(*p)
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        v = np.zeros(3)
+        lib().oracle_tracer_interp(self.h, pa, _dp(ra), _dp(v))
+        return v
+
+    def tracers_init(self):
+        return lib().oracle_tracers_init(self.h)
+
+    def tracers_move(self):
+        """One predict + advect of every tracer on the current field."""
+        lib().oracle_tracers_move(self.h)
+
+    def tracer_positions(self):
+        """tracers::print's positions: (n, 3) float32 by tracer id."""
+        n = lib().oracle_tracer_count(self.h)
+        out = np.zeros((max(n, 1), 3), dtype=np.float32)
+        lib().oracle_tracer_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out[:n]
 
     @property
     def nt(self):

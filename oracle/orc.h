@@ -124,6 +124,13 @@ typedef struct {
     double omega0[3];
 } orc_obj;
 
+/* tracer (Utilities/tracers.hh:31-56) */
+typedef struct {
+    int alive;
+    int p_crd[3];
+    double rp0[3], rpf[3], v0[3], vf[3];
+} orc_tracer;
+
 typedef struct orc_sim {
     rmt3d_config cfg;
     int n_obj;
@@ -164,6 +171,10 @@ typedef struct orc_sim {
     orc_mg mac, fem;
     double mac_res, fem_res;
     int mac_cycles, fem_cycles;
+
+    /* tracers */
+    int trace, ntr;
+    orc_tracer *tr;
 } orc_sim;
 
 /* orc_core.c */
@@ -190,6 +201,12 @@ void orc_chk_images(orc_sim *S, double *img);
 void orc_chk_restore(orc_sim *S, const double *img);
 int orc_chk_read(const char *dir, int o, size_t L, double *img);
 int orc_write_chk_pt(orc_sim *S, const char *dir);
+
+/* orc_tracers.c */
+int orc_tracers_init(orc_sim *S);
+void orc_tracers_predict(orc_sim *S);
+void orc_tracers_advect(orc_sim *S);
+void orc_tracer_positions(const orc_sim *S, float *xyz);
 
 /* orc_diag.c: macro.dat / centroid.dat reductions */
 void orc_macro(orc_sim *S, double *out);

@@ -211,7 +211,16 @@ int oracle_mg_direct(orc_sim *S, int which) { return (which ? S->fem.pinv : S->m
 double oracle_det_sin(double x) { return det_sin(x); }
 double oracle_det_cos(double x) { return det_cos(x); }
 double oracle_det_log(double x) { return det_log(x); }
+double oracle_det_exp(double x) { return det_exp(x); }
 void oracle_macro(orc_sim *S, double *out) { orc_macro(S, out); }
+int oracle_tracer_count(orc_sim *S) { return S->trace ? S->ntr : 0; }
+void orc_tracer_interp(const orc_sim *S, const int p[3], const double r[3], double v[3]);
+void oracle_tracer_interp(orc_sim *S, const int *p, const double *r, double *v) { orc_tracer_interp(S, p, r, v); }
+int oracle_tracers_init(orc_sim *S) { return orc_tracers_init(S); }
+void oracle_tracers_move(orc_sim *S) { orc_tracers_predict(S); orc_tracers_advect(S); }
+void oracle_tracer_positions(orc_sim *S, float *xyz) { orc_tracer_positions(S, xyz); }
+uint32_t orc_taus_stream(uint32_t seed, int n, uint32_t *out);
+void oracle_taus(uint32_t seed, int n, uint32_t *out) { orc_taus_stream(seed, n, out); }
 void oracle_centroids(orc_sim *S, double *out) { orc_centroids(S, out); }
 
 /* threads the oracle's parallel loops use (OpenMP; 1 without it) */

@@ -354,7 +354,7 @@ void orc_destroy(orc_sim *S)
 {
     if (!S) return;
     free(S->lx); free(S->ly); free(S->lz);
-    free(S->obj); free(S->sm); free(S->avg_vel);
+    free(S->obj); free(S->sm); free(S->avg_vel); free(S->tr);
     free(S->u); free(S->rm);
     orc_band_free(&S->ext); orc_band_free(&S->tmp);
     free(S->mask); free(S->list); free(S->next);
@@ -479,6 +479,7 @@ int orc_initialize(orc_sim *S)
         if (init_err < 0) return init_err;
         if (init_err == 0) {
             orc_set_bc_cc(S);
+            if ((S->cfg.dump_code & 1) && S->cfg.ntracers > 0 && orc_tracers_init(S)) return RMT3D_ERR_FATAL;
             compute_grad_basis(S);
             return 0;
         }
@@ -490,6 +491,8 @@ int orc_initialize(orc_sim *S)
     if (init_refmap(S)) return -1;
     init_extrapolate(S);
     orc_set_bc_cc(S);
+    /* fluid_3d::alloc_tracers (fluid_3d.cc:115-117, 217-219) */
+    if ((S->cfg.dump_code & 1) && S->cfg.ntracers > 0 && orc_tracers_init(S)) return RMT3D_ERR_FATAL;
     compute_grad_basis(S);
     return init_err;
 }
@@ -1321,6 +1324,7 @@ int orc_step(orc_sim *S, int debug)
 {
     (void)debug;
     S->nt++;
+    if (S->trace) orc_tracers_predict(S);   /* fluid_3d.cc:458-461 */
     orc_compute_stress(S);
     normal_derivatives_velocity(S);
     if (normal_derivatives_refmap(S)) return RMT3D_ERR_FATAL;
@@ -1347,6 +1351,7 @@ int orc_step(orc_sim *S, int debug)
     orc_redraw_boundary(S);
     orc_set_bc_cc(S);
     orc_communicate(S, ORC_COMM_DVEL);
+    if (S->trace) orc_tracers_advect(S);    /* fluid_3d.cc:569-572 */
     return RMT3D_NORMAL_EXIT;
 }
 

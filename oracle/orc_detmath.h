@@ -57,4 +57,21 @@ DET_FN double det_log(double x)
     double lf = 2 * s + 2 * s * (s2 * q);
     return e * DET_LN2_HI + (e * DET_LN2_LO + lf);
 }
+
+// Deterministic exponential, same operations on host and device: x = k ln2
+// + r with k = rint(x / ln2) and ln2 split as above (|r| <= 0.35), exp r
+// from its Taylor series through r^14 (the next term is below 1e-18),
+// scaled by 2^k.  Error against libm below 2 ulp.
+DET_FN double det_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.78) return INFINITY;
+    if (x < -745.2) return 0.0;
+    double k = rint(x * 1.4426950408889634);
+    double r = (x - k * DET_LN2_HI) - k * DET_LN2_LO;
+    double p = 1.0 + r * (1.0 + r * (1.0 / 2 + r * (1.0 / 6 + r * (1.0 / 24 + r * (1.0 / 120 + r * (1.0 / 720 +
+               r * (1.0 / 5040 + r * (1.0 / 40320 + r * (1.0 / 362880 + r * (1.0 / 3628800 + r * (1.0 / 39916800 +
+               r * (1.0 / 479001600 + r * (1.0 / 6227020800.0 + r * (1.0 / 87178291200.0))))))))))))));
+    return ldexp(p, (int)k);
+}
 #endif

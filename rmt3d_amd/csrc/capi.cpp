@@ -1,6 +1,8 @@
 // C ABI of include/rmt3d.h over rmt::Sim.  No exception crosses the boundary.
 #include <new>
 #include <string>
+#include <vector>
+#include <algorithm>
 #include "fluid3d.h"
 
 struct rmt3d_ctx {
@@ -81,6 +83,25 @@ int rmt3d_band_count(rmt3d_ctx *ctx) { GUARD(ctx->sim.band_count()); }
 int rmt3d_write_checkpoint(rmt3d_ctx *ctx, int snum, int fnum, const char *chk_dirname) {
     if (!chk_dirname) return RMT3D_ERR_ARG;
     GUARD(ctx->sim.write_checkpoint(snum, fnum, chk_dirname));
+}
+int rmt3d_write_tracers(rmt3d_ctx *ctx, const char *filename, int binary) {
+    if (!filename) return RMT3D_ERR_ARG;
+    GUARD(ctx->sim.write_tracers(filename, binary));
+}
+int rmt3d_tracer_positions(rmt3d_ctx *ctx, float *xyz, int *n) {
+    if (!ctx || !n) return RMT3D_ERR_ARG;
+    try {
+        std::vector<float> v;
+        int e = ctx->sim.tracer_positions(v);
+        if (e) return e;
+        int cnt = (int)(v.size() / 3);
+        if (!xyz || *n < cnt) { *n = cnt; return xyz ? RMT3D_ERR_ARG : 0; }
+        std::copy(v.begin(), v.end(), xyz);
+        *n = cnt;
+        return 0;
+    } catch (...) {
+        return RMT3D_ERR_FATAL;
+    }
 }
 int rmt3d_macro_data(rmt3d_ctx *ctx, rmt3d_macro *out) {
     if (!out) return RMT3D_ERR_ARG;

@@ -984,6 +984,10 @@ int Sim::initialize() {
         if (init_err < 0) return init_err;
         if (init_err == 0) {
             fill_bc_cc();
+            if ((cfg.dump_code & 1) && cfg.ntracers > 0) {
+                int e = init_tracers();
+                if (e) return e;
+            }
             compute_grad_basis();
             return check_status("initialize");
         }
@@ -999,6 +1003,11 @@ int Sim::initialize() {
     launch_zero_i32(stream, ext.n, 1);
     init_extrapolate();
     fill_bc_cc();
+    // fluid_3d::alloc_tracers (fluid_3d.cc:115-117, 217-219)
+    if ((cfg.dump_code & 1) && cfg.ntracers > 0) {
+        int e = init_tracers();
+        if (e) return e;
+    }
     compute_grad_basis();
     int e = check_status("initialize");
     if (e) return e;
@@ -1059,6 +1068,9 @@ int Sim::step(int debug) {
     P.dt = dt;
     P.time = time;
     launch_zero_i32(stream, d_status, 1);
+    // tracers::predict on the velocities at the start of the step
+    // (fluid_3d.cc:458-461)
+    if (trace) launch_tracer_predict(stream, TQ, TR, U, dt);
     Maps M = maps();
     mark(3);
     kprof[2].begin(stream);
@@ -1126,6 +1138,8 @@ int Sim::step(int debug) {
     fill_bc_cc();
     comm_fluid(ORC_DVEL);
     mark_end(2);
+    // tracers::advect on the end-of-step velocities (fluid_3d.cc:569-572)
+    if (trace) launch_tracer_advect(stream, TQ, TR, U, dt);
     e = check_status("step");
     if (e) return e;
     return RMT3D_NORMAL_EXIT;
@@ -1322,6 +1336,166 @@ int Sim::centroids(double *out) {
 // ---------------------------------------------------------------------------
 // checkpoint and slice output (fluid_3d_chk_pt.cc:3-72, fluid_3d_io.cc:3-418)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// tracers (Utilities/tracers.{hh,cc})
+// ---------------------------------------------------------------------------
+namespace {
+// gsl_rng_taus (GSL 2.x, taus.c): L'Ecuyer's maximally equidistributed
+// combined Tausworthe generator (Math. Comp. 65 (1996) 203-213) with GSL's
+// seeding by the LCG 69069 and six warm-up draws; uniform() is the 32-bit
+// output over 2^32
+struct Taus {
+    uint32_t s1, s2, s3;
+    explicit Taus(uint32_t seed) {
+        if (seed == 0) seed = 1;
+        s1 = 69069u * seed;
+        if (s1 < 2) s1 += 2;
+        s2 = 69069u * s1;
+        if (s2 < 8) s2 += 8;
+        s3 = 69069u * s2;
+        if (s3 < 16) s3 += 16;
+        for (int i = 0; i < 6; i++) next();
+    }
+    uint32_t next() {
+        s1 = (((s1 & 4294967294u) << 12) ^ (((s1 << 13) ^ s1) >> 19));
+        s2 = (((s2 & 4294967288u) << 4) ^ (((s2 << 2) ^ s2) >> 25));
+        s3 = (((s3 & 4294967280u) << 17) ^ (((s3 << 3) ^ s3) >> 11));
+        return s1 ^ s2 ^ s3;
+    }
+    double uniform() { return next() / 4294967296.0; }
+};
+}  // namespace
+
+// tracers::tracers (tracers.cc:7-66): ntracers uniform random positions in
+// the box (taus seeded with 1), or the positions of the checkpoint's
+// tr.<chk_num>; tracers::set (tracers.hh:160-172) places each in its cube
+int Sim::init_tracers() {
+    int n = cfg.ntracers;
+    const double h[3] = {g.dx, g.dy, g.dz};
+    const int cells[3] = {g.m, g.n, g.o};
+    const double low[3] = {hx[0], hy[0], hz[0]};
+    for (int f = 0; f < 3; f++) {
+        TQ.h[f] = h[f];
+        TQ.low[f] = low[f];
+        double glow = cfg.xyz_bounds[2 * f] + (0 + 0.5) * h[f];   // gx[0] (fluid_3d.cc:132)
+        TQ.lo[f] = glow - 0.5 * h[f];
+        TQ.hi[f] = TQ.lo[f] + cells[f] * h[f];
+        TQ.prd[f] = g.prd[f];
+        for (int lr = 0; lr < 2; lr++) {
+            TQ.bct[f][lr] = cfg.bct[f][2 * f + lr];
+            TQ.bcv[f][lr] = cfg.bcv[f][2 * f + lr];
+        }
+    }
+    TQ.lim[0] = g.m4; TQ.lim[1] = g.n4; TQ.lim[2] = g.o4;
+    std::vector<double> pos(3 * (size_t)n);
+    if (cfg.chk_num <= 0) {
+        Taus rng(1);
+        double ax = (cfg.xyz_bounds[0] + 0.5 * h[0]) - 0.5 * h[0];
+        double ay = (cfg.xyz_bounds[2] + 0.5 * h[1]) - 0.5 * h[1];
+        double az = (cfg.xyz_bounds[4] + 0.5 * h[2]) - 0.5 * h[2];
+        double Lx = g.m * h[0], Ly = g.n * h[1], Lz = g.o * h[2];
+        for (int i = 0; i < n; i++) {
+            double x = ax + Lx * rng.uniform();
+            double y = ay + Ly * rng.uniform();
+            double z = az + Lz * rng.uniform();
+            pos[3 * i] = x; pos[3 * i + 1] = y; pos[3 * i + 2] = z;
+        }
+    } else {
+        // tracers::init_from_chkpt_files (tracers.cc:142-172)
+        char fn[600];
+        snprintf(fn, sizeof fn, "%s/tr.%05d", cfg.chk_dirname, cfg.chk_num);
+        printf("tracers:: initializing tracers from %s\n", fn);
+        FILE *fh = fopen(fn, "rb");
+        if (!fh) { err = std::string("cannot open ") + fn; return RMT3D_ERR_FATAL; }
+        int lnum = 0;
+        std::vector<float> tout(3 * (size_t)n);
+        size_t r1 = fread(&lnum, sizeof(int), 1, fh);
+        size_t r2 = (r1 == 1 && lnum == n) ? fread(tout.data(), sizeof(float), 3 * (size_t)n, fh) : 0;
+        fclose(fh);
+        if (r1 != 1 || lnum != n || r2 != 3 * (size_t)n) {
+            err = "tracer:: Reading check point file failed, or read size doesn't match that specified in cfg file.";
+            return RMT3D_ERR_FATAL;
+        }
+        for (size_t a = 0; a < 3 * (size_t)n; a++) pos[a] = tout[a];
+    }
+    std::vector<int> alive(n), pc[3];
+    std::vector<double> rp[3];
+    for (int f = 0; f < 3; f++) { pc[f].assign(n, 0); rp[f].assign(n, 0.0); }
+    for (int i = 0; i < n; i++) {
+        bool ok = true;
+        for (int f = 0; f < 3 && ok; f++) {
+            double t_ind = (pos[3 * i + f] - low[f]) / h[f] - 1;
+            int p = (int)t_ind;
+            double r = t_ind - p - 0.5;
+            pc[f][i] = p;
+            rp[f][i] = r;
+            bool lo = (p < 0) || (p == 0 && r < 0);
+            bool hi = (p > TQ.lim[f] - 4) || (p == TQ.lim[f] - 4 && r >= 0);
+            if (lo || hi) ok = false;
+        }
+        alive[i] = ok;
+    }
+    if (!TR.alive || TR.n != n) {
+        TR.n = n;
+        TR.alive = (int *)dalloc(sizeof(int) * n);
+        for (int f = 0; f < 3; f++) {
+            TR.p[f] = (int *)dalloc(sizeof(int) * n);
+            TR.rp0[f] = (double *)dalloc(sizeof(double) * n);
+            TR.rpf[f] = (double *)dalloc(sizeof(double) * n);
+            TR.v0[f] = (double *)dalloc(sizeof(double) * n);
+            TR.vf[f] = (double *)dalloc(sizeof(double) * n);
+        }
+        if (dalloc_failed) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    }
+    HCHK(hipMemcpyAsync(TR.alive, alive.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+    for (int f = 0; f < 3; f++) {
+        HCHK(hipMemcpyAsync(TR.p[f], pc[f].data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(TR.rp0[f], rp[f].data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(TR.rpf[f], rp[f].data(), sizeof(double) * n, hipMemcpyHostToDevice, stream));
+    }
+    HCHK(hipStreamSynchronize(stream));
+    trace = true;
+    return 0;
+}
+
+// tracers::print (tracers.cc:110-139): single-precision positions by id,
+// 0 for a tracer that has left the box
+int Sim::tracer_positions(std::vector<float> &xyz) {
+    int n = TR.n;
+    xyz.assign(3 * (size_t)n, 0.f);
+    if (!trace || n <= 0) return 0;
+    std::vector<int> alive(n), pc[3];
+    std::vector<double> rp[3];
+    HCHK(hipMemcpyAsync(alive.data(), TR.alive, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    for (int f = 0; f < 3; f++) {
+        pc[f].resize(n);
+        rp[f].resize(n);
+        HCHK(hipMemcpyAsync(pc[f].data(), TR.p[f], sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        HCHK(hipMemcpyAsync(rp[f].data(), TR.rp0[f], sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+    }
+    HCHK(hipStreamSynchronize(stream));
+    for (int i = 0; i < n; i++)
+        if (alive[i])
+            for (int f = 0; f < 3; f++) xyz[3 * (size_t)i + f] = (float)(TQ.low[f] + (1.5 + rp[f][i] + pc[f][i]) * TQ.h[f]);
+    return 0;
+}
+
+int Sim::write_tracers(const char *filename, int binary) {
+    std::vector<float> xyz;
+    int e = tracer_positions(xyz);
+    if (e) return e;
+    FILE *fh = fopen(filename, binary ? "wb" : "w");
+    if (!fh) { err = std::string("cannot open ") + filename; return RMT3D_ERR_ARG; }
+    int num = TR.n;
+    if (binary) {
+        fwrite(&num, sizeof(int), 1, fh);
+        fwrite(xyz.data(), sizeof(float), xyz.size(), fh);
+    } else {
+        for (size_t a = 0; a < xyz.size(); a += 3) fprintf(fh, "%g %g %g\n", xyz[a], xyz[a + 1], xyz[a + 2]);
+    }
+    return fclose(fh) == 0 ? 0 : RMT3D_ERR_ARG;
+}
+
 int Sim::write_checkpoint(int snum, int fnum, const char *dir) {
     size_t L = (size_t)(g.m + 1) * (g.n + 1), n8 = 8 * (size_t)g.o * L;
     std::vector<double> img(n8);
@@ -1334,6 +1508,21 @@ int Sim::write_checkpoint(int snum, int fnum, const char *dir) {
     if (he == hipSuccess) he = hipStreamSynchronize(stream);
     hipFree(d);
     HCHK(he);
+    // the run's tracer file of this frame goes with the checkpoint
+    // (fluid_3d_chk_pt.cc:28-32)
+    if (cfg.dirname[0]) {
+        char src[600], dst[600];
+        snprintf(src, sizeof src, "%s/tr.%05d", cfg.dirname, fnum);
+        snprintf(dst, sizeof dst, "%s/tr.%05d", dir, fnum);
+        if (FILE *in = fopen(src, "rb")) {
+            FILE *out = fopen(dst, "wb");
+            char buf[65536];
+            size_t k;
+            while (out && (k = fread(buf, 1, sizeof buf, in)) > 0) fwrite(buf, 1, k, out);
+            if (out) fclose(out);
+            fclose(in);
+        }
+    }
     int e = chk_write_files(dir, img.data(), g.o, L, err);
     if (e) return e;
     cfg.chk_num = fnum;

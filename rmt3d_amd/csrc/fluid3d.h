@@ -159,6 +159,13 @@ class Sim {
     int band_count();
     int mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res);
     int write_checkpoint(int snum, int fnum, const char *dir);
+    // tracers (Utilities/tracers.{hh,cc})
+    bool trace = false;
+    TrParams TQ{};
+    TracerDev TR{};
+    int init_tracers();
+    int tracer_positions(std::vector<float> &xyz);
+    int write_tracers(const char *filename, int binary);
     int macro_data(rmt3d_macro *m);
     int centroids(double *out);
     int slice_dims(int dim, int t, int &d1, int &d2) const;

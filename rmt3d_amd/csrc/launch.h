@@ -194,6 +194,27 @@ void launch_centroids(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
                       int nobj, double *scratch, double *out, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 size_t diag_scratch_doubles(int nobj);
 
+// ---- k_tracers.hip ----
+// tracer constants: cell size, the padded grid's lowest cell centre (low),
+// the box (lo, hi), padded node counts, periodicity and the boundary type
+// and value of each velocity component on its own low / high face
+struct TrParams {
+    double h[3], low[3], lo[3], hi[3];
+    int lim[3];
+    int prd[3];
+    int bct[3][2];
+    double bcv[3][2];
+};
+// tracer state, one entry per tracer id (structure of arrays)
+struct TracerDev {
+    int n;
+    int *alive;
+    int *p[3];
+    double *rp0[3], *rpf[3], *v0[3], *vf[3];
+};
+void launch_tracer_predict(hipStream_t s, const TrParams &Q, const TracerDev &T, const Fluid &U, double dt);
+void launch_tracer_advect(hipStream_t s, const TrParams &Q, const TracerDev &T, const Fluid &U, double dt);
+
 // ---- chk_pt.cpp (host) ----
 extern const char *const CHK_NAMES[8];
 // writes the eight files from their images (8 x o x L doubles)

@@ -24,7 +24,8 @@ EXPORTS = [
     "rmt3d_estimate_bytes", "rmt3d_server_create", "rmt3d_server_submit", "rmt3d_server_wait",
     "rmt3d_server_info", "rmt3d_server_error", "rmt3d_server_result", "rmt3d_server_release",
     "rmt3d_server_stats", "rmt3d_server_destroy", "rmt3d_write_checkpoint", "rmt3d_write_params",
-    "rmt3d_slice", "rmt3d_write_slice", "rmt3d_macro_data", "rmt3d_centroids",
+    "rmt3d_slice", "rmt3d_write_slice", "rmt3d_macro_data", "rmt3d_centroids", "rmt3d_write_tracers",
+    "rmt3d_tracer_positions",
 ]
 
 
@@ -83,6 +84,8 @@ def lib():
         L.rmt3d_server_destroy.argtypes = [P]
         L.rmt3d_server_destroy.restype = None
         L.rmt3d_macro_data.argtypes = [P, C.POINTER(_abi.Macro)]
+        L.rmt3d_write_tracers.argtypes = [P, C.c_char_p, C.c_int]
+        L.rmt3d_tracer_positions.argtypes = [P, C.POINTER(C.c_float), ip]
         L.rmt3d_centroids.argtypes = [P, dp]
         L.rmt3d_write_checkpoint.argtypes = [P, C.c_int, C.c_int, C.c_char_p]
         L.rmt3d_write_params.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_char_p]
@@ -268,6 +271,19 @@ class Simulation:
         out = np.zeros(6 * k)
         self._check(lib().rmt3d_centroids(self.h, _dp(out)), "centroids")
         return out.reshape(k, 6)[:self.cfg.n_obj]
+
+    # ---- tracers ----
+    def tracer_positions(self):
+        """tracers::print's positions: (n, 3) float32 by tracer id."""
+        n = C.c_int(0)
+        self._check(lib().rmt3d_tracer_positions(self.h, None, C.byref(n)), "tracer_positions")
+        out = np.zeros((max(n.value, 1), 3), dtype=np.float32)
+        self._check(lib().rmt3d_tracer_positions(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)),
+                    "tracer_positions")
+        return out[:n.value]
+
+    def write_tracers(self, filename, binary=True):
+        self._check(lib().rmt3d_write_tracers(self.h, os.fsencode(filename), int(binary)), "write_tracers")
 
     # ---- checkpoint and slice formats ----
     def write_checkpoint(self, snum, fnum, dirname):
