@@ -239,6 +239,20 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 int rmt3d_write_tracers(rmt3d_ctx *ctx, const char *filename, int binary);
 int rmt3d_tracer_positions(rmt3d_ctx *ctx, float *xyz, int *n);
 
+/* Contours (fluid_3d::output_contours, fluid_3d_io.cc:595-958): the zero
+ * level set of each object's phi over the lattice of cell centres,
+ * triangulated by marching cubes, written as the ctr file: int n_obj, then
+ * per object int nvert, int ntri and, if nvert > 0, nvert float xyz
+ * positions, nvert float normals and ntri int vertex triples.  Vertex
+ * positions, normals and order follow the reference; the triangle table is
+ * derived by this library (rmt3d_amd/csrc/gen_mc_table.py) with the
+ * reference's corner and edge numbering and its triangle count per box, but
+ * its own split of each polygon into triangles.  rmt3d_contour returns
+ * object obj's (1-based) arrays; with all three arrays NULL it only sets
+ * the counts. */
+int rmt3d_write_contours(rmt3d_ctx *ctx, const char *filename);
+int rmt3d_contour(rmt3d_ctx *ctx, int obj, float *pts, float *nrm, int *nvert, int *tri, int *ntri);
+
 /* The macro.dat quantities of stepper::collect_macro_data
  * (Sim/stepper.hh:378-440), reduced on the device.  Sums run in a fixed
  * blocked order (256 blocks of consecutive cells, a tree within each block,

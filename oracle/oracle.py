@@ -13,7 +13,7 @@ from rmt3d_amd import _abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_io.c", "orc_diag.c", "orc_tracers.c", "orc_api.c"]
+SOURCES = ["orc_core.c", "orc_extrap.c", "orc_comm.c", "orc_mg.c", "orc_io.c", "orc_diag.c", "orc_tracers.c", "orc_contour.c", "orc_api.c"]
 
 _lib = None
 
@@ -136,6 +136,8 @@ def lib():
         L.oracle_tracer_positions.argtypes = [P, C.POINTER(C.c_float)]
         L.oracle_taus.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_uint32)]
         L.oracle_tracer_interp.argtypes = [P, ip, dp, dp]
+        L.oracle_mc_table.argtypes = [C.c_int, C.POINTER(C.c_byte)]
+        L.oracle_contour.argtypes = [P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, ip, C.c_int, ip]
         L.oracle_tracers_init.argtypes = [P]
         L.oracle_tracers_move.argtypes = [P]
         L.oracle_centroids.argtypes = [P, dp]
@@ -175,6 +177,16 @@ def taus(seed, n):
     restatement)."""
     out = np.zeros(n, dtype=np.uint32)
     lib().oracle_taus(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def mc_table():
+    """The oracle's marching-cubes table: list of triangle edge triples per key."""
+    out = []
+    buf = (C.c_byte * 64)()
+    for key in range(256):
+        n = lib().oracle_mc_table(key, buf)
+        out.append([tuple(buf[3 * t + v] for v in range(3)) for t in range(n)])
     return out
 
 
@@ -353,6 +365,19 @@ class OracleSim:
         out = np.zeros(6 * k)
         lib().oracle_centroids(self.h, _dp(out))
         return out.reshape(k, 6)[:self.cfg.n_obj]
+
+    def contour(self, cnum, cap=200000):
+        """Object cnum's (1-based) contour: (pts, normals, triangles)."""
+        pts = np.zeros((cap, 3), dtype=np.float32)
+        nrm = np.zeros((cap, 3), dtype=np.float32)
+        tri = np.zeros((2 * cap, 3), dtype=np.int32)
+        nt = C.c_int()
+        nv = lib().oracle_contour(self.h, cnum, pts.ctypes.data_as(C.POINTER(C.c_float)),
+                                  nrm.ctypes.data_as(C.POINTER(C.c_float)), cap,
+                                  tri.ctypes.data_as(C.POINTER(C.c_int)), 2 * cap, C.byref(nt))
+        if nv < 0:
+            raise ValueError("contour larger than the buffers")
+        return pts[:nv], nrm[:nv], tri[:nt.value]
 
     def tracer_interp(self, p, r):
         """The tricubic velocity interpolant of the cube at padded node p."""

@@ -214,6 +214,26 @@ double oracle_det_log(double x) { return det_log(x); }
 double oracle_det_exp(double x) { return det_exp(x); }
 void oracle_macro(orc_sim *S, double *out) { orc_macro(S, out); }
 int oracle_tracer_count(orc_sim *S) { return S->trace ? S->ntr : 0; }
+int orc_mc_table(int key, signed char *tri);
+int orc_contour(orc_sim *S, int cnum, float **pts, float **nrm, int **tri, int *ntri);
+int oracle_mc_table(int key, signed char *tri) { return orc_mc_table(key, tri); }
+/* one object's contour into caller arrays of capacity cap vertices / tcap
+ * triangles; returns the vertex count (or -1 when it does not fit) */
+int oracle_contour(orc_sim *S, int cnum, float *pts, float *nrm, int cap, int *tri, int tcap, int *ntri)
+{
+    float *p, *n;
+    int *t, nt;
+    int nv = orc_contour(S, cnum, &p, &n, &t, &nt);
+    *ntri = nt;
+    int ok = nv <= cap && nt <= tcap;
+    if (ok) {
+        memcpy(pts, p, sizeof(float) * 3 * (size_t)nv);
+        memcpy(nrm, n, sizeof(float) * 3 * (size_t)nv);
+        memcpy(tri, t, sizeof(int) * 3 * (size_t)nt);
+    }
+    free(p); free(n); free(t);
+    return ok ? nv : -1;
+}
 void orc_tracer_interp(const orc_sim *S, const int p[3], const double r[3], double v[3]);
 void oracle_tracer_interp(orc_sim *S, const int *p, const double *r, double *v) { orc_tracer_interp(S, p, r, v); }
 int oracle_tracers_init(orc_sim *S) { return orc_tracers_init(S); }

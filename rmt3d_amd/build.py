@@ -16,9 +16,9 @@ LIB = os.path.join(HERE, "librmt3d.so")
 DRIVER = os.path.join(HERE, "run_sim")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["k_fluid.hip", "k_mg.hip", "k_band.hip", "k_io.hip", "k_diag.hip", "k_tracers.hip", "fluid3d.cpp", "sim_params.cpp", "chk_pt.cpp",
+SOURCES = ["k_fluid.hip", "k_mg.hip", "k_band.hip", "k_io.hip", "k_diag.hip", "k_tracers.hip", "k_contour.hip", "fluid3d.cpp", "sim_params.cpp", "chk_pt.cpp",
            "capi.cpp", "serve.cpp"]
-HEADERS = ["dev.h", "launch.h", "fluid3d.h", "detmath.h"]
+HEADERS = ["dev.h", "launch.h", "fluid3d.h", "detmath.h", "mc_table.h"]
 # FP64 parity path: no contraction so the arithmetic order is the reference's;
 # packed FP32 instructions stay disabled on this hardware pool.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -84,6 +84,32 @@ int rmt3d_write_checkpoint(rmt3d_ctx *ctx, int snum, int fnum, const char *chk_d
     if (!chk_dirname) return RMT3D_ERR_ARG;
     GUARD(ctx->sim.write_checkpoint(snum, fnum, chk_dirname));
 }
+int rmt3d_write_contours(rmt3d_ctx *ctx, const char *filename) {
+    if (!filename) return RMT3D_ERR_ARG;
+    GUARD(ctx->sim.write_contours(filename));
+}
+int rmt3d_contour(rmt3d_ctx *ctx, int obj, float *pts, float *nrm, int *nvert, int *tri, int *ntri) {
+    if (!ctx || !nvert || !ntri || obj < 1) return RMT3D_ERR_ARG;
+    try {
+        std::vector<float> p, n;
+        std::vector<int> t;
+        int e = ctx->sim.contour(obj, p, n, t);
+        if (e) return e;
+        int nv = (int)(p.size() / 3), nt = (int)(t.size() / 3);
+        bool fits = pts && nrm && tri && *nvert >= nv && *ntri >= nt;
+        int query = !pts && !nrm && !tri;
+        *nvert = nv;
+        *ntri = nt;
+        if (query) return 0;
+        if (!fits) return RMT3D_ERR_ARG;
+        std::copy(p.begin(), p.end(), pts);
+        std::copy(n.begin(), n.end(), nrm);
+        std::copy(t.begin(), t.end(), tri);
+        return 0;
+    } catch (...) {
+        return RMT3D_ERR_FATAL;
+    }
+}
 int rmt3d_write_tracers(rmt3d_ctx *ctx, const char *filename, int binary) {
     if (!filename) return RMT3D_ERR_ARG;
     GUARD(ctx->sim.write_tracers(filename, binary));

@@ -1496,6 +1496,91 @@ int Sim::write_tracers(const char *filename, int binary) {
     return fclose(fh) == 0 ? 0 : RMT3D_ERR_ARG;
 }
 
+// ---------------------------------------------------------------------------
+// contours (fluid_3d_io.cc:595-958)
+// ---------------------------------------------------------------------------
+int Sim::contour(int cnum, std::vector<float> &pts, std::vector<float> &nrm, std::vector<int> &tri) {
+    size_t nn = (size_t)(g.m + 1) * (g.n + 1) * (g.o + 1);
+    std::vector<void *> mem;
+    auto get = [&](size_t bytes) -> void * {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 8) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+        mem.push_back(q);
+        return q;
+    };
+    auto release = [&]() { for (void *q : mem) hipFree(q); };
+    CtBufs B{};
+    B.phi = (double *)get(sizeof(double) * g.Ng);
+    B.vcnt = (int *)get(sizeof(int) * nn);
+    B.voff = (int *)get(sizeof(int) * nn);
+    B.tcnt = (int *)get(sizeof(int) * nn);
+    B.toff = (int *)get(sizeof(int) * nn);
+    B.vidx = (int *)get(sizeof(int) * 3 * nn);
+    B.bsum = (int *)get(sizeof(int) * (nn / 1024 + 2));
+    B.tot = (int *)get(sizeof(int) * 2);
+    // cell-centre coordinates gx, gy, gz (fluid_3d.cc:132-134), continued to
+    // index m, where the reference reads past its arrays
+    std::vector<double> hc[3];
+    const int cells[3] = {g.m, g.n, g.o};
+    const double hh[3] = {g.dx, g.dy, g.dz};
+    double *dc[3];
+    for (int a = 0; a < 3; a++) {
+        hc[a].resize(cells[a] + 1);
+        for (int i = 0; i <= cells[a]; i++) hc[a][i] = cfg.xyz_bounds[2 * a] + (i + 0.5) * hh[a];
+        dc[a] = (double *)get(sizeof(double) * (cells[a] + 1));
+    }
+    for (void *q : mem)
+        if (!q) { release(); err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    for (int a = 0; a < 3; a++)
+        hipMemcpyAsync(dc[a], hc[a].data(), sizeof(double) * (cells[a] + 1), hipMemcpyHostToDevice, stream);
+    B.gx = dc[0]; B.gy = dc[1]; B.gz = dc[2];
+    launch_contour_count(stream, g, d_obj, n_obj, maps(), cnum, B);
+    int tot[2] = {0, 0};
+    hipMemcpyAsync(h_int, B.tot, sizeof(int) * 2, hipMemcpyDeviceToHost, stream);
+    hipError_t he = hipStreamSynchronize(stream);
+    if (he != hipSuccess) { release(); HCHK(he); }
+    tot[0] = h_int[0];
+    tot[1] = h_int[1];
+    B.pts = (float *)get(sizeof(float) * 3 * (size_t)tot[0]);
+    B.nrm = (float *)get(sizeof(float) * 3 * (size_t)tot[0]);
+    B.tri = (int *)get(sizeof(int) * 3 * (size_t)tot[1]);
+    if (!B.pts || !B.nrm || !B.tri) { release(); err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
+    launch_contour_emit(stream, g, B);
+    pts.resize(3 * (size_t)tot[0]);
+    nrm.resize(3 * (size_t)tot[0]);
+    tri.resize(3 * (size_t)tot[1]);
+    hipMemcpyAsync(pts.data(), B.pts, sizeof(float) * pts.size(), hipMemcpyDeviceToHost, stream);
+    hipMemcpyAsync(nrm.data(), B.nrm, sizeof(float) * nrm.size(), hipMemcpyDeviceToHost, stream);
+    hipMemcpyAsync(tri.data(), B.tri, sizeof(int) * tri.size(), hipMemcpyDeviceToHost, stream);
+    he = hipStreamSynchronize(stream);
+    release();
+    HCHK(he);
+    return 0;
+}
+
+// the ctr file: the object count, then per object the vertex and triangle
+// counts and, when there are vertices, positions, normals (floats) and the
+// triangles' vertex indices (fluid_3d_io.cc:595-848)
+int Sim::write_contours(const char *filename) {
+    FILE *fh = fopen(filename, "wb");
+    if (!fh) { err = std::string("cannot open ") + filename; return RMT3D_ERR_ARG; }
+    int no = n_obj;
+    fwrite(&no, sizeof(int), 1, fh);
+    for (int i = 0; i < n_obj; i++) {
+        std::vector<float> pts, nrm;
+        std::vector<int> tri;
+        int e = contour(i + 1, pts, nrm, tri);
+        if (e) { fclose(fh); return e; }
+        int od[2] = {(int)(pts.size() / 3), (int)(tri.size() / 3)};
+        fwrite(od, sizeof(int), 2, fh);
+        if (od[0] == 0) continue;
+        fwrite(pts.data(), sizeof(float), pts.size(), fh);
+        fwrite(nrm.data(), sizeof(float), nrm.size(), fh);
+        fwrite(tri.data(), sizeof(int), tri.size(), fh);
+    }
+    return fclose(fh) == 0 ? 0 : RMT3D_ERR_ARG;
+}
+
 int Sim::write_checkpoint(int snum, int fnum, const char *dir) {
     size_t L = (size_t)(g.m + 1) * (g.n + 1), n8 = 8 * (size_t)g.o * L;
     std::vector<double> img(n8);

@@ -166,6 +166,9 @@ class Sim {
     int init_tracers();
     int tracer_positions(std::vector<float> &xyz);
     int write_tracers(const char *filename, int binary);
+    // contours (fluid_3d::output_contours, fluid_3d_io.cc:595-958)
+    int contour(int cnum, std::vector<float> &pts, std::vector<float> &nrm, std::vector<int> &tri);
+    int write_contours(const char *filename);
     int macro_data(rmt3d_macro *m);
     int centroids(double *out);
     int slice_dims(int dim, int t, int &d1, int &d2) const;

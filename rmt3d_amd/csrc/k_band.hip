@@ -486,6 +486,15 @@ __global__ void k_scan_add(int *out, const int *bsum, int n) {
     if (t < n) out[t] += bsum[t / SCAN_BS];
 }
 
+// exclusive scan of n ints (n <= SCAN_BS^2 * 64); bsum holds n/SCAN_BS + 1
+// ints of scratch, *total the sum
+void launch_exclusive_scan(hipStream_t s, const int *in, int *out, int *bsum, int n, int *total) {
+    int nb = (n + SCAN_BS - 1) / SCAN_BS;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BS), 0, s, in, out, bsum, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, s, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256), dim3(256), 0, s, out, (const int *)bsum, n);
+}
+
 __global__ void k_place(Geo g, Band B, Scratch W) {
     int n = *W.st_n;
     if (n > W.st_cap) n = W.st_cap;

@@ -194,6 +194,24 @@ void launch_centroids(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
                       int nobj, double *scratch, double *out, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 size_t diag_scratch_doubles(int nobj);
 
+// exclusive scan (k_band.hip): bsum needs n / 1024 + 1 ints
+void launch_exclusive_scan(hipStream_t s, const int *in, int *out, int *bsum, int n, int *total);
+
+// ---- k_contour.hip ----
+// one object's zero contour (fluid_3d::send_contour_data / output_contour,
+// fluid_3d_io.cc:595-945): phi per padded cell, then per node of the
+// (m+1)(n+1)(o+1) lattice of cell centres its vertex and triangle counts
+struct CtBufs {
+    double *phi;                  // Ng
+    int *vcnt, *voff, *tcnt, *toff, *vidx, *bsum, *tot;
+    float *pts, *nrm;             // 3 per vertex
+    int *tri;                     // 3 per triangle
+    const double *gx, *gy, *gz;   // cell-centre coordinates, m+1 / n+1 / o+1 entries
+};
+void launch_contour_count(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Maps &M, int cnum,
+                          const CtBufs &B);
+void launch_contour_emit(hipStream_t s, const Geo &g, const CtBufs &B);
+
 // ---- k_tracers.hip ----
 // tracer constants: cell size, the padded grid's lowest cell centre (low),
 // the box (lo, hi), padded node counts, periodicity and the boundary type

@@ -5,7 +5,7 @@
 // (dt clamp, frame step count, init_iter warm-up), then step frame by frame.
 // After each frame (and once before the first) it writes what dump_code
 // asks for: tracers (bit 1, tr.<frame> binary and tr_st.<frame> text),
-// slices (bit 4, one per out_flag bit, write_params naming
+// contours (bit 2, ctr.<frame>), slices (bit 4, one per out_flag bit, write_params naming
 // "<dirname>/<field>.<frame>"), checkpoints (bit 8, every chkpt_freq
 // frames, "<dirname>/chk.<frame>") and the macro.dat / centroid.dat records
 // (bit 32).  Usage:
@@ -27,7 +27,7 @@ static const char *const FIELD_NAMES[22] = {"vx", "vy", "vz", "wx",  "wy",     "
                                             "phi", "X", "Y",  "Z",   "OID",    "LID",  "pr_phi", "pr_X",
                                             "pr_Y", "pr_Z", "div_u", "div_eu", "rho", "J"};
 
-// stepper::write_files without contours (stepper.hh:580-610)
+// stepper::write_files (stepper.hh:580-610)
 static int write_files(rmt3d_ctx *ctx, const rmt3d_config &cfg, int snum, int fnum) {
     char buf[1024];
     if ((cfg.dump_code & 1) && cfg.ntracers > 0) {
@@ -36,6 +36,11 @@ static int write_files(rmt3d_ctx *ctx, const rmt3d_config &cfg, int snum, int fn
         if (rc) return rc;
         snprintf(buf, sizeof buf, "%s/tr_st.%05d", cfg.dirname, fnum);
         if ((rc = rmt3d_write_tracers(ctx, buf, 0))) return rc;
+    }
+    if (cfg.dump_code & 2) {
+        snprintf(buf, sizeof buf, "%s/ctr.%05d", cfg.dirname, fnum);
+        int rc = rmt3d_write_contours(ctx, buf);
+        if (rc) return rc;
     }
     if (cfg.dump_code & 4) {
         for (int t = 0; t < 22; t++) {
@@ -175,7 +180,7 @@ int main(int argc, char **argv) {
     printf("# %s: %dx%dx%d, %d object(s), %s, dt %.6g, %d steps per frame, %d step(s) to run\n", argv[1],
            cfg.sys_size[0], cfg.sys_size[1], cfg.sys_size[2], cfg.n_obj,
            precision == RMT3D_PREC_MIXED ? "mixed" : "fp64", dt, steps, total);
-    if (cfg.dump_code & (1 | 4 | 8 | 32)) mkdir(cfg.dirname, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+    if (cfg.dump_code & (1 | 2 | 4 | 8 | 32)) mkdir(cfg.dirname, S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
     // the first macro record is taken with the frame counter still at 0
     // (stepper.hh:216, 341-344)
     if ((rc = write_files(ctx, cfg, init_step, init_frame)) ||

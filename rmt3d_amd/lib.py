@@ -25,7 +25,7 @@ EXPORTS = [
     "rmt3d_server_info", "rmt3d_server_error", "rmt3d_server_result", "rmt3d_server_release",
     "rmt3d_server_stats", "rmt3d_server_destroy", "rmt3d_write_checkpoint", "rmt3d_write_params",
     "rmt3d_slice", "rmt3d_write_slice", "rmt3d_macro_data", "rmt3d_centroids", "rmt3d_write_tracers",
-    "rmt3d_tracer_positions",
+    "rmt3d_tracer_positions", "rmt3d_write_contours", "rmt3d_contour",
 ]
 
 
@@ -85,6 +85,9 @@ def lib():
         L.rmt3d_server_destroy.restype = None
         L.rmt3d_macro_data.argtypes = [P, C.POINTER(_abi.Macro)]
         L.rmt3d_write_tracers.argtypes = [P, C.c_char_p, C.c_int]
+        L.rmt3d_write_contours.argtypes = [P, C.c_char_p]
+        fp = C.POINTER(C.c_float)
+        L.rmt3d_contour.argtypes = [P, C.c_int, fp, fp, ip, ip, ip]
         L.rmt3d_tracer_positions.argtypes = [P, C.POINTER(C.c_float), ip]
         L.rmt3d_centroids.argtypes = [P, dp]
         L.rmt3d_write_checkpoint.argtypes = [P, C.c_int, C.c_int, C.c_char_p]
@@ -271,6 +274,22 @@ class Simulation:
         out = np.zeros(6 * k)
         self._check(lib().rmt3d_centroids(self.h, _dp(out)), "centroids")
         return out.reshape(k, 6)[:self.cfg.n_obj]
+
+    # ---- contours ----
+    def contour(self, obj):
+        """Object obj's (1-based) contour: (pts, normals, triangles)."""
+        nv, nt = C.c_int(0), C.c_int(0)
+        self._check(lib().rmt3d_contour(self.h, obj, None, None, C.byref(nv), None, C.byref(nt)), "contour")
+        pts = np.zeros((max(nv.value, 1), 3), dtype=np.float32)
+        nrm = np.zeros_like(pts)
+        tri = np.zeros((max(nt.value, 1), 3), dtype=np.int32)
+        fp = C.POINTER(C.c_float)
+        self._check(lib().rmt3d_contour(self.h, obj, pts.ctypes.data_as(fp), nrm.ctypes.data_as(fp), C.byref(nv),
+                                        tri.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nt)), "contour")
+        return pts[:nv.value], nrm[:nv.value], tri[:nt.value]
+
+    def write_contours(self, filename):
+        self._check(lib().rmt3d_write_contours(self.h, os.fsencode(filename)), "write_contours")
 
     # ---- tracers ----
     def tracer_positions(self):
