@@ -223,7 +223,8 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * multigrid layout (no counterpart in the reference): 10/11 -> MAC/FEM
  * level count, 12/13 -> first level run by the one-workgroup tail kernel,
  * 14/15 -> tail levels held in LDS (bit 0 FP64 hierarchy, bit 1 FP32),
- * 16/17 -> colour passes per smoothing launch on the finest level. */
+ * 16/17 -> colour passes per smoothing launch on the finest level,
+ * 18/19 -> 1 when the coarsest level is solved by fast diagonalisation. */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
 /* Tracers (Utilities/tracers.{hh,cc}): with dump_code & 1 and ntracers > 0

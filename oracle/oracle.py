@@ -123,6 +123,10 @@ def lib():
         L.oracle_sym_pinv.argtypes = [C.c_int, dp, dp]
         L.oracle_mg_direct.argtypes = [P, C.c_int]
         L.oracle_mg_direct.restype = C.c_int
+        L.oracle_mg_fdm.argtypes = [P, C.c_int]
+        L.oracle_mg_fdm.restype = C.c_int
+        L.oracle_level_apply.argtypes = [C.c_int, ip, ip, dp, dp]
+        L.oracle_fdm_solve.argtypes = [C.c_int, ip, ip, dp, dp]
         L.oracle_threads.restype = C.c_int
         L.oracle_heaviside.argtypes = [P, D]
         L.oracle_heaviside.restype = D
@@ -161,6 +165,27 @@ def sym_pinv(A):
     P = np.zeros((n, n))
     lib().oracle_sym_pinv(n, _dp(A), _dp(P))
     return P
+
+
+def _i3(v):
+    return (C.c_int * 3)(*[int(a) for a in v])
+
+
+def level_apply(fem, d, prd, x):
+    """y = A x for the MAC (fem 0) or FEM (fem 1) operator of one level of
+    extents d (unknowns per dimension, x fastest)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    y = np.zeros_like(x)
+    lib().oracle_level_apply(int(fem), _i3(d), _i3(prd), _dp(x), _dp(y))
+    return y
+
+
+def fdm_solve(fem, d, prd, b):
+    """The fast-diagonalisation solve of A x = b on one level."""
+    b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+    x = np.zeros_like(b)
+    lib().oracle_fdm_solve(int(fem), _i3(d), _i3(prd), _dp(b), _dp(x))
+    return x
 
 
 def chk_read(dirname, dims):
@@ -319,6 +344,9 @@ class OracleSim:
 
     def mg_direct(self, which):
         return bool(lib().oracle_mg_direct(self.h, which))
+
+    def mg_fdm(self, which):
+        return bool(lib().oracle_mg_fdm(self.h, which))
 
     # ---- checkpoint and slice formats ----
     def set_band(self, cell, c, x, xpred):

@@ -82,6 +82,12 @@ typedef struct {
 /* a level with at most this many unknowns ends the hierarchy and is solved
  * exactly (pseudo-inverse) */
 #define ORC_MG_DIRECT_NN 160
+/* The hierarchy instead ends at the first coarse level (below the finest)
+ * with more than ORC_MG_SMALL_NN and at most ORC_MG_FDM_NN unknowns, solved
+ * exactly by fast diagonalisation */
+#define ORC_MG_SMALL_NN 6144
+#define ORC_MG_FDM_NN 300000
+#define ORC_MG_FDM_DMAX 512      /* and at most this many per dimension */
 
 /* Moore-Penrose pseudo-inverse of a symmetric n x n matrix (row-major) */
 void orc_sym_pinv(int n, const double *A, double *P);
@@ -105,6 +111,12 @@ typedef struct {
     /* pseudo-inverse of the coarsest operator (nn x nn, symmetric) when that
      * level has at most ORC_MG_DIRECT_NN unknowns, else NULL */
     double *pinv;
+    /* fast-diagonalisation solve of the coarsest level (fdm = 1): per
+     * dimension a, the 1D eigenbasis S[a] (d x d, S[q*d + p] = mode p at
+     * point q) and its transpose ST[a]; dinv = 1 / eigenvalue of each
+     * tensor-product mode (0 for the null mode) */
+    int fdm;
+    double *fS[3], *fST[3], *fdinv;
     /* mg_pred (mg_probs.hh:37-78) */
     int iP, iV;
     double V_avg, V_var, alpha;

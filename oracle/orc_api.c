@@ -203,6 +203,20 @@ int oracle_lsq(int npt, const double *pos, const double *val, const double *w, d
 
 void oracle_mg_set_coarse_sweeps(orc_sim *S, int n) { S->mac.coarse_sweeps = n; S->fem.coarse_sweeps = n; }
 void oracle_sym_pinv(int n, const double *A, double *P) { orc_sym_pinv(n, A, P); }
+void orc_level_apply(int fem, const int d[3], const int prd[3], const double *x, double *y);
+void orc_fdm_solve_level(int fem, const int d[3], const int prd[3], const double *b, double *x);
+/* y = A x and the fast-diagonalisation solve on one level of given extents */
+void oracle_level_apply(int fem, const int *d, const int *prd, const double *x, double *y)
+{
+    orc_level_apply(fem, d, prd, x, y);
+}
+void oracle_fdm_solve(int fem, const int *d, const int *prd, const double *b, double *x)
+{
+    orc_fdm_solve_level(fem, d, prd, b, x);
+}
+/* 1 when the coarsest level of problem `which` is solved by fast
+ * diagonalisation */
+int oracle_mg_fdm(orc_sim *S, int which) { return (which ? S->fem.fdm : S->mac.fdm); }
 /* 1 when the coarsest level of problem `which` (0 MAC, 1 FEM) is solved
  * exactly by its pseudo-inverse */
 int oracle_mg_direct(orc_sim *S, int which) { return (which ? S->fem.pinv : S->mac.pinv) != NULL; }

@@ -29,14 +29,28 @@
  * hierarchy ends at the first level with at most ORC_MG_DIRECT_NN unknowns,
  * solved exactly: x = A^+ b with the Moore-Penrose pseudo-inverse (the
  * minimum-norm least-squares solution, so the singular all-Neumann and
- * periodic problems need no special case).  A hierarchy that cannot coarsen
- * that far ends with 4 Gauss-Seidel sweeps on its coarsest level.
+ * periodic problems need no special case).  A hierarchy whose first coarse
+ * level (or a later one) has more than ORC_MG_SMALL_NN and at most
+ * ORC_MG_FDM_NN unknowns ends there instead, solved exactly by fast
+ * diagonalisation (Lynch, Rice & Thomas, Numer. Math. 6, 1964): both
+ * operators are sums of Kronecker products of 1D matrices -- MAC
+ * T x I x I + I x T x I + I x I x T with T the 1D second difference (Neumann
+ * ends folded), FEM 12 (K x M x M + M x K x M + M x M x K) with K, M the 1D
+ * Q1 stiffness and mass matrices of unit elements (Neumann ends halved) -- so
+ * with per-dimension bases S (S^T M S = I, S^T K S = diag(lam); M = I for
+ * MAC) x = (Sx.Sy.Sz) D^+ (Sx.Sy.Sz)^T b, D = sum of the three lam (times
+ * 12 for FEM), one contraction per dimension each way.  The 1D bases are
+ * closed forms: DCT-I on the Q1 nodes of a wall dimension, DCT-II on the
+ * cells of a MAC wall dimension, the real Fourier basis on a periodic one.
+ * The null mode (all three lam zero) is dropped.  A hierarchy that reaches
+ * neither ends with 4 Gauss-Seidel sweeps on its coarsest level.
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
 #include "orc.h"
+#include "orc_detmath.h"
 
 #define NCOARSE_SWEEPS 4
 /* over-relaxation of the MAC red-black sweeps: 1.25 brings the V-cycles per
@@ -58,6 +72,79 @@ static int coarse_dim(int fem, int d, int prd)
     return (d - 1) / 2 + 1;
 }
 
+/* The 1D basis of one dimension of the fast-diagonalisation level:
+ * S[q*d + p] = mode p at point q, lam[p] its eigenvalue.  Angles are reduced
+ * exactly in integers before det_cos / det_sin.  Modes:
+ *   FEM wall (d nodes, n = d-1 unit elements): cos(pi p q / n),
+ *     lam = 6 (1 - cos(pi p / n)) / (2 + cos(pi p / n))  (K v = lam M v);
+ *   MAC wall (d cells): cos(pi p (2q+1) / 2d), lam = -2 (1 - cos(pi p / d));
+ *   periodic: p = 0 constant, p = 2k-1 cos(2 pi k q / d), p = 2k
+ *     sin(2 pi k q / d), the FEM or MAC lam of the angle 2 pi k / d.
+ * Each mode is scaled to unit M-norm (FEM: M = tridiag(1, 4, 1) / 6 with
+ * wall ends 2/6; MAC: M = I), summed in point order. */
+static void fdm_basis(int fem, int d, int prd, double *S, double *lam)
+{
+    const double pi = 3.14159265358979323846;
+    double *v = (double *)malloc(sizeof(double) * (size_t)d);
+    for (int p = 0; p < d; p++) {
+        double c;
+        if (!prd && fem) {
+            int n = d - 1;
+            for (int q = 0; q < d; q++) v[q] = det_cos(pi * (double)(((long)p * q) % (2L * n)) / n);
+            c = det_cos(pi * p / n);
+        } else if (!prd) {
+            for (int q = 0; q < d; q++) v[q] = det_cos(pi * (double)(((long)p * (2 * q + 1)) % (4L * d)) / (2.0 * d));
+            c = det_cos(pi * p / d);
+        } else {
+            int k = (p + 1) / 2;
+            for (int q = 0; q < d; q++) {
+                double ang = 2.0 * pi * (double)(((long)k * q) % d) / d;
+                v[q] = (p == 0 || (p & 1)) ? det_cos(ang) : det_sin(ang);
+            }
+            c = det_cos(2.0 * pi * k / d);
+        }
+        lam[p] = fem ? 6.0 * (1.0 - c) / (2.0 + c) : -2.0 * (1.0 - c);
+        double nrm = 0;
+        for (int q = 0; q < d; q++) {
+            if (!fem) { nrm += v[q] * v[q]; continue; }
+            int lo = q - 1, hi = q + 1;
+            int end = 0;
+            if (prd) { lo = (lo + d) % d; hi = hi % d; }
+            else { end = (q == 0 || q == d - 1); }
+            double mv = (end ? 2.0 : 4.0) * v[q];
+            if (lo >= 0) mv += v[lo];
+            if (hi < d) mv += v[hi];
+            nrm += v[q] * (mv / 6.0);
+        }
+        double sq = sqrt(nrm);
+        for (int q = 0; q < d; q++) S[(size_t)q * d + p] = v[q] / sq;
+    }
+    free(v);
+}
+
+static void fdm_setup(orc_mg *G, const orc_mglev *L)
+{
+    double *lam[3];
+    for (int a = 0; a < 3; a++) {
+        int d = L->d[a];
+        G->fS[a] = (double *)malloc(sizeof(double) * (size_t)d * d);
+        G->fST[a] = (double *)malloc(sizeof(double) * (size_t)d * d);
+        lam[a] = (double *)malloc(sizeof(double) * (size_t)d);
+        fdm_basis(G->fem, d, G->prd[a], G->fS[a], lam[a]);
+        for (int q = 0; q < d; q++)
+            for (int p = 0; p < d; p++) G->fST[a][(size_t)p * d + q] = G->fS[a][(size_t)q * d + p];
+    }
+    G->fdinv = (double *)malloc(sizeof(double) * (size_t)L->nn);
+    double scale = G->fem ? 12.0 : 1.0;
+    for (int k = 0; k < L->d[2]; k++)
+        for (int j = 0; j < L->d[1]; j++)
+            for (int i = 0; i < L->d[0]; i++) {
+                double D = scale * ((lam[0][i] + lam[1][j]) + lam[2][k]);
+                G->fdinv[i + L->d[0] * (j + L->d[1] * k)] = (D == 0) ? 0.0 : 1.0 / D;
+            }
+    for (int a = 0; a < 3; a++) free(lam[a]);
+}
+
 void orc_mg_setup(orc_mg *G, int fem, const int d[3], const int prd[3])
 {
     memset(G, 0, sizeof *G);
@@ -74,6 +161,11 @@ void orc_mg_setup(orc_mg *G, int fem, const int d[3], const int prd[3])
         L->r = (double *)calloc((size_t)L->nn, sizeof(double));
         l++;
         if (L->nn <= ORC_MG_DIRECT_NN) break;
+        if (l > 1 && L->nn > ORC_MG_SMALL_NN && L->nn <= ORC_MG_FDM_NN && cur[0] <= ORC_MG_FDM_DMAX &&
+            cur[1] <= ORC_MG_FDM_DMAX && cur[2] <= ORC_MG_FDM_DMAX) {
+            G->fdm = 1;
+            break;
+        }
         int ok = l < 16;
         for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
         if (!ok) break;
@@ -81,7 +173,9 @@ void orc_mg_setup(orc_mg *G, int fem, const int d[3], const int prd[3])
     }
     G->nlev = l;
     G->pinv = NULL;
-    if (G->lev[l - 1].nn <= ORC_MG_DIRECT_NN) {
+    if (G->fdm) {
+        fdm_setup(G, &G->lev[l - 1]);
+    } else if (G->lev[l - 1].nn <= ORC_MG_DIRECT_NN) {
         /* the coarsest operator column by column, then its pseudo-inverse */
         int n = G->lev[l - 1].nn;
         double *A = (double *)calloc((size_t)n * n, sizeof(double));
@@ -119,6 +213,13 @@ void orc_mg_free(orc_mg *G)
     }
     free(G->pinv);
     G->pinv = NULL;
+    for (int a = 0; a < 3; a++) {
+        free(G->fS[a]); free(G->fST[a]);
+        G->fS[a] = G->fST[a] = NULL;
+    }
+    free(G->fdinv);
+    G->fdinv = NULL;
+    G->fdm = 0;
     G->nlev = 0;
 }
 
@@ -411,11 +512,101 @@ static void prolong_add(orc_mg *G, int lev)
     }
 }
 
+/* out[.., p, ..] = sum over q of W[q*d + p] in[.., q, ..] along dimension a,
+ * each output summed over ascending q from +0, times dinv[.] when given.
+ * Along y and z the loops run over whole x-lines (or xy-planes) per term,
+ * which keeps each output's order of operations. */
+static void fdm_contract(const orc_mglev *L, int a, const double *W, const double *in, double *out,
+                         const double *dinv)
+{
+    int d0 = L->d[0], d1 = L->d[1], d2 = L->d[2], da = L->d[a];
+    if (a == 0) {
+#pragma omp parallel for schedule(static)
+        for (int row = 0; row < d1 * d2; row++) {
+            const double *src = in + (size_t)row * d0;
+            double *dst = out + (size_t)row * d0;
+            for (int p = 0; p < d0; p++) {
+                double s = 0;
+                for (int q = 0; q < d0; q++) s += W[(size_t)q * d0 + p] * src[q];
+                if (dinv) s *= dinv[(size_t)row * d0 + p];
+                dst[p] = s;
+            }
+        }
+        return;
+    }
+    int inner = a == 1 ? d0 : d0 * d1, nouter = a == 1 ? d2 : 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int o = 0; o < nouter; o++)
+        for (int p = 0; p < da; p++) {
+            const double *src = in + (size_t)o * da * inner;
+            size_t off = ((size_t)o * da + p) * inner;
+            double *dst = out + off;
+            for (int ii = 0; ii < inner; ii++) dst[ii] = 0;
+            for (int q = 0; q < da; q++) {
+                double w = W[(size_t)q * da + p];
+                const double *sq = src + (size_t)q * inner;
+                for (int ii = 0; ii < inner; ii++) dst[ii] += w * sq[ii];
+            }
+            if (dinv)
+                for (int ii = 0; ii < inner; ii++) dst[ii] *= dinv[off + ii];
+        }
+}
+
+/* x = S D^+ S^T b: forward along x, y, z (the last scaled by D^+), back
+ * along z, y, x, through the level's r and x arrays */
+static void fdm_solve(orc_mg *G, orc_mglev *L)
+{
+    fdm_contract(L, 0, G->fS[0], L->b, L->r, NULL);
+    fdm_contract(L, 1, G->fS[1], L->r, L->x, NULL);
+    fdm_contract(L, 2, G->fS[2], L->x, L->r, G->fdinv);
+    fdm_contract(L, 2, G->fST[2], L->r, L->x, NULL);
+    fdm_contract(L, 1, G->fST[1], L->x, L->r, NULL);
+    fdm_contract(L, 0, G->fST[0], L->r, L->x, NULL);
+}
+
+/* Test hooks on a single d[0] x d[1] x d[2] level: y = A x, and the
+ * fast-diagonalisation solve x of A x = b. */
+static void one_level(orc_mg *G, int fem, const int d[3], const int prd[3])
+{
+    memset(G, 0, sizeof *G);
+    G->fem = fem;
+    G->nlev = 1;
+    orc_mglev *L = &G->lev[0];
+    for (int a = 0; a < 3; a++) { G->prd[a] = prd[a]; L->d[a] = d[a]; }
+    L->nn = d[0] * d[1] * d[2];
+    L->x = (double *)calloc((size_t)L->nn, sizeof(double));
+    L->b = (double *)calloc((size_t)L->nn, sizeof(double));
+    L->r = (double *)calloc((size_t)L->nn, sizeof(double));
+}
+
+void orc_level_apply(int fem, const int d[3], const int prd[3], const double *x, double *y)
+{
+    orc_mg G;
+    one_level(&G, fem, d, prd);
+    orc_mg_apply(&G, 0, x, y);
+    orc_mg_free(&G);
+}
+
+void orc_fdm_solve_level(int fem, const int d[3], const int prd[3], const double *b, double *x)
+{
+    orc_mg G;
+    one_level(&G, fem, d, prd);
+    orc_mglev *L = &G.lev[0];
+    fdm_setup(&G, L);
+    G.fdm = 1;
+    memcpy(L->b, b, sizeof(double) * (size_t)L->nn);
+    fdm_solve(&G, L);
+    memcpy(x, L->x, sizeof(double) * (size_t)L->nn);
+    orc_mg_free(&G);
+}
+
 static void vcycle_rec(orc_mg *G, int lev)
 {
     if (lev == G->nlev - 1) {
         orc_mglev *L = &G->lev[lev];
-        if (G->pinv) {
+        if (G->fdm) {
+            fdm_solve(G, L);
+        } else if (G->pinv) {
             /* x = A^+ b, each row summed in column order */
             int n = L->nn;
             for (int r = 0; r < n; r++) {

@@ -172,6 +172,8 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out) {
         case 15: *out = S.fem.tail_lds64 + 2 * S.fem.tail_lds32; return 0;
         case 16: *out = S.mac.npass[0]; return 0;
         case 17: *out = S.fem.npass[0]; return 0;
+        case 18: *out = S.mac.fdm; return 0;
+        case 19: *out = S.fem.fdm; return 0;
         }
     } catch (...) {
         return RMT3D_ERR_FATAL;

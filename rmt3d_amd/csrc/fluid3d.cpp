@@ -148,6 +148,23 @@ static void sym_pinv(int n, const double *A0, double *P) {
         }
 }
 
+// words of fast-diagonalisation operands the hierarchy of a d[0] x d[1] x
+// d[2] problem allocates: the coarsening rule of MGSolver::setup
+static size_t mg_fdm_words(int fem, const int d[3], const int prd[3]) {
+    int cur[3] = {d[0], d[1], d[2]};
+    for (int lev = 0; lev < 16; lev++) {
+        size_t nn = (size_t)cur[0] * cur[1] * cur[2];
+        if (nn <= (size_t)MG_DIRECT_NN) return 0;
+        if (lev > 0 && nn > (size_t)MG_TAIL_NN && nn <= (size_t)MG_FDM_NN && cur[0] <= MG_FDM_DMAX &&
+            cur[1] <= MG_FDM_DMAX && cur[2] <= MG_FDM_DMAX)
+            return nn + 2 * ((size_t)cur[0] * cur[0] + (size_t)cur[1] * cur[1] + (size_t)cur[2] * cur[2]);
+        for (int a = 0; a < 3; a++)
+            if (!can_coarsen(fem, cur[a], prd[a])) return 0;
+        for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
+    }
+    return 0;
+}
+
 void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     fem = fem_;
     mixed = mixed_;
@@ -186,12 +203,63 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         nlev++;
         if (!ok_alloc) { alloc_failed = true; break; }
         if (l.nn <= MG_DIRECT_NN) break;
+        if (nlev > 1 && l.nn > MG_TAIL_NN && l.nn <= MG_FDM_NN && cur[0] <= MG_FDM_DMAX && cur[1] <= MG_FDM_DMAX &&
+            cur[2] <= MG_FDM_DMAX) {
+            fdm = true;
+            break;
+        }
         bool ok = nlev < 16;
         for (int a = 0; a < 3; a++) ok = ok && can_coarsen(fem, cur[a], prd[a]);
         if (!ok) break;
         for (int a = 0; a < 3; a++) cur[a] = coarse_dim(fem, cur[a], prd[a]);
     }
-    if (!alloc_failed && L[nlev - 1].nn <= MG_DIRECT_NN) {
+    if (!alloc_failed && fdm) {
+        // bases S[a] and transposes ST[a] of the three dimensions, then
+        // 1 / eigenvalue per mode, packed in one array
+        const MGLevel<double> &c = L[nlev - 1];
+        size_t off[7], tot = 0;
+        for (int a = 0; a < 3; a++) {
+            off[a] = tot; tot += (size_t)c.d[a] * c.d[a];
+            off[3 + a] = tot; tot += (size_t)c.d[a] * c.d[a];
+        }
+        off[6] = tot; tot += c.nn;
+        std::vector<double> h(tot);
+        std::vector<double> lam[3];
+        for (int a = 0; a < 3; a++) {
+            int d = c.d[a];
+            lam[a].resize(d);
+            double *S = h.data() + off[a], *ST = h.data() + off[3 + a];
+            mg_fdm_basis(fem, d, prd[a], S, lam[a].data());
+            for (int q = 0; q < d; q++)
+                for (int p = 0; p < d; p++) ST[(size_t)p * d + q] = S[(size_t)q * d + p];
+        }
+        double scale = fem ? 12.0 : 1.0;
+        double *dinv = h.data() + off[6];
+        for (int k = 0; k < c.d[2]; k++)
+            for (int j = 0; j < c.d[1]; j++)
+                for (int i = 0; i < c.d[0]; i++) {
+                    double D = scale * ((lam[0][i] + lam[1][j]) + lam[2][k]);
+                    dinv[i + c.d[0] * (j + c.d[1] * k)] = (D == 0) ? 0.0 : 1.0 / D;
+                }
+        bool ok = hipMalloc(&d_fdm, sizeof(double) * tot) == hipSuccess &&
+                  hipMemcpy(d_fdm, h.data(), sizeof(double) * tot, hipMemcpyHostToDevice) == hipSuccess;
+        bytes += sizeof(double) * tot;
+        if (ok) {
+            for (int a = 0; a < 3; a++) { fdm64.S[a] = d_fdm + off[a]; fdm64.ST[a] = d_fdm + off[3 + a]; }
+            fdm64.dinv = d_fdm + off[6];
+        }
+        if (ok && mixed) {
+            std::vector<float> hf(h.begin(), h.end());
+            ok = hipMalloc(&f_fdm, sizeof(float) * tot) == hipSuccess &&
+                 hipMemcpy(f_fdm, hf.data(), sizeof(float) * tot, hipMemcpyHostToDevice) == hipSuccess;
+            bytes += sizeof(float) * tot;
+            if (ok) {
+                for (int a = 0; a < 3; a++) { fdm32.S[a] = f_fdm + off[a]; fdm32.ST[a] = f_fdm + off[3 + a]; }
+                fdm32.dinv = f_fdm + off[6];
+            }
+        }
+        if (!ok) alloc_failed = true;
+    } else if (!alloc_failed && L[nlev - 1].nn <= MG_DIRECT_NN) {
         const MGLevel<double> &c = L[nlev - 1];
         int n = c.nn;
         std::vector<double> A((size_t)n * n), Pm((size_t)n * n);
@@ -210,8 +278,8 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     }
     // the tail starts at the first level small enough for one workgroup,
     // with all its levels in LDS when they fit
-    tail_lev = nlev - 1;
-    while (tail_lev > 0 && L[tail_lev - 1].nn <= MG_TAIL_NN && nlev - (tail_lev - 1) <= MG_TAIL_MAXLEV &&
+    tail_lev = fdm ? nlev : nlev - 1;
+    while (!fdm && tail_lev > 0 && L[tail_lev - 1].nn <= MG_TAIL_NN && nlev - (tail_lev - 1) <= MG_TAIL_MAXLEV &&
            mg_tail_lds_bytes(L + tail_lev - 1, nlev - tail_lev + 1) <= MG_TAIL_LDS_MAX)
         tail_lev--;
     // RMT3D_NO_GRAPH=1 launches the V-cycles one kernel at a time (for
@@ -233,8 +301,10 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         }
     }
     mg_sweep_enable();
-    tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
-    tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
+    if (!fdm) {
+        tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
+        tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
+    }
     double eps = DBL_EPSILON, c = fem ? 32.0 : 6.0;
     tol = c * c * eps * eps * 1e4;
     max_norm_tol = c * eps * 1e2;
@@ -261,6 +331,10 @@ void MGSolver::release() {
     if (d_pinv) hipFree(d_pinv);
     if (f_pinv) hipFree(f_pinv);
     d_pinv = nullptr; f_pinv = nullptr;
+    if (d_fdm) hipFree(d_fdm);
+    if (f_fdm) hipFree(f_fdm);
+    d_fdm = nullptr; f_fdm = nullptr;
+    fdm = false;
     if (d_max) hipFree(d_max);
     if (h_max) hipHostFree(h_max);
     if (ev_check) hipEventDestroy(ev_check);
@@ -324,6 +398,10 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
 }
 
 void MGSolver::vcycle64(hipStream_t s, int lev) {
+    if (fdm && lev == nlev - 1) {
+        launch_mg_fdm(s, L[lev], fdm64);
+        return;
+    }
     if (lev >= tail_lev) {
         launch_mg_tail(s, fem, L + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds64, (const double *)d_pinv);
         return;
@@ -338,6 +416,10 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
 }
 
 void MGSolver::vcycle32(hipStream_t s, int lev) {
+    if (fdm && lev == nlev - 1) {
+        launch_mg_fdm(s, Lf[lev], fdm32);
+        return;
+    }
     if (lev >= tail_lev) {
         launch_mg_tail(s, fem, Lf + lev, nlev - lev, prd, v_down, v_up, coarse_sweeps, tail_lds32, (const float *)f_pinv);
         return;
@@ -476,6 +558,13 @@ double Sim::estimate_bytes(const rmt3d_config &c, const rmt3d_object *o, int pre
     double mg_words = (m * n * k + fm * fn * fo) * 8.0 / 7.0;
     double mg = 3 * 8 * mg_words + (prec == RMT3D_PREC_MIXED ? 3 * 4 * mg_words : 0);
     double diag = 8.0 * (7 * 256 * std::max(1, n_obj) + 4096);
+    // fast-diagonalisation operands (bases, transposes, 1/eigenvalues) of
+    // the two hierarchies, where they end at such a level
+    {
+        int dm[3] = {(int)m, (int)n, (int)k}, df[3] = {(int)fm, (int)fn, (int)fo};
+        diag += (double)mg_fdm_words(0, dm, c.prd) * (prec == RMT3D_PREC_MIXED ? 12 : 8);
+        diag += (double)mg_fdm_words(1, df, c.prd) * (prec == RMT3D_PREC_MIXED ? 12 : 8);
+    }
     return Ng * per_cell + cap * per_entry + mg + 16.0 * fm * fn * fo + diag + (1 << 20);
 }
 

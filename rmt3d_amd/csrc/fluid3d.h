@@ -61,6 +61,11 @@ struct MGSolver {
     double last_res = 0;
     KProf *prof = nullptr;     // times the finest-level smoother launches
     KProf *vprof = nullptr;    // times whole V-cycles (one defect-correction step when mixed)
+    bool fdm = false;                      // coarsest level solved by fast diagonalisation
+    double *d_fdm = nullptr;               // its bases, transposes and 1/eigenvalues (FP64)
+    float *f_fdm = nullptr;                // their FP32 copy (mixed)
+    FdmDev<double> fdm64{};
+    FdmDev<float> fdm32{};
     double *d_pinv = nullptr;              // pseudo-inverse of the coarsest operator (or null)
     float *f_pinv = nullptr;               // its FP32 copy (mixed)
     unsigned long long *d_max = nullptr;

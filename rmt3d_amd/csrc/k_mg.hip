@@ -12,6 +12,8 @@
 // neighbours, so the update order is free and the result is deterministic.
 #include <hip/hip_ext.h>
 
+#include <vector>
+
 #include "dev.h"
 #include "launch.h"
 
@@ -1090,7 +1092,252 @@ void launch_zero_f(hipStream_t s, float *a, int n) {
     hipLaunchKernelGGL(k_zero_f, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
 }
 
+// ---------------------------------------------------------------------------
+// Fast-diagonalisation solve of the coarsest level (Lynch, Rice & Thomas,
+// Numer. Math. 6, 1964).  Both operators are sums of Kronecker products of
+// 1D matrices: MAC T.I.I + I.T.I + I.I.T (T the 1D second difference with
+// Neumann ends folded in), FEM 12 (K.M.M + M.K.M + M.M.K) (K, M the 1D Q1
+// stiffness and mass matrices of unit elements, Neumann ends halved).  With
+// per-dimension bases S (S^T M S = I, S^T K S = diag(lam); M = I for MAC),
+// x = (Sx.Sy.Sz) D^+ (Sx.Sy.Sz)^T b, D the sum of the three lam (times 12
+// for FEM): three contractions forward (the last scaled by D^+) and three
+// back.  Each output sums its d terms in ascending order, as the oracle's
+// restatement does, so FP64 results agree bit for bit.
+// ---------------------------------------------------------------------------
+
+// The closed-form bases, angles reduced exactly in integers:
+//   FEM wall (d nodes, n = d-1 elements): cos(pi p q / n),
+//     lam = 6 (1 - cos(pi p / n)) / (2 + cos(pi p / n));
+//   MAC wall (d cells): cos(pi p (2q+1) / 2d), lam = -2 (1 - cos(pi p / d));
+//   periodic: p = 0 constant, p = 2k-1 cos(2 pi k q / d), p = 2k
+//     sin(2 pi k q / d), with the lam of the angle 2 pi k / d.
+// Each mode is scaled to unit M-norm (FEM M = tridiag(1, 4, 1) / 6, wall
+// ends 2/6; MAC M = I), summed in point order.
+void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam) {
+    const double pi = 3.14159265358979323846;
+    std::vector<double> v(d);
+    for (int p = 0; p < d; p++) {
+        double c;
+        if (!prd && fem) {
+            int n = d - 1;
+            for (int q = 0; q < d; q++) v[q] = det_cos(pi * (double)(((long)p * q) % (2L * n)) / n);
+            c = det_cos(pi * p / n);
+        } else if (!prd) {
+            for (int q = 0; q < d; q++) v[q] = det_cos(pi * (double)(((long)p * (2 * q + 1)) % (4L * d)) / (2.0 * d));
+            c = det_cos(pi * p / d);
+        } else {
+            int k = (p + 1) / 2;
+            for (int q = 0; q < d; q++) {
+                double ang = 2.0 * pi * (double)(((long)k * q) % d) / d;
+                v[q] = (p == 0 || (p & 1)) ? det_cos(ang) : det_sin(ang);
+            }
+            c = det_cos(2.0 * pi * k / d);
+        }
+        lam[p] = fem ? 6.0 * (1.0 - c) / (2.0 + c) : -2.0 * (1.0 - c);
+        double nrm = 0;
+        for (int q = 0; q < d; q++) {
+            if (!fem) {
+                nrm += v[q] * v[q];
+                continue;
+            }
+            int lo = q - 1, hi = q + 1;
+            int end = 0;
+            if (prd) {
+                lo = (lo + d) % d;
+                hi = hi % d;
+            } else {
+                end = (q == 0 || q == d - 1);
+            }
+            double mv = (end ? 2.0 : 4.0) * v[q];
+            if (lo >= 0) mv += v[lo];
+            if (hi < d) mv += v[hi];
+            nrm += v[q] * (mv / 6.0);
+        }
+        double sq = sqrt(nrm);
+        for (int q = 0; q < d; q++) S[(size_t)q * d + p] = v[q] / sq;
+    }
+}
+
+// out[.., p, ..] = sum over q of W[q*d + p] in[.., q, ..] along one
+// dimension, each output summed over ascending q from +0.  A workgroup is
+// one wavefront of 64 pencils and one block of FDM_PB outputs p
+// (blockIdx.y); it first copies the da x FDM_PB slice of W it needs into
+// LDS (columns past the end repeat column da-1, unused), so the inner loop
+// reads W as LDS broadcasts, and loads pencil values FDM_QU at a time so
+// that their latencies overlap.
+constexpr int FDM_PB = 8;
+constexpr int FDM_QU = 16;
+
+// copies n values src[f(e)] to dst[g(e)] with all of a lane's loads issued
+// before its stores (n <= 64 * FDM_LB per call)
+constexpr int FDM_LB = 32;
+
+template <typename T>
+__device__ __forceinline__ void fdm_stage_w(T *wsl, const T *__restrict__ W, int da, int p0) {
+    const int last = da - 1 - p0, n = da * FDM_PB;
+    for (int e0 = 0; e0 < n; e0 += 64 * FDM_LB) {
+        T v[FDM_LB];
+#pragma unroll
+        for (int u = 0; u < FDM_LB; u++) {
+            int e = min(e0 + u * 64 + (int)threadIdx.x, n - 1);
+            int q = e / FDM_PB, r = e - q * FDM_PB;
+            v[u] = W[q * da + p0 + (r < last ? r : last)];
+        }
+#pragma unroll
+        for (int u = 0; u < FDM_LB; u++) {
+            int e = e0 + u * 64 + threadIdx.x;
+            if (e < n) wsl[e] = v[u];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void fdm_terms(T (&acc)[FDM_PB], const T (&v)[FDM_QU], const T *wsl, int q0, int da) {
+#pragma unroll
+    for (int u = 0; u < FDM_QU; u++) {
+        if (q0 + u < da) {
+            const T *w = wsl + (q0 + u) * FDM_PB;
+#pragma unroll
+            for (int r = 0; r < FDM_PB; r++) acc[r] += w[r] * v[u];
+        }
+    }
+}
+
+// y or z: lanes take consecutive x positions (coalesced loads and stores).
+// Pencil c starts at (c % inner) + (c / inner) * ostride and steps by
+// stride; SCALE multiplies each output by dinv.
+template <typename T, bool SCALE>
+__global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__restrict__ out,
+                                                 const T *__restrict__ W, const T *__restrict__ dinv, int ncols,
+                                                 int inner, int ostride, int da, int stride) {
+    extern __shared__ unsigned char fdm_lds[];
+    T *wsl = reinterpret_cast<T *>(fdm_lds);
+    const int p0 = blockIdx.y * FDM_PB;
+    fdm_stage_w(wsl, W, da, p0);
+    __syncthreads();
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    int co = c / inner;
+    int base = (c - co * inner) + co * ostride;
+    T acc[FDM_PB];
+#pragma unroll
+    for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
+    // the next FDM_QU pencil values load while the current ones are used
+    T v[FDM_QU], vn[FDM_QU];
+#pragma unroll
+    for (int u = 0; u < FDM_QU; u++) {
+        v[u] = in[base + min(u, da - 1) * stride];
+        vn[u] = T(0);
+    }
+    for (int q0 = 0; q0 < da; q0 += FDM_QU) {
+        if (q0 + FDM_QU < da)
+#pragma unroll
+            for (int u = 0; u < FDM_QU; u++) vn[u] = in[base + min(q0 + FDM_QU + u, da - 1) * stride];
+        fdm_terms(acc, v, wsl, q0, da);
+#pragma unroll
+        for (int u = 0; u < FDM_QU; u++) v[u] = vn[u];
+    }
+#pragma unroll
+    for (int r = 0; r < FDM_PB; r++)
+        if (p0 + r < da) {
+            int e = base + (p0 + r) * stride;
+            T o = acc[r];
+            if (SCALE) o *= dinv[e];
+            out[e] = o;
+        }
+}
+
+// x: the workgroup also stages rb (<= 64) consecutive rows in LDS (pitch
+// d0 + 1, so that lanes on different rows read different banks); lane =
+// row, outputs stored straight to the rows.
+template <typename T>
+__global__ void __launch_bounds__(64) k_fdm_rows(const T *__restrict__ in, T *__restrict__ out,
+                                                 const T *__restrict__ W, int d0, int nrows, int rb) {
+    extern __shared__ unsigned char fdm_lds[];
+    T *wsl = reinterpret_cast<T *>(fdm_lds);
+    T *tile = wsl + d0 * FDM_PB;
+    const int ld = d0 + 1;
+    const int p0 = blockIdx.y * FDM_PB;
+    const int r0 = blockIdx.x * rb;
+    const int nr = min(rb, nrows - r0);
+    fdm_stage_w(wsl, W, d0, p0);
+    const T *src = in + (size_t)r0 * d0;
+    const int ne = nr * d0;
+    for (int e0 = 0; e0 < ne; e0 += 64 * FDM_LB) {
+        T v[FDM_LB];
+#pragma unroll
+        for (int u = 0; u < FDM_LB; u++) v[u] = src[min(e0 + u * 64 + (int)threadIdx.x, ne - 1)];
+#pragma unroll
+        for (int u = 0; u < FDM_LB; u++) {
+            int e = e0 + u * 64 + threadIdx.x;
+            if (e < ne) {
+                int r = e / d0;
+                tile[r * ld + (e - r * d0)] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x;
+    const int rl = lane < nr ? lane : nr - 1;     // idle lanes repeat the last row, unstored
+    const T *row = tile + rl * ld;
+    T acc[FDM_PB];
+#pragma unroll
+    for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
+    for (int q0 = 0; q0 < d0; q0 += FDM_QU) {
+        T v[FDM_QU];
+#pragma unroll
+        for (int u = 0; u < FDM_QU; u++) v[u] = row[min(q0 + u, d0 - 1)];
+        fdm_terms(acc, v, wsl, q0, d0);
+    }
+    if (lane < nr) {
+        T *dst = out + (size_t)(r0 + lane) * d0;
+#pragma unroll
+        for (int r = 0; r < FDM_PB; r++)
+            if (p0 + r < d0) dst[p0 + r] = acc[r];
+    }
+}
+
+template <typename T>
+static void fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, const T *W, const T *dinv, const T *in, T *out) {
+    int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
+    int da = L.d[a];
+    size_t wbytes = sizeof(T) * (size_t)da * FDM_PB;
+    dim3 blk(64);
+    if (a == 0) {
+        // 64 rows per workgroup while W's slice and the tile fit in 64 KiB
+        int rb = 64;
+        while (rb > 1 && wbytes + sizeof(T) * (size_t)rb * (d0 + 1) > 65536) rb /= 2;
+        int nrows = d1 * d2;
+        size_t lds = wbytes + sizeof(T) * (size_t)rb * (d0 + 1);
+        hipLaunchKernelGGL(k_fdm_rows<T>, dim3((nrows + rb - 1) / rb, (d0 + FDM_PB - 1) / FDM_PB), blk, lds, s, in,
+                           out, W, d0, nrows, rb);
+        return;
+    }
+    int ncols = L.nn / da;
+    int inner = a == 1 ? d0 : d0 * d1, ostride = a == 1 ? d0 * d1 : 0, stride = a == 1 ? d0 : d0 * d1;
+    dim3 g((ncols + 63) / 64, (da + FDM_PB - 1) / FDM_PB);
+    if (dinv)
+        hipLaunchKernelGGL((k_fdm_cols<T, true>), g, blk, wbytes, s, in, out, W, dinv, ncols, inner, ostride, da,
+                           stride);
+    else
+        hipLaunchKernelGGL((k_fdm_cols<T, false>), g, blk, wbytes, s, in, out, W, (const T *)nullptr, ncols,
+                           inner, ostride, da, stride);
+}
+
+// forward along x, y, z (the last scaled by D^+), back along z, y, x,
+// through the level's r and x arrays (b is left as it is)
+template <typename T>
+void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
+    fdm_pass<T>(s, L, 0, F.S[0], nullptr, L.b, L.r);
+    fdm_pass<T>(s, L, 1, F.S[1], nullptr, L.r, L.x);
+    fdm_pass<T>(s, L, 2, F.S[2], F.dinv, L.x, L.r);
+    fdm_pass<T>(s, L, 2, F.ST[2], nullptr, L.r, L.x);
+    fdm_pass<T>(s, L, 1, F.ST[1], nullptr, L.x, L.r);
+    fdm_pass<T>(s, L, 0, F.ST[0], nullptr, L.r, L.x);
+}
+
 #define INST(T)                                                                                              \
+    template void launch_mg_fdm<T>(hipStream_t, const MGLevel<T> &, const FdmDev<T> &);                     \
     template void launch_mg_smooth<T>(hipStream_t, int, const MGLevel<T> &, const int *, int, hipEvent_t,    \
                                       hipEvent_t);                                                           \
     template void launch_mg_sweep<T>(hipStream_t, int, const MGLevel<T> &, const int *, const SweepPlan &,   \

@@ -63,6 +63,25 @@ struct MGLevel {
 // exactly with the pseudo-inverse of its operator
 constexpr int MG_DIRECT_NN = 160;
 
+// ... unless the hierarchy first reaches a coarse level (below the finest)
+// with more than MG_TAIL_NN and at most MG_FDM_NN unknowns: that level ends
+// it and is solved exactly by fast diagonalisation
+constexpr int MG_FDM_NN = 300000;
+constexpr int MG_FDM_DMAX = 512;      // and at most this many per dimension
+// the fast-diagonalisation operands of one level: per dimension a, the 1D
+// eigenbasis S[a] (S[q*d + p] = mode p at point q) and its transpose ST[a];
+// dinv[e] = 1 / eigenvalue of the tensor-product mode e, 0 for the null mode
+template <typename T>
+struct FdmDev {
+    const T *S[3], *ST[3], *dinv;
+};
+// host: the 1D basis of one dimension (fem 1: Q1 nodes, 0: MAC cells) and
+// its eigenvalues
+void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam);
+// x = S D^+ S^T b on level L (six contractions through L.r and L.x)
+template <typename T>
+void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F);
+
 // the levels below a size threshold, processed by one workgroup
 constexpr int MG_TAIL_MAXLEV = 8;       // a level of <= 6144 unknowns coarsens at most 4 more times
 constexpr int MG_TAIL_NN = 6144;       // largest level handed to the tail kernel

@@ -34,15 +34,7 @@ def test_mg_solve_matches_oracle(which, maker):
     assert rg <= (6 if which == 0 else 32) * 2.220446049250313e-16 * 100
 
 
-@pytest.mark.parametrize("which", [0, 1])
-@pytest.mark.parametrize("dims,prd", [((64, 64, 64), (0, 0, 0)), ((64, 64, 64), (1, 1, 1)),
-                                      ((64, 48, 80), (1, 0, 1)), ((96, 64, 56), (0, 1, 0))])
-@pytest.mark.parametrize("prec", [0, 1])
-def test_mg_solve_fused_levels_match_oracle(which, dims, prd, prec):
-    # levels between the finest and the one-workgroup tail run the fused
-    # multi-colour sweeps (LDS tiles with halos, wrapped on periodic
-    # dimensions); FP64 must match the oracle bit for bit, the mixed path
-    # converges to the same FP64 tolerance
+def _mg_pair_solve(which, dims, prd, prec, seed):
     from rmt3d_amd import _abi
     cfg, objs = configs.taylor_green(8)
     cfg.sys_size[:] = list(dims)
@@ -53,15 +45,15 @@ def test_mg_solve_fused_levels_match_oracle(which, dims, prd, prec):
     g, o = pair(cfg, [], precision=_abi.PREC_MIXED if prec else _abi.PREC_FP64)
     mg_dims = g.mg_dims(which)
     n = int(np.prod(mg_dims))
-    rng = np.random.default_rng(3 + which)
+    rng = np.random.default_rng(seed + which)
     b = rng.standard_normal(n)
     b -= b.mean()
     x0 = np.zeros(n)
     xg, ncg, rg = g.mg_solve(which, b, x0)
     xo, nco, ro = o.mg_solve(which, b, x0)
-    print("MG fused", dims, prd, which, prec, "levels", g.diag(10 + which), "tail", g.diag(12 + which),
-          "cycles", ncg, nco, "res", rg, ro)
-    assert g.diag(12 + which) >= 2          # at least one level runs the fused sweeps
+    print("MG", dims, prd, which, prec, "levels", g.diag(10 + which), "tail", g.diag(12 + which),
+          "fdm", g.diag(18 + which), "cycles", ncg, nco, "res", rg, ro, flush=True)
+    assert g.diag(18 + which) == float(o.mg_fdm(which))
     assert rg <= (6 if which == 0 else 32) * 2.220446049250313e-16 * 100
     d = rel_l2(xg - xg.mean(), xo - xo.mean())
     if prec == 0:
@@ -69,6 +61,31 @@ def test_mg_solve_fused_levels_match_oracle(which, dims, prd, prec):
         assert d == 0.0
     else:
         assert d < 1e-9
+    return g
+
+
+@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("dims,prd", [((64, 64, 64), (0, 0, 0)), ((64, 64, 64), (1, 1, 1)),
+                                      ((64, 48, 80), (1, 0, 1)), ((96, 64, 56), (0, 1, 0))])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mg_solve_fdm_level_matches_oracle(which, dims, prd, prec):
+    # the first coarse level ends the hierarchy and is solved by fast
+    # diagonalisation (walls, periodic, mixed and anisotropic extents); FP64
+    # must match the oracle bit for bit, the mixed path converges to the same
+    # FP64 tolerance
+    g = _mg_pair_solve(which, dims, prd, prec, 3)
+    assert g.diag(18 + which) == 1.0 and g.diag(10 + which) == 2.0
+
+
+@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("dims,prd", [((160, 160, 160), (0, 0, 0)), ((160, 128, 192), (1, 1, 1))])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mg_solve_fused_levels_match_oracle(which, dims, prd, prec):
+    # a level between the finest and the fast-diagonalisation level runs the
+    # fused multi-colour sweeps (LDS tiles with halos, wrapped on periodic
+    # dimensions)
+    g = _mg_pair_solve(which, dims, prd, prec, 5)
+    assert g.diag(18 + which) == 1.0 and g.diag(10 + which) == 3.0
 
 
 def _init_pair(cfg, objs):

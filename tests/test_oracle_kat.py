@@ -416,6 +416,98 @@ def test_hierarchy_ends_at_direct_level(which, dims, prd, nlev):
         assert np.abs(A @ x - b).max() <= 1e-12
 
 
+# ------------------------------------------------- fast-diagonalisation level
+FDM_SHAPES = [((6, 5, 4), (0, 0, 0)), ((8, 4, 6), (1, 0, 0)), ((4, 8, 4), (1, 1, 1)), ((7, 2, 3), (0, 0, 0)),
+              ((2, 6, 4), (1, 0, 1)), ((9, 9, 9), (0, 0, 0)), ((3, 5, 2), (1, 1, 0)), ((12, 3, 5), (0, 1, 1))]
+
+
+def _level_dense(fem, d, prd):
+    from oracle.oracle import level_apply
+    n = int(np.prod(d))
+    return np.stack([level_apply(fem, d, prd, e) for e in np.eye(n)], 1)
+
+
+@pytest.mark.parametrize("fem", [0, 1])
+@pytest.mark.parametrize("d,prd", FDM_SHAPES)
+def test_fdm_solve_inverts_the_level_operator(fem, d, prd):
+    # x = S D^+ S^T b solves A x = b for every b in the range of A (walls,
+    # periodic and mixed dimensions, odd periodic extents, extent 2), and
+    # differs from the minimum-norm solution pinv(A) b only by a constant
+    from oracle.oracle import fdm_solve, level_apply
+    rng = np.random.default_rng(sum(d) + 7 * fem)
+    A = _level_dense(fem, d, prd)
+    y = rng.standard_normal(len(A))
+    b = A @ y
+    x = fdm_solve(fem, d, prd, b)
+    assert np.abs(level_apply(fem, d, prd, x) - b).max() <= 1e-13 * np.abs(b).max()
+    xp = np.linalg.pinv(A) @ b
+    assert np.abs((x - x.mean()) - (xp - xp.mean())).max() <= 1e-13 * np.abs(xp).max()
+    # the null mode (the constants) is dropped by D^+: the right-hand side
+    # M 1 along it (FEM: the Q1 mass row sums, 1/2 on a wall node per wall
+    # dimension; MAC: the constants) solves to zero
+    w = []
+    for a in range(3):
+        v = np.ones(d[a])
+        if fem and not prd[a]:
+            v[0] = v[-1] = 0.5
+        w.append(v)
+    bn = np.einsum("k,j,i->kji", w[2], w[1], w[0]).ravel()
+    assert np.abs(A @ np.ones(len(A))).max() <= 1e-13
+    assert np.abs(fdm_solve(fem, d, prd, bn)).max() <= 1e-13
+
+
+@pytest.mark.parametrize("fem", [0, 1])
+def test_fdm_solve_of_one_mode(fem):
+    # a single separable mode is an eigenvector: for MAC on walls
+    # b = cos(pi p (i+1/2)/d) ... has x = b / (sum of the 1D eigenvalues)
+    from oracle.oracle import fdm_solve
+    d, prd = (8, 6, 5), (0, 0, 0)
+    p = (3, 1, 2)
+    if fem:
+        g = [np.cos(np.pi * p[a] * np.arange(d[a]) / (d[a] - 1)) for a in range(3)]
+        c = [np.cos(np.pi * p[a] / (d[a] - 1)) for a in range(3)]
+        lam = [6 * (1 - ci) / (2 + ci) for ci in c]
+        # K v = lam M v per dimension: A (gx gy gz) = 12 sum lam_a M-products
+        mv = []
+        for a in range(3):
+            v = g[a]
+            m = 4 * v
+            m[0], m[-1] = 2 * v[0], 2 * v[-1]
+            m[1:] += v[:-1]
+            m[:-1] += v[1:]
+            mv.append(m / 6)
+        b = 12 * (lam[0] * np.einsum("k,j,i->kji", mv[2], mv[1], mv[0]) +
+                  lam[1] * np.einsum("k,j,i->kji", mv[2], mv[1], mv[0]) +
+                  lam[2] * np.einsum("k,j,i->kji", mv[2], mv[1], mv[0])).ravel()
+        want = np.einsum("k,j,i->kji", g[2], g[1], g[0]).ravel()
+    else:
+        g = [np.cos(np.pi * p[a] * (np.arange(d[a]) + 0.5) / d[a]) for a in range(3)]
+        lam = [-2 * (1 - np.cos(np.pi * p[a] / d[a])) for a in range(3)]
+        want = np.einsum("k,j,i->kji", g[2], g[1], g[0]).ravel()
+        b = sum(lam) * want
+    x = fdm_solve(fem, d, prd, b)
+    assert np.abs(x - want).max() <= 1e-13
+
+
+@pytest.mark.parametrize("which,dims,prd,levels,fdm", [
+    (0, (64, 64, 64), (0, 0, 0), [(64, 64, 64), (32, 32, 32)], True),
+    (1, (64, 64, 64), (0, 0, 0), [(65, 65, 65), (33, 33, 33)], True),
+    (0, (96, 64, 56), (0, 1, 0), [(96, 64, 56), (48, 32, 28)], True),
+    (1, (160, 160, 160), (0, 0, 0), [(161, 161, 161), (81, 81, 81), (41, 41, 41)], True),
+    (0, (32, 32, 32), (0, 0, 0), [(32, 32, 32), (16, 16, 16), (8, 8, 8), (4, 4, 4)], False),
+    (0, (1024, 16, 16), (0, 0, 0), [(1024, 16, 16), (512, 8, 8)], True),
+    (0, (2048, 8, 8), (0, 0, 0), [(2048, 8, 8), (1024, 4, 4), (512, 2, 2)], False),
+])
+def test_hierarchy_ends_at_fdm_level(which, dims, prd, levels, fdm):
+    # the first coarse level with 6144 < unknowns <= 300000 and at most 512
+    # per dimension ends the hierarchy; smaller problems keep the
+    # pseudo-inverse level, and a level too long in one dimension is passed
+    cfg, objs = _mg_cfg(dims, prd)
+    S = OracleSim(cfg, objs)
+    assert [tuple(int(v) for v in l) for l in S.mg_levels(which)] == levels
+    assert S.mg_fdm(which) == fdm
+
+
 # ---------------------------------------------------------------- band layers
 def _bfs_layers(rm_c, oid, pad_dims, nlayers):
     """6-neighbour BFS distance from the object's primary cells over the
