@@ -1159,36 +1159,23 @@ void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam) {
 }
 
 // out[.., p, ..] = sum over q of W[q*d + p] in[.., q, ..] along one
-// dimension, each output summed over ascending q from +0.  A workgroup is
-// one wavefront of 64 pencils and one block of FDM_PB outputs p
-// (blockIdx.y); it first copies the da x FDM_PB slice of W it needs into
-// LDS (columns past the end repeat column da-1, unused), so the inner loop
-// reads W as LDS broadcasts, and loads pencil values FDM_QU at a time so
-// that their latencies overlap.
+// dimension, each output summed over ascending q from +0.  A wavefront
+// computes FDM_PB consecutive outputs p for 64 pencils (the same p for all
+// lanes, so its W entries are LDS broadcasts).  The kernels are bound by
+// memory latency, not bandwidth or arithmetic: each workgroup issues all the
+// loads it stages into LDS (W's da x FDM_PB slices, and for x the row tile)
+// as one batch, and streams pencil values FDM_QU at a time, double-buffered.
 constexpr int FDM_PB = 8;
-constexpr int FDM_QU = 16;
+constexpr int FDM_QU = 32;
 
-// copies n values src[f(e)] to dst[g(e)] with all of a lane's loads issued
-// before its stores (n <= 64 * FDM_LB per call)
-constexpr int FDM_LB = 32;
-
+// W slice element e of output block p0: W[q*da + p0 + min(r, last)] with
+// q = e / FDM_PB, r = e % FDM_PB (columns past the end repeat column da-1,
+// unused)
 template <typename T>
-__device__ __forceinline__ void fdm_stage_w(T *wsl, const T *__restrict__ W, int da, int p0) {
-    const int last = da - 1 - p0, n = da * FDM_PB;
-    for (int e0 = 0; e0 < n; e0 += 64 * FDM_LB) {
-        T v[FDM_LB];
-#pragma unroll
-        for (int u = 0; u < FDM_LB; u++) {
-            int e = min(e0 + u * 64 + (int)threadIdx.x, n - 1);
-            int q = e / FDM_PB, r = e - q * FDM_PB;
-            v[u] = W[q * da + p0 + (r < last ? r : last)];
-        }
-#pragma unroll
-        for (int u = 0; u < FDM_LB; u++) {
-            int e = e0 + u * 64 + threadIdx.x;
-            if (e < n) wsl[e] = v[u];
-        }
-    }
+__device__ __forceinline__ T fdm_w_elem(const T *__restrict__ W, int da, int p0, int e) {
+    int q = e / FDM_PB, r = e - q * FDM_PB;
+    int last = da - 1 - p0;
+    return W[q * da + p0 + (r < last ? r : last)];
 }
 
 template <typename T>
@@ -1203,9 +1190,10 @@ __device__ __forceinline__ void fdm_terms(T (&acc)[FDM_PB], const T (&v)[FDM_QU]
     }
 }
 
-// y or z: lanes take consecutive x positions (coalesced loads and stores).
-// Pencil c starts at (c % inner) + (c / inner) * ostride and steps by
-// stride; SCALE multiplies each output by dinv.
+// y or z: one wavefront per workgroup, lanes on consecutive x positions
+// (coalesced loads and stores), blockIdx.y the output block.  Pencil c
+// starts at (c % inner) + (c / inner) * ostride and steps by stride; SCALE
+// multiplies each output by dinv.
 template <typename T, bool SCALE>
 __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__restrict__ out,
                                                  const T *__restrict__ W, const T *__restrict__ dinv, int ncols,
@@ -1213,22 +1201,31 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
     extern __shared__ unsigned char fdm_lds[];
     T *wsl = reinterpret_cast<T *>(fdm_lds);
     const int p0 = blockIdx.y * FDM_PB;
-    fdm_stage_w(wsl, W, da, p0);
-    __syncthreads();
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    int co = c / inner;
-    int base = (c - co * inner) + co * ostride;
-    T acc[FDM_PB];
-#pragma unroll
-    for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
-    // the next FDM_QU pencil values load while the current ones are used
+    const int nw = da * FDM_PB;
+    const int c = min((int)(blockIdx.x * blockDim.x + threadIdx.x), ncols - 1);
+    const int co = c / inner;
+    const int base = (c - co * inner) + co * ostride;
     T v[FDM_QU], vn[FDM_QU];
+    // the first pencil values and the W slice, one batch of loads
 #pragma unroll
     for (int u = 0; u < FDM_QU; u++) {
         v[u] = in[base + min(u, da - 1) * stride];
         vn[u] = T(0);
     }
+    for (int e0 = 0; e0 < nw; e0 += 64 * FDM_QU) {
+        T wv[FDM_QU];
+#pragma unroll
+        for (int u = 0; u < FDM_QU; u++) wv[u] = fdm_w_elem(W, da, p0, min(e0 + u * 64 + (int)threadIdx.x, nw - 1));
+#pragma unroll
+        for (int u = 0; u < FDM_QU; u++) {
+            int e = e0 + u * 64 + threadIdx.x;
+            if (e < nw) wsl[e] = wv[u];
+        }
+    }
+    __syncthreads();
+    T acc[FDM_PB];
+#pragma unroll
+    for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
     for (int q0 = 0; q0 < da; q0 += FDM_QU) {
         if (q0 + FDM_QU < da)
 #pragma unroll
@@ -1237,6 +1234,7 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) v[u] = vn[u];
     }
+    if (blockIdx.x * blockDim.x + threadIdx.x >= (unsigned)ncols) return;
 #pragma unroll
     for (int r = 0; r < FDM_PB; r++)
         if (p0 + r < da) {
@@ -1247,37 +1245,53 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
         }
 }
 
-// x: the workgroup also stages rb (<= 64) consecutive rows in LDS (pitch
-// d0 + 1, so that lanes on different rows read different banks); lane =
-// row, outputs stored straight to the rows.
+// x: a workgroup of nwv wavefronts stages rb (<= 64) consecutive rows in LDS
+// (pitch d0 + 1) and the W slices of nwv output blocks, in one batch of
+// loads; wavefront w computes block blockIdx.y * nwv + w for all rows (lane
+// = row) and stores it straight to the rows.
 template <typename T>
-__global__ void __launch_bounds__(64) k_fdm_rows(const T *__restrict__ in, T *__restrict__ out,
-                                                 const T *__restrict__ W, int d0, int nrows, int rb) {
+__global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *__restrict__ out,
+                                                  const T *__restrict__ W, int d0, int nrows, int rb) {
     extern __shared__ unsigned char fdm_lds[];
-    T *wsl = reinterpret_cast<T *>(fdm_lds);
-    T *tile = wsl + d0 * FDM_PB;
+    const int nwv = blockDim.x >> 6, npb = (d0 + FDM_PB - 1) / FDM_PB;
+    const int nw = d0 * FDM_PB;                   // one block's W slice
+    T *wsl = reinterpret_cast<T *>(fdm_lds);      // nwv slices
+    T *tile = wsl + nwv * nw;
     const int ld = d0 + 1;
-    const int p0 = blockIdx.y * FDM_PB;
     const int r0 = blockIdx.x * rb;
     const int nr = min(rb, nrows - r0);
-    fdm_stage_w(wsl, W, d0, p0);
+    const int pb0 = blockIdx.y * nwv;
+    const int nbw = min(nwv, npb - pb0);          // blocks of this workgroup
     const T *src = in + (size_t)r0 * d0;
-    const int ne = nr * d0;
-    for (int e0 = 0; e0 < ne; e0 += 64 * FDM_LB) {
-        T v[FDM_LB];
+    const int nt = nr * d0, ntot = nbw * nw + nt;
+    for (int e0 = 0; e0 < ntot; e0 += blockDim.x * FDM_QU) {
+        T v[FDM_QU];
 #pragma unroll
-        for (int u = 0; u < FDM_LB; u++) v[u] = src[min(e0 + u * 64 + (int)threadIdx.x, ne - 1)];
+        for (int u = 0; u < FDM_QU; u++) {
+            int e = min(e0 + u * (int)blockDim.x + (int)threadIdx.x, ntot - 1);
+            if (e < nbw * nw) {
+                int b = e / nw;
+                v[u] = fdm_w_elem(W, d0, (pb0 + b) * FDM_PB, e - b * nw);
+            } else {
+                v[u] = src[e - nbw * nw];
+            }
+        }
 #pragma unroll
-        for (int u = 0; u < FDM_LB; u++) {
-            int e = e0 + u * 64 + threadIdx.x;
-            if (e < ne) {
-                int r = e / d0;
-                tile[r * ld + (e - r * d0)] = v[u];
+        for (int u = 0; u < FDM_QU; u++) {
+            int e = e0 + u * blockDim.x + threadIdx.x;
+            if (e < nbw * nw) {
+                wsl[e] = v[u];
+            } else if (e < ntot) {
+                int t = e - nbw * nw, r = t / d0;
+                tile[r * ld + (t - r * d0)] = v[u];
             }
         }
     }
     __syncthreads();
-    const int lane = threadIdx.x;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wv >= nbw) return;
+    const int p0 = (pb0 + wv) * FDM_PB;
+    const T *ws = wsl + wv * nw;
     const int rl = lane < nr ? lane : nr - 1;     // idle lanes repeat the last row, unstored
     const T *row = tile + rl * ld;
     T acc[FDM_PB];
@@ -1287,7 +1301,7 @@ __global__ void __launch_bounds__(64) k_fdm_rows(const T *__restrict__ in, T *__
         T v[FDM_QU];
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) v[u] = row[min(q0 + u, d0 - 1)];
-        fdm_terms(acc, v, wsl, q0, d0);
+        fdm_terms(acc, v, ws, q0, d0);
     }
     if (lane < nr) {
         T *dst = out + (size_t)(r0 + lane) * d0;
@@ -1302,25 +1316,27 @@ static void fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, const T *W, cons
     int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
     int da = L.d[a];
     size_t wbytes = sizeof(T) * (size_t)da * FDM_PB;
-    dim3 blk(64);
     if (a == 0) {
-        // 64 rows per workgroup while W's slice and the tile fit in 64 KiB
+        // up to 4 output blocks and 64 rows per workgroup, within 64 KiB of LDS
+        int npb = (d0 + FDM_PB - 1) / FDM_PB;
+        int nwv = npb < 4 ? npb : 4;
+        while (nwv > 1 && nwv * wbytes > 32768) nwv--;
         int rb = 64;
-        while (rb > 1 && wbytes + sizeof(T) * (size_t)rb * (d0 + 1) > 65536) rb /= 2;
+        while (rb > 1 && nwv * wbytes + sizeof(T) * (size_t)rb * (d0 + 1) > 65536) rb /= 2;
         int nrows = d1 * d2;
-        size_t lds = wbytes + sizeof(T) * (size_t)rb * (d0 + 1);
-        hipLaunchKernelGGL(k_fdm_rows<T>, dim3((nrows + rb - 1) / rb, (d0 + FDM_PB - 1) / FDM_PB), blk, lds, s, in,
-                           out, W, d0, nrows, rb);
+        size_t lds = nwv * wbytes + sizeof(T) * (size_t)rb * (d0 + 1);
+        hipLaunchKernelGGL(k_fdm_rows<T>, dim3((nrows + rb - 1) / rb, (npb + nwv - 1) / nwv), dim3(64 * nwv), lds,
+                           s, in, out, W, d0, nrows, rb);
         return;
     }
     int ncols = L.nn / da;
     int inner = a == 1 ? d0 : d0 * d1, ostride = a == 1 ? d0 * d1 : 0, stride = a == 1 ? d0 : d0 * d1;
     dim3 g((ncols + 63) / 64, (da + FDM_PB - 1) / FDM_PB);
     if (dinv)
-        hipLaunchKernelGGL((k_fdm_cols<T, true>), g, blk, wbytes, s, in, out, W, dinv, ncols, inner, ostride, da,
-                           stride);
+        hipLaunchKernelGGL((k_fdm_cols<T, true>), g, dim3(64), wbytes, s, in, out, W, dinv, ncols, inner, ostride,
+                           da, stride);
     else
-        hipLaunchKernelGGL((k_fdm_cols<T, false>), g, blk, wbytes, s, in, out, W, (const T *)nullptr, ncols,
+        hipLaunchKernelGGL((k_fdm_cols<T, false>), g, dim3(64), wbytes, s, in, out, W, (const T *)nullptr, ncols,
                            inner, ostride, da, stride);
 }
 
