@@ -85,10 +85,12 @@ def sweep_words(np_):
     return 1.0 + np_ / 4.0 + np_ / 4.0
 
 
-# One V-cycle, per finest unknown, at 2 pre + 2 post sweeps: the sum over the
-# levels (geometric factor 8/7) of the smoothing passes, residual, restriction,
-# prolongation and coarse-correction zeroing.
-VCYCLE_WORDS = 21.0
+# One V-cycle, per finest unknown, at 2 pre + 2 post sweeps of the finest
+# level (red-black: 4 sweeps of 3 words, residual 3, restriction 1.25,
+# prolongation 2.125 = 18.4 words) and the fast-diagonalisation solve of the
+# first coarse level, which ends the hierarchy of the 128^3 workload (six
+# passes reading and writing 1/8 as many unknowns: 1.5 words).
+VCYCLE_WORDS = 19.9
 # One FP64 defect-correction step of the mixed path adds the FP64 residual
 # (read x, b, write r) per finest unknown.
 DEFECT_WORDS64 = 3.0
