@@ -583,10 +583,18 @@ __device__ __forceinline__ bool upwind_map(const Geo &g, const Fluid &U, const M
     return true;
 }
 
-__global__ __launch_bounds__(64) void k_upwind(Geo g, Fluid U, Maps M, int *status) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > g.m + 1) return;
+// one work-item per cell of the (m+1) x (n+1) x (o+1) block from (1,1,1),
+// x fastest (a flat grid: rows of m+1 cells would leave most of a third
+// 64-wide workgroup per row idle)
+__global__ __launch_bounds__(256) void k_upwind(Geo g, Fluid U, Maps M, int *status) {
+    const int w = g.m + 1, h = g.n + 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w * h * (g.o + 1)) return;
+    int row = t / w;
+    int i = t - row * w + 1;
+    int k = row / h;
+    int j = row - k * h + 1;
+    k += 1;
     int ind = IX(g, i, j, k);
     const int str[3] = {1, g.m4, g.mn4};
     if ((M.r.c[ind] & OMASK) > 0)
@@ -1129,7 +1137,8 @@ void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, co
     }
 }
 void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status) {
-    hipLaunchKernelGGL(k_upwind, rows(g.m + 1, g.n + 1, g.o + 1, 64), dim3(64), 0, s, g, U, M, status);
+    int nup = (g.m + 1) * (g.n + 1) * (g.o + 1);
+    hipLaunchKernelGGL(k_upwind, dim3((nup + 255) / 256), dim3(256), 0, s, g, U, M, status);
 }
 static FaceC make_face(const Geo &g, const FaceSpec &f) {
     FaceC F;

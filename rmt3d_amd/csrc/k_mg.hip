@@ -317,45 +317,86 @@ __device__ __forceinline__ void resid_at(const MGLevel<T> &L, const Prd &P, int 
     int k = qdiv(row, D1, r1);                 \
     int j = row - k * (D1);
 
-// Node order of a residual pass: MAC levels flat; FEM levels the four colour
-// layouts one after the other, so interior nodes take the fast path.
+// Node order of a residual pass.  MAC levels, and FEM levels with an extent
+// below 4, are flat (x fastest).  Other FEM levels take the interior box
+// (1..D-2 in every dimension: no wall, no wrap) first, flat, so that its
+// nodes take the fast path together and neighbouring rows are read while
+// cached, then the boundary: the planes k = 0 and k = D2-1, then per inner
+// k the rows j = 0 and j = D1-1 and the two end nodes of every inner row.
 struct NodeMap {
-    int fem, total;
-    float r0, r1;            // MAC: reciprocals of D0, D1
-    int off[5];              // FEM: first work-item of each colour
-    FemColour F[4];
+    int fem, plain, total;
+    int D0, D1, D2;
+    float r0, r1;            // flat: reciprocals of D0, D1
+    int NI, wi, wj;          // interior count and extents in x and y
+    float rwi, rwj;
+    int S;                   // boundary nodes per inner k: 2 D0 + 2 (D1 - 2)
+    float rS;
 };
 
 static __host__ __device__ NodeMap node_map(const int d[3], int fem) {
     NodeMap M{};
     M.fem = fem;
-    if (!fem) {
-        M.total = d[0] * d[1] * d[2];
-        M.r0 = 1.0f / d[0];
-        M.r1 = 1.0f / d[1];
-        return M;
-    }
-    M.off[0] = 0;
-    for (int c = 0; c < 4; c++) {
-        M.F[c] = fem_colour(d, c);
-        M.off[c + 1] = M.off[c] + M.F[c].NI + M.F[c].NA + M.F[c].NB;
-    }
-    M.total = M.off[4];
+    M.D0 = d[0];
+    M.D1 = d[1];
+    M.D2 = d[2];
+    M.total = d[0] * d[1] * d[2];
+    M.r0 = 1.0f / d[0];
+    M.r1 = 1.0f / d[1];
+    M.plain = !fem || d[0] < 4 || d[1] < 4 || d[2] < 4;
+    if (M.plain) return M;
+    M.wi = d[0] - 2;
+    M.wj = d[1] - 2;
+    M.NI = M.wi * M.wj * (d[2] - 2);
+    M.rwi = 1.0f / M.wi;
+    M.rwj = 1.0f / M.wj;
+    M.S = 2 * d[0] + 2 * (d[1] - 2);
+    M.rS = 1.0f / M.S;
     return M;
 }
 
 // returns true for an interior FEM node
 template <typename T>
 __device__ __forceinline__ bool map_node(const MGLevel<T> &L, const NodeMap &M, int t, int &i, int &j, int &k) {
-    if (!M.fem) {
-        int row = qdiv(t, L.d[0], M.r0);
-        i = t - row * L.d[0];
-        k = qdiv(row, L.d[1], M.r1);
-        j = row - k * L.d[1];
+    const int D0 = M.D0, D1 = M.D1;
+    if (M.plain) {
+        int row = qdiv(t, D0, M.r0);
+        i = t - row * D0;
+        k = qdiv(row, D1, M.r1);
+        j = row - k * D1;
         return false;
     }
-    int c = (t >= M.off[1]) + (t >= M.off[2]) + (t >= M.off[3]);
-    return fem_colour_node(L, M.F[c], c & 1, t - M.off[c], i, j, k);
+    if (t < M.NI) {
+        int row = qdiv(t, M.wi, M.rwi);
+        i = 1 + (t - row * M.wi);
+        int kk = qdiv(row, M.wj, M.rwj);
+        j = 1 + (row - kk * M.wj);
+        k = 1 + kk;
+        return true;
+    }
+    t -= M.NI;
+    const int plane = D0 * D1;
+    if (t < 2 * plane) {
+        int hi = t >= plane;
+        int r = t - hi * plane;
+        k = hi ? M.D2 - 1 : 0;
+        j = qdiv(r, D0, M.r0);
+        i = r - j * D0;
+        return false;
+    }
+    t -= 2 * plane;
+    int kk = qdiv(t, M.S, M.rS);
+    k = 1 + kk;
+    int r = t - kk * M.S;
+    if (r < 2 * D0) {
+        int hi = r >= D0;
+        j = hi ? D1 - 1 : 0;
+        i = r - hi * D0;
+    } else {
+        r -= 2 * D0;
+        j = 1 + (r >> 1);
+        i = (r & 1) ? D0 - 1 : 0;
+    }
+    return false;
 }
 
 // r = b - A x at one node; the interior FEM form is fem_off's sum
