@@ -1211,23 +1211,28 @@ constexpr int FDM_QU = 32;
 
 // W slice element e of output block p0: W[q*da + p0 + min(r, last)] with
 // q = e / FDM_PB, r = e % FDM_PB (columns past the end repeat column da-1,
-// unused)
+// unused); rows q >= da are zero.  The terms run in branch-free chunks of
+// FDM_QU, then of 4: the slice has fdm_rows(da) rows (da rounded up to a
+// multiple of 4) and pencil values past da are zero too.  The extra terms
+// are +0 x w, which leave a sum that started at +0 unchanged (such a sum is
+// never -0).
+__host__ __device__ constexpr int fdm_rows(int da) { return (da + 3) / 4 * 4; }
+
 template <typename T>
 __device__ __forceinline__ T fdm_w_elem(const T *__restrict__ W, int da, int p0, int e) {
     int q = e / FDM_PB, r = e - q * FDM_PB;
     int last = da - 1 - p0;
-    return W[q * da + p0 + (r < last ? r : last)];
+    T w = W[min(q, da - 1) * da + p0 + (r < last ? r : last)];
+    return q < da ? w : T(0);
 }
 
-template <typename T>
-__device__ __forceinline__ void fdm_terms(T (&acc)[FDM_PB], const T (&v)[FDM_QU], const T *wsl, int q0, int da) {
+template <int U, typename T>
+__device__ __forceinline__ void fdm_terms(T (&acc)[FDM_PB], const T *v, const T *wsl, int q0) {
 #pragma unroll
-    for (int u = 0; u < FDM_QU; u++) {
-        if (q0 + u < da) {
-            const T *w = wsl + (q0 + u) * FDM_PB;
+    for (int u = 0; u < U; u++) {
+        const T *w = wsl + (q0 + u) * FDM_PB;
 #pragma unroll
-            for (int r = 0; r < FDM_PB; r++) acc[r] += w[r] * v[u];
-        }
+        for (int r = 0; r < FDM_PB; r++) acc[r] += w[r] * v[u];
     }
 }
 
@@ -1242,10 +1247,11 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
     extern __shared__ unsigned char fdm_lds[];
     T *wsl = reinterpret_cast<T *>(fdm_lds);
     const int p0 = blockIdx.y * FDM_PB;
-    const int nw = da * FDM_PB;
+    const int nw = fdm_rows(da) * FDM_PB;
     const int c = min((int)(blockIdx.x * blockDim.x + threadIdx.x), ncols - 1);
     const int co = c / inner;
     const int base = (c - co * inner) + co * ostride;
+    const int dfull = da / FDM_QU * FDM_QU, dpad = fdm_rows(da);
     T v[FDM_QU], vn[FDM_QU];
     // the first pencil values and the W slice, one batch of loads
 #pragma unroll
@@ -1267,13 +1273,22 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
     T acc[FDM_PB];
 #pragma unroll
     for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
-    for (int q0 = 0; q0 < da; q0 += FDM_QU) {
-        if (q0 + FDM_QU < da)
+    for (int q0 = 0; q0 < dfull; q0 += FDM_QU) {
+        if (q0 + FDM_QU < dfull)
 #pragma unroll
-            for (int u = 0; u < FDM_QU; u++) vn[u] = in[base + min(q0 + FDM_QU + u, da - 1) * stride];
-        fdm_terms(acc, v, wsl, q0, da);
+            for (int u = 0; u < FDM_QU; u++) vn[u] = in[base + (q0 + FDM_QU + u) * stride];
+        fdm_terms<FDM_QU>(acc, v, wsl, q0);
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) v[u] = vn[u];
+    }
+    for (int q0 = dfull; q0 < dpad; q0 += 4) {
+        T t4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            T x = in[base + min(q0 + u, da - 1) * stride];
+            t4[u] = q0 + u < da ? x : T(0);
+        }
+        fdm_terms<4>(acc, t4, wsl, q0);
     }
     if (blockIdx.x * blockDim.x + threadIdx.x >= (unsigned)ncols) return;
 #pragma unroll
@@ -1295,7 +1310,7 @@ __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *_
                                                   const T *__restrict__ W, int d0, int nrows, int rb) {
     extern __shared__ unsigned char fdm_lds[];
     const int nwv = blockDim.x >> 6, npb = (d0 + FDM_PB - 1) / FDM_PB;
-    const int nw = d0 * FDM_PB;                   // one block's W slice
+    const int nw = fdm_rows(d0) * FDM_PB;         // one block's W slice
     T *wsl = reinterpret_cast<T *>(fdm_lds);      // nwv slices
     T *tile = wsl + nwv * nw;
     const int ld = d0 + 1;
@@ -1338,11 +1353,20 @@ __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *_
     T acc[FDM_PB];
 #pragma unroll
     for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
+    // row values come from LDS; here a guarded chunk of FDM_QU terms
+    // measured faster than the zero-padded chunks of the y and z passes
     for (int q0 = 0; q0 < d0; q0 += FDM_QU) {
         T v[FDM_QU];
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) v[u] = row[min(q0 + u, d0 - 1)];
-        fdm_terms(acc, v, ws, q0, d0);
+#pragma unroll
+        for (int u = 0; u < FDM_QU; u++) {
+            if (q0 + u < d0) {
+                const T *w = ws + (q0 + u) * FDM_PB;
+#pragma unroll
+                for (int r = 0; r < FDM_PB; r++) acc[r] += w[r] * v[u];
+            }
+        }
     }
     if (lane < nr) {
         T *dst = out + (size_t)(r0 + lane) * d0;
@@ -1356,7 +1380,7 @@ template <typename T>
 static void fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, const T *W, const T *dinv, const T *in, T *out) {
     int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
     int da = L.d[a];
-    size_t wbytes = sizeof(T) * (size_t)da * FDM_PB;
+    size_t wbytes = sizeof(T) * (size_t)fdm_rows(da) * FDM_PB;
     if (a == 0) {
         // up to 4 output blocks and 64 rows per workgroup, within 64 KiB of LDS
         int npb = (d0 + FDM_PB - 1) / FDM_PB;
