@@ -66,7 +66,8 @@ def _mg_pair_solve(which, dims, prd, prec, seed):
 
 @pytest.mark.parametrize("which", [0, 1])
 @pytest.mark.parametrize("dims,prd", [((64, 64, 64), (0, 0, 0)), ((64, 64, 64), (1, 1, 1)),
-                                      ((64, 48, 80), (1, 0, 1)), ((96, 64, 56), (0, 1, 0))])
+                                      ((64, 48, 80), (1, 0, 1)), ((96, 64, 56), (0, 1, 0)),
+                                      ((192, 96, 4), (0, 0, 0)), ((4, 96, 192), (0, 0, 0))])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_mg_solve_fdm_level_matches_oracle(which, dims, prd, prec):
     # the first coarse level ends the hierarchy and is solved by fast
