@@ -303,11 +303,17 @@ __global__ __launch_bounds__(1024) void k_stress_fluid(Geo g, Phys P, Fluid U, M
 // one thread per solid cell, one launch per face direction; launched in
 // 64-work-item workgroups, which the bound tells the compiler so that the
 // per-cell state fits in registers instead of spilling to scratch
-template <int F>
-__global__ __launch_bounds__(64) void k_stress_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count) {
+// the three faces in one launch, blockIdx.y = F (uniform per workgroup);
+// the faces write disjoint fields, so they may run concurrently
+__global__ __launch_bounds__(64) void k_stress_solid3(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list,
+                                                      const int *count) {
     int n = *count;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-        stress_face<F>(g, P, O, U, M, list[t]);
+    const int F = blockIdx.y;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        if (F == 0) stress_face<0>(g, P, O, U, M, list[t]);
+        else if (F == 1) stress_face<1>(g, P, O, U, M, list[t]);
+        else stress_face<2>(g, P, O, U, M, list[t]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1121,9 +1127,7 @@ void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, co
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, count);
     int ncell = (g.m + 3) * (g.n + 3) * (g.o + 3);
     hipLaunchKernelGGL(k_stress_fluid, dim3((ncell + 1023) / 1024), dim3(1024), 0, s, g, P, U, M, list, count);
-    hipLaunchKernelGGL(k_stress_solid<0>, dim3(4096), dim3(64), 0, s, g, P, O, U, M, list, count);
-    hipLaunchKernelGGL(k_stress_solid<1>, dim3(4096), dim3(64), 0, s, g, P, O, U, M, list, count);
-    hipLaunchKernelGGL(k_stress_solid<2>, dim3(4096), dim3(64), 0, s, g, P, O, U, M, list, count);
+    hipLaunchKernelGGL(k_stress_solid3, dim3(4096, 3), dim3(64), 0, s, g, P, O, U, M, list, count);
 }
 void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    const int *list, const int *count, int *status) {
