@@ -68,6 +68,8 @@ def parse():
                     help="instead of the step: time the checkpoint write (device gather + files) and the "
                          "resume (files + device scatter) of the FP64 state after --warmup steps")
     ap.add_argument("--io-reps", type=int, default=5)
+    ap.add_argument("--shear", action="store_true",
+                    help="time the 256^3 shear workload (64 spheres, periodic x and y) instead")
     ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=16)
     return ap.parse_args()
@@ -369,6 +371,36 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_launch": n, "launches": 5}), flush=True)
 
 
+def run_shear(args):
+    """The largest workload, 256^3 with 64 neo-Hookean spheres between
+    moving walls (periodic x and y), on the mixed and the FP64 path: after
+    initialize() and --warmup steps, the wall time of --steps steps (each
+    step returns after its status read-back).  One JSON line."""
+    from rmt3d_amd import _abi, configs
+    from rmt3d_amd import lib as rlib
+    res = {}
+    for prec, name in ((_abi.PREC_MIXED, "mixed"), (_abi.PREC_FP64, "fp64")):
+        cfg, objs = configs.shear_spheres(256)
+        S = rlib.Simulation(cfg, objs, precision=prec)
+        S.initialize()
+        for _ in range(args.warmup):
+            S.step()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            S.step()
+        dt = (time.perf_counter() - t0) / args.steps
+        res[name] = {"ms_per_step": round(dt * 1e3, 3), "cell_steps_per_s": round(256 ** 3 / dt, 1),
+                     "mac_vcycles_last_step": S.diag(1), "fem_vcycles_last_step": S.diag(2),
+                     "levels_mac_fem": [S.diag(10), S.diag(11)], "fdm_level": [S.diag(18), S.diag(19)]}
+        print("bench: shear %s %.1f ms/step" % (name, dt * 1e3), file=sys.stderr, flush=True)
+        del S
+    print(json.dumps({"metric": "FSI cell-steps/s, 256^3 shear workload (64 spheres)",
+                      "value": res["mixed"]["cell_steps_per_s"], "unit": "cell-steps/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+                      "data": "synthetic", "config": {"workload": "shear_spheres_256", "grid": [256, 256, 256],
+                                                      "objects": 64, "mixed": res["mixed"], "fp64": res["fp64"]}}))
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -376,6 +408,9 @@ def main():
         return
     if args.io:
         run_io(args)
+        return
+    if args.shear:
+        run_shear(args)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
