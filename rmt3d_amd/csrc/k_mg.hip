@@ -1204,8 +1204,9 @@ void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam) {
 // computes FDM_PB consecutive outputs p for 64 pencils (the same p for all
 // lanes, so its W entries are LDS broadcasts).  The kernels are bound by
 // memory latency, not bandwidth or arithmetic: each workgroup issues all the
-// loads it stages into LDS (W's da x FDM_PB slices, and for x the row tile)
-// as one batch, and streams pencil values FDM_QU at a time, double-buffered.
+// loads it stages into LDS (W's fdm_rows(da) x FDM_PB slices, and for x the
+// row tile) as one batch; along y and z it streams pencil values from memory
+// FDM_QU at a time, double-buffered.
 constexpr int FDM_PB = 8;
 constexpr int FDM_QU = 32;
 
