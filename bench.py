@@ -382,12 +382,18 @@ def run_shear(args):
     for prec, name in ((_abi.PREC_MIXED, "mixed"), (_abi.PREC_FP64, "fp64")):
         cfg, objs = configs.shear_spheres(256)
         S = rlib.Simulation(cfg, objs, precision=prec)
-        S.initialize()
+        S.prepare_run(S.initialize())
         for _ in range(args.warmup):
-            S.step()
+            rc = S.step()
+            if rc:
+                raise RuntimeError("shear %s warm-up step returned %d" % (name, rc))
+        S.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            S.step()
+            rc = S.step()
+            if rc:
+                raise RuntimeError("shear %s step returned %d" % (name, rc))
+        S.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         res[name] = {"ms_per_step": round(dt * 1e3, 3), "cell_steps_per_s": round(256 ** 3 / dt, 1),
                      "mac_vcycles_last_step": S.diag(1), "fem_vcycles_last_step": S.diag(2),

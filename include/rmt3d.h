@@ -50,8 +50,8 @@ extern "C" {
 
 /* Arithmetic precision of a context. */
 #define RMT3D_PREC_FP64  0   /* every stage in FP64 (the parity path)           */
-#define RMT3D_PREC_MIXED 1   /* FP32 stencils + FP32 MG preconditioner inside an
-                                FP64 defect-correction loop                      */
+#define RMT3D_PREC_MIXED 1   /* FP32 multigrid preconditioner inside an FP64
+                                defect-correction loop; all other stages FP64    */
 
 /* One immersed object: the per-object keys of the .cfg
  * (object_list, basic_specs, extra_specs, srho, shear_mod;
@@ -170,7 +170,11 @@ int rmt3d_set_time(rmt3d_ctx *ctx, double t);
 
 /* State transfer in the reference's index order (the checkpoint path,
  * fluid_3d_chk_pt.cc:74-163 reads the same fields).  upload replaces the
- * fields whose pointers are non-NULL; download fills them. */
+ * fields whose pointers are non-NULL; download fills them.  A band upload
+ * needs all four band arrays (cells ascending, the entries of one cell in
+ * multimap order) and replaces the whole band, each entry as a fresh
+ * extrap_mmap insertion (extrap_mmap.cc:3-60): face values and det F zero,
+ * face normals not yet computed. */
 int rmt3d_upload_state(rmt3d_ctx *ctx, const rmt3d_state *st);
 int rmt3d_download_state(rmt3d_ctx *ctx, rmt3d_state *st);
 /* Number of band entries currently held (to size rmt3d_state arrays). */
@@ -224,7 +228,11 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * level count, 12/13 -> first level run by the one-workgroup tail kernel,
  * 14/15 -> tail levels held in LDS (bit 0 FP64 hierarchy, bit 1 FP32),
  * 16/17 -> colour passes per smoothing launch on the finest level,
- * 18/19 -> 1 when the coarsest level is solved by fast diagonalisation. */
+ * 18/19 -> 1 when the coarsest level is solved by fast diagonalisation;
+ * branch counters of the last rmt3d_step (no counterpart in the reference):
+ * 20 -> collision_stress contributions (lower face x object pair, both
+ * stress passes), 21 -> primary maps redrawn into the band, 22 -> band maps
+ * redrawn onto the primary grid (redraw_boundary, fluid_3d_ex.cc:336-394). */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
 /* Tracers (Utilities/tracers.{hh,cc}): with dump_code & 1 and ntracers > 0

@@ -98,6 +98,12 @@ def lib():
         L.oracle_set_time.argtypes = [P, D]
         L.oracle_band_count.argtypes = [P]
         L.oracle_compute_stress.argtypes = [P]
+        L.oracle_collision_face.argtypes = [P, C.c_int, C.c_int, dp]
+        L.oracle_redraw.argtypes = [P]
+        L.oracle_set_bc_cc.argtypes = [P]
+        L.oracle_set_bc_cf.argtypes = [P]
+        L.oracle_extrap_now.argtypes = [P]
+        L.oracle_gphi.argtypes = [P, C.c_int, dp, ip, C.POINTER(C.c_uint32)]
         L.oracle_scalar.argtypes = [P, C.c_int]
         L.oracle_scalar.restype = D
         L.oracle_download.argtypes = [P, C.POINTER(_abi.State)]
@@ -288,6 +294,52 @@ class OracleSim:
 
     def div_u(self):
         return lib().oracle_div_u(self.h)
+
+    def step_counters(self):
+        """Branch counters of the last step: (collision contributions, primary
+        maps redrawn into the band, band maps redrawn onto the primary grid)."""
+        return tuple(int(self.scalar(w)) for w in (9, 10, 11))
+
+    def collision_face(self, eid, F):
+        """collision_stress<F> of the cell at eid (the reference's
+        non-ghost-offset index) alone: the traction row it adds."""
+        fs = np.zeros(3)
+        lib().oracle_collision_face(self.h, int(eid), int(F), _dp(fs))
+        return fs
+
+    def redraw(self):
+        """redraw_boundary on the current state."""
+        lib().oracle_redraw(self.h)
+
+    def compute_stress(self):
+        """compute_stress on the current state (writes sigma and each map's
+        face normals, det F and energy)."""
+        lib().oracle_compute_stress(self.h)
+
+    def extrap_now(self):
+        """One extrapolation pass from the primary xpred, the current band
+        becoming the previous one."""
+        lib().oracle_extrap_now(self.h)
+
+    def gphi_primary(self):
+        out = np.zeros(9 * self.Ng)
+        lib().oracle_gphi(self.h, 0, _dp(out), None, None)
+        return out.reshape(9, self.Ng)
+
+    def gphi_prev_band(self):
+        n = lib().oracle_gphi(self.h, 1, None, None, None)
+        out = np.zeros(9 * max(n, 1))
+        cell = np.zeros(max(n, 1), dtype=np.int32)
+        c = np.zeros(max(n, 1), dtype=np.uint32)
+        lib().oracle_gphi(self.h, 1, _dp(out), cell.ctypes.data_as(C.POINTER(C.c_int)),
+                          c.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return cell[:n], c[:n], out[:9 * n].reshape(9, n)
+
+    def set_bc_cc(self):
+        lib().oracle_set_bc_cc(self.h)
+
+    def set_bc_cf(self):
+        lib().oracle_set_bc_cf(self.h)
 
     def download(self, band=True):
         Ng = self.Ng

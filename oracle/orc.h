@@ -184,6 +184,11 @@ typedef struct orc_sim {
     double mac_res, fem_res;
     int mac_cycles, fem_cycles;
 
+    /* branch counters of the current step (reset by orc_step): face and
+     * object-pair contributions of collision_stress, primary maps moved to
+     * the band and band maps moved to the primary grid by redraw_boundary */
+    long n_coll, n_redraw_out, n_redraw_in;
+
     /* tracers */
     int trace, ntr;
     orc_tracer *tr;
@@ -201,6 +206,7 @@ double orc_blocked_sum(const double *a, int n, double scale);
 double orc_phi(const orc_sim *S, int obj_index, const double xi[3]);
 double orc_heaviside(const orc_sim *S, double phi);
 void orc_compute_stress(orc_sim *S);
+void orc_collision_face(orc_sim *S, int eid, int F, double fs[3]);
 
 void orc_init_extrapolate(orc_sim *S);
 void orc_fill_pp_rhs(orc_sim *S, double *rp);

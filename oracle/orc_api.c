@@ -35,9 +35,43 @@ void oracle_set_time(orc_sim *S, double t) { S->time = t; }
 double oracle_div_u(orc_sim *S) { return orc_div_u(S); }
 int oracle_band_count(orc_sim *S) { return S->ext.n; }
 void oracle_compute_stress(orc_sim *S) { orc_compute_stress(S); }
+void oracle_collision_face(orc_sim *S, int eid, int F, double *fs) { orc_collision_face(S, eid, F, fs); }
+void oracle_redraw(orc_sim *S) { S->n_redraw_out = S->n_redraw_in = 0; orc_redraw_boundary(S); }
+void oracle_set_bc_cc(orc_sim *S) { orc_set_bc_cc(S); }
+void oracle_set_bc_cf(orc_sim *S) { orc_set_bc_cf(S); }
+/* One extrapolation pass from the current primary xpred, with the current
+ * band kept as the previous one (get_refmap_prev), as update_reference_map
+ * runs it (fluid_3d.cc:1182-1190). */
+void oracle_extrap_now(orc_sim *S)
+{
+    orc_band_copy(&S->tmp, &S->ext, S->Ng);
+    orc_band_reset(&S->ext, S->Ng);
+    orc_extrapolate(S);
+}
+/* Face normals grad_phi[F][a] (written by compute_stress): which 0 -> the
+ * primary maps, out[(3F+a) Ng + cell]; which 1 -> the previous band (tmp),
+ * out[(3F+a) n + e] with cell[e], c[e].  Returns the entry count. */
+int oracle_gphi(orc_sim *S, int which, double *out, int *cell, uint32_t *c)
+{
+    if (which == 0) {
+        for (int a = 0; a < S->Ng; a++)
+            for (int q = 0; q < 9; q++) out[(size_t)q * S->Ng + a] = S->rm[a].gphi[q / 3][q % 3];
+        return S->Ng;
+    }
+    int n = S->tmp.n;
+    if (!out) return n;
+    for (int e = 0; e < n; e++) {
+        cell[e] = S->tmp.e[e].cell;
+        c[e] = S->tmp.e[e].r.c;
+        for (int q = 0; q < 9; q++) out[(size_t)q * n + e] = S->tmp.e[e].r.gphi[q / 3][q % 3];
+    }
+    return n;
+}
 
 /* scalars: 0 dt_reg, 1 eps, 2 K_stiff, 3 vmax, 4 mac cycles, 5 fem cycles,
- * 6 mac res, 7 fem res, 8 nmask */
+ * 6 mac res, 7 fem res, 8 nmask; counters of the last step: 9 collision
+ * contributions, 10 primary maps redrawn into the band, 11 band maps redrawn
+ * onto the primary grid */
 double oracle_scalar(orc_sim *S, int which)
 {
     switch (which) {
@@ -50,6 +84,9 @@ double oracle_scalar(orc_sim *S, int which)
     case 6: return S->mac_res;
     case 7: return S->fem_res;
     case 8: return (double)S->nmask;
+    case 9: return (double)S->n_coll;
+    case 10: return (double)S->n_redraw_out;
+    case 11: return (double)S->n_redraw_in;
     }
     return 0;
 }

@@ -1255,9 +1255,14 @@ static void collision_stress(orc_sim *S, int eid, int F, double fs[3])
             double avg = (cs[0] + cs[3] + cs[5]) / 3.;
             for (int a = 0; a < 3; a++) cs[symind(a, a)] -= avg;
             for (int a = 0; a < 3; a++) fs[a] += cs[symind(F, a)];
+#pragma omp atomic
+            S->n_coll++;
         }
     }
 }
+
+/* collision_stress of one lower face alone (a test hook: fs is added to) */
+void orc_collision_face(orc_sim *S, int eid, int F, double fs[3]) { collision_stress(S, eid, F, fs); }
 
 /* fluid_3d::compute_stress (fluid_3d.cc:1360-1461); fluid_stress<F> is the
  * within_local_domain branch (fluid_3d.cc:2001-2031), always taken here */
@@ -1324,6 +1329,7 @@ int orc_step(orc_sim *S, int debug)
 {
     (void)debug;
     S->nt++;
+    S->n_coll = S->n_redraw_out = S->n_redraw_in = 0;
     if (S->trace) orc_tracers_predict(S);   /* fluid_3d.cc:458-461 */
     orc_compute_stress(S);
     normal_derivatives_velocity(S);

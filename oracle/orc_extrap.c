@@ -554,6 +554,7 @@ void orc_redraw_boundary(orc_sim *S)
                 loc[nl].r = *prime;
                 nl++;
                 memset(prime, 0, sizeof *prime);
+                S->n_redraw_out++;
             } else {
                 occupied = 1;
             }
@@ -576,6 +577,7 @@ void orc_redraw_boundary(orc_sim *S)
                 *prime = *sf;
                 prime->c = oid;
                 occupied = 1;
+                S->n_redraw_in++;
                 /* extrap_mmap::rm_mem (extrap_mmap.cc:207-257): first match */
                 int j0 = -1;
                 for (int b = 0; b < nl; b++)

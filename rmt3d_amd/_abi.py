@@ -12,6 +12,8 @@ MAX_VEL_PROF = 8
 SIM_FTEST, SIM_STEST, SIM_OBJECTS, SIM_MILLS = 0, 1, 2, 3
 BASIC_SPHERE, SPHERE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+# step return codes (Utilities/defs.hh:47-50)
+NORMAL_EXIT, REFMAP_UPDATE_ERR, PROJECTION_ERR = 0, 1, 2
 
 
 class Object(C.Structure):

@@ -174,6 +174,13 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out) {
         case 17: *out = S.fem.npass[0]; return 0;
         case 18: *out = S.mac.fdm; return 0;
         case 19: *out = S.fem.fdm; return 0;
+        case 20: case 21: case 22: {
+            int c[3];
+            int e = S.step_counters(c);
+            if (e) return e;
+            *out = c[which - 20];
+            return 0;
+        }
         }
     } catch (...) {
         return RMT3D_ERR_FATAL;

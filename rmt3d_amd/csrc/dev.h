@@ -39,6 +39,10 @@ struct Phys {
     double wall_pos[6];
     int sim_type;
     int n_obj;
+    // branch counters of the current step: [0] collision contributions
+    // (face x object pair), [1] primary maps redrawn into the band, [2] band
+    // maps redrawn onto the primary grid
+    int *counters;
 };
 
 // per-object constants (sl_mat + sphere geometry), device array

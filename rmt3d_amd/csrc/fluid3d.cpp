@@ -862,7 +862,9 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     W.fill = (int *)dalloc(sizeof(int) * Ng);
     W.scan_tmp = (int *)dalloc(sizeof(int) * ((Ng + 1023) / 1024 + 1));
     W.err = (int *)dalloc(sizeof(int));
-    d_status = (int *)dalloc(sizeof(int));
+    // status word, then the three branch counters of the step (Phys::counters)
+    d_status = (int *)dalloc(4 * sizeof(int));
+    P.counters = d_status + 1;
     d_solid_list = (int *)dalloc(sizeof(int) * Ng);
     d_solid_n = (int *)dalloc(sizeof(int));
     d_u64 = (unsigned long long *)dalloc(sizeof(unsigned long long) * 4);
@@ -1160,6 +1162,7 @@ int Sim::step(int debug) {
     // tracers::predict on the velocities at the start of the step
     // (fluid_3d.cc:458-461)
     if (trace) launch_tracer_predict(stream, TQ, TR, U, dt);
+    launch_zero_i32(stream, P.counters, 3);
     Maps M = maps();
     mark(3);
     kprof[2].begin(stream);
@@ -1318,6 +1321,61 @@ int Sim::upload(const rmt3d_state *st) {
     if (st->rm_x) for (int a = 0; a < 3; a++) up(R.x[a], st->rm_x + a * n, w);
     if (st->rm_xpred) for (int a = 0; a < 3; a++) up(R.xpred[a], st->rm_xpred + a * n, w);
     HCHK(hipStreamSynchronize(stream));
+    if (st->band_cell || st->band_c || st->band_x || st->band_xpred) return upload_band(st);
+    return 0;
+}
+
+// Replaces the band by st's nband entries (cells ascending, the entries of a
+// cell in multimap order), as a freshly added extrap_mmap entry: face values
+// zero, face normals not computed (-1e200, fields.hh:311-330), det F and
+// energies zero.  All four band arrays are required.
+int Sim::upload_band(const rmt3d_state *st) {
+    int nb = st->nband;
+    if (!st->band_cell || !st->band_c || !st->band_x || !st->band_xpred || nb < 0) {
+        err = "band upload needs band_cell, band_c, band_x and band_xpred";
+        return RMT3D_ERR_ARG;
+    }
+    if (nb > ext.cap) {
+        err = "band upload exceeds the band capacity";
+        return RMT3D_ERR_ARG;
+    }
+    std::vector<int> cnt((size_t)g.Ng, 0), start((size_t)g.Ng, 0);
+    for (int e = 0; e < nb; e++) {
+        int c = st->band_cell[e];
+        if (c < 0 || c >= g.Ng || (e > 0 && c < st->band_cell[e - 1])) {
+            err = "band cells must be ascending and inside the padded grid";
+            return RMT3D_ERR_ARG;
+        }
+        cnt[c]++;
+    }
+    for (int c = 0, run = 0; c < g.Ng; c++) { start[c] = run; run += cnt[c]; }
+    size_t wb = sizeof(double) * (size_t)nb;
+    std::vector<double> nc((size_t)nb, -1.e200);
+    HCHK(hipMemcpyAsync(ext.start, start.data(), sizeof(int) * (size_t)g.Ng, hipMemcpyHostToDevice, stream));
+    HCHK(hipMemcpyAsync(ext.cnt, cnt.data(), sizeof(int) * (size_t)g.Ng, hipMemcpyHostToDevice, stream));
+    HCHK(hipMemcpyAsync(ext.n, &nb, sizeof(int), hipMemcpyHostToDevice, stream));
+    if (nb > 0) {
+        HCHK(hipMemcpyAsync(ext.cell, st->band_cell, sizeof(int) * (size_t)nb, hipMemcpyHostToDevice, stream));
+        HCHK(hipMemcpyAsync(ext.c, st->band_c, sizeof(uint32_t) * (size_t)nb, hipMemcpyHostToDevice, stream));
+        for (int a = 0; a < 3; a++) {
+            HCHK(hipMemcpyAsync(ext.x[a], st->band_x + (size_t)a * nb, wb, hipMemcpyHostToDevice, stream));
+            HCHK(hipMemcpyAsync(ext.xpred[a], st->band_xpred + (size_t)a * nb, wb, hipMemcpyHostToDevice, stream));
+            HCHK(hipMemsetAsync(ext.detF[a], 0, wb, stream));
+            HCHK(hipMemsetAsync(ext.en[a], 0, wb, stream));
+        }
+        for (int a = 0; a < 18; a++) HCHK(hipMemsetAsync(ext.fx[a], 0, wb, stream));
+        for (int a = 0; a < 9; a++)
+            HCHK(hipMemcpyAsync(ext.gphi[a], nc.data(), wb, hipMemcpyHostToDevice, stream));
+    }
+    HCHK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// the branch counters of the last step (Phys::counters)
+int Sim::step_counters(int out[3]) {
+    HCHK(hipMemcpyAsync(h_int, P.counters, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHK(hipStreamSynchronize(stream));
+    for (int a = 0; a < 3; a++) out[a] = h_int[a];
     return 0;
 }
 

@@ -162,6 +162,8 @@ class Sim {
     int upload(const rmt3d_state *st);
     int download(rmt3d_state *st);
     int band_count();
+    int upload_band(const rmt3d_state *st);
+    int step_counters(int out[3]);
     int mg_solve(int which, const double *rhs, double *x, double tol, int *ncyc, double *res);
     int write_checkpoint(int snum, int fnum, const char *dir);
     // tracers (Utilities/tracers.{hh,cc})

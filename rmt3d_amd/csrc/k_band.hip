@@ -767,6 +767,7 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             for (int a = 0; a < 3; a++) { px[a] = 0; pp[a] = 0; }
             changed = true;
             cleared = true;
+            atomicAdd(&P.counters[1], 1);
         } else {
             occupied = true;
         }
@@ -786,6 +787,7 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             for (int a = 0; a < 3; a++) { px[a] = lx[l][a]; pp[a] = lp[l][a]; }
             take = lsrc[l];
             occupied = true;
+            atomicAdd(&P.counters[2], 1);
             changed = true;
             cleared = false;
             int j0 = -1;

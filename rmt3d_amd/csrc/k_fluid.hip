@@ -223,6 +223,7 @@ __device__ __forceinline__ void collision_stress(const Geo &g, const Phys &P, co
             double avg = (cs[0] + cs[3] + cs[5]) / 3.;
             for (int a = 0; a < 3; a++) cs[symind(a, a)] -= avg;
             for (int a = 0; a < 3; a++) fs[a] += cs[symind(F, a)];
+            atomicAdd(&P.counters[0], 1);
         }
     }
 }

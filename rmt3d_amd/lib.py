@@ -215,6 +215,11 @@ class Simulation:
         self._check(lib().rmt3d_diag(self.h, which, C.byref(out)), "diag")
         return out.value
 
+    def step_counters(self):
+        """Branch counters of the last step: (collision contributions, primary
+        maps redrawn into the band, band maps redrawn onto the primary grid)."""
+        return tuple(int(self.diag(w)) for w in (20, 21, 22))
+
     def div_u(self):
         return self.diag(0)
 
@@ -260,6 +265,20 @@ class Simulation:
                 setattr(st, k, _dp(a))
             keep.append(a)
         self._check(lib().rmt3d_upload_state(self.h, C.byref(st)), "upload")
+
+    def set_band(self, cell, c, x, xpred):
+        """Replace the band (cells ascending; x, xpred as (3, n))."""
+        st = _abi.State()
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        c = np.ascontiguousarray(c, dtype=np.uint32)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        xpred = np.ascontiguousarray(xpred, dtype=np.float64)
+        st.nband = len(cell)
+        st.band_cell = cell.ctypes.data_as(C.POINTER(C.c_int32))
+        st.band_c = c.ctypes.data_as(C.POINTER(C.c_uint32))
+        st.band_x = _dp(x)
+        st.band_xpred = _dp(xpred)
+        self._check(lib().rmt3d_upload_state(self.h, C.byref(st)), "upload band")
 
     # ---- diagnostics (stepper::collect_macro_data) ----
     def macro_data(self):
