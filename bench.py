@@ -182,6 +182,10 @@ def run_gpu(args, precision):
     fields = sim.download(band=False)
     res = {"value": cells * args.steps / elapsed, "elapsed": elapsed, "roofline": roof, "vcycle": vc,
            "mac_cycles": sim.diag(1), "fem_cycles": sim.diag(2), "dt": sim.dt,
+           "autotuned": {"mac_passes_per_launch": sim.diag(16), "fem_passes_per_launch": sim.diag(17),
+                         "mac_colour_pass_workgroup": sim.diag(24), "fem_colour_pass_workgroup": sim.diag(25),
+                         "mac_fdm_outputs_per_item_xyz": int(sim.diag(26)),
+                         "fem_fdm_outputs_per_item_xyz": int(sim.diag(27))},
            "stress_ms_per_step": prof["stress"][0] / max(args.kernel_steps, 1), "stage_ms_per_step": stages,
            "fields": {k: fields[k] for k in ("u", "v", "w", "p")}}
     sim.close()
@@ -433,7 +437,7 @@ def main():
     config = {"workload": "stretched_sphere_%d" % args.n, "grid": [args.n] * 3, "objects": 1,
               "lambda": 1.3, "G": 1.0, "mu": 0.01, "nlayers": 4, "precision": head,
               "dt": g["dt"], "mac_vcycles_last_step": g["mac_cycles"],
-              "fem_vcycles_last_step": g["fem_cycles"], "vcycle": g["vcycle"],
+              "fem_vcycles_last_step": g["fem_cycles"], "vcycle": g["vcycle"], "autotuned": g["autotuned"],
               "stress_ms_per_step": round(g["stress_ms_per_step"], 3),
               "stage_ms_per_step": g["stage_ms_per_step"]}
     if len(runs) == 2:

@@ -225,4 +225,7 @@ constexpr int ST_FATAL_REDRAW = 8;    // two deep maps
 constexpr int ST_BAND_OVERFLOW = 16;  // band capacity exceeded
 constexpr int ST_FATAL_INIT = 32;     // overlapping primaries
 
+// words (128 bytes apart) taking the per-wavefront maxima of |u| at init
+constexpr int INIT_VMAX_SLOTS = 32;
+
 }  // namespace rmt

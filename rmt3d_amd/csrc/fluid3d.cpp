@@ -1,4 +1,5 @@
 // Host orchestration of the MI355X FSI step (see fluid3d.h).
+#include <map>
 #include <mutex>
 #include "fluid3d.h"
 #include <float.h>
@@ -372,7 +373,8 @@ double MGSolver::resid_max(hipStream_t s) {
 // residual array (free while smoothing); an odd count ends with a copy back
 // into x.  Per-kernel profiling stamps each launch's own start and end.
 template <typename T>
-static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3], int np, int nsweeps, KProf *kp) {
+static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3], int np, int nsweeps, KProf *kp,
+                   int bs = 0) {
     int ncol = fem ? 4 : 2;
     // a halo no wider than a periodic extent wraps at most once
     for (int a = 0; a < 3; a++)
@@ -384,7 +386,7 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (kp) kp->pair(e0, e1);
             if (np == 1) {
-                launch_mg_smooth(s, fem, l, prd, c0, e0, e1);
+                launch_mg_smooth(s, fem, l, prd, c0, e0, e1, bs);
                 continue;
             }
             SweepPlan P = mg_sweep_plan(fem, l.d, prd, (int)sizeof(T), std::min(np, ncol - c0));
@@ -397,6 +399,134 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
     if (m & 1) launch_mg_copy(s, (const T *)l.r, l.x, l.nn);
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner of the multigrid launch shapes.  The reference picks its
+// V-cycle schedule at run time by measurement (mg_opt searches (v_up,
+// v_down), mg_probs.hh:80-208, off by default).  Here the smoothing counts
+// stay the reference's 2 + 2 and what is searched is how the same arithmetic
+// is laid out on the GPU, level by level, in the precision the V-cycles run
+// in:
+//   - per smoothing level above the tail: colour passes per launch (1, 2 or
+//     4 FEM; 1 or 2 MAC) and, for single-pass launches, the workgroup size
+//     (64 ... 1024) of the colour-pass kernel;
+//   - per dimension of a fast-diagonalisation level: outputs per work-item
+//     (2, 4, 8) of its contractions.
+// Every candidate performs the same operations in the same order on every
+// unknown (the fused sweeps are bit-identical to single-colour passes), so a
+// choice changes time only, never results.  Each candidate runs a few times
+// on the level's own arrays (zero at set-up, zeroed again afterwards), timed
+// with events on the context stream; the minimum over three interleaved
+// rounds decides.  Choices are cached per operator, level shape and
+// precision for the life of the process, so contexts of one shape (a serving
+// mix) tune once.  RMT3D_TUNE=0 keeps the built-in defaults; RMT3D_SWEEP, when
+// set, fixes the passes per launch.
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneChoice {
+    int npass[16];
+    int bs[16];
+    int pb[3];
+};
+std::mutex g_tune_mu;
+std::map<std::vector<int>, TuneChoice> g_tune_cache;
+
+// minimum over rounds of the time of `reps` runs of op(c), per candidate c
+template <typename Op>
+std::vector<double> time_candidates(hipStream_t s, int ncand, Op op) {
+    const int reps = 3, rounds = 3;
+    std::vector<double> best(ncand, 1e30);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return best;
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return best; }
+    for (int c = 0; c < ncand; c++) op(c);   // warm-up (code objects, caches)
+    for (int r = 0; r < rounds; r++)
+        for (int c = 0; c < ncand; c++) {
+            hipEventRecord(e0, s);
+            for (int k = 0; k < reps; k++) op(c);
+            hipEventRecord(e1, s);
+            float ms = 0;
+            if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                best[c] = std::min(best[c], (double)ms);
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return best;
+}
+
+int argmin(const std::vector<double> &v) { return (int)(std::min_element(v.begin(), v.end()) - v.begin()); }
+}  // namespace
+
+template <typename T>
+void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool sweep_fixed) {
+    const int ncol = fem ? 4 : 2;
+    int last = fdm ? nlev - 1 : tail_lev;   // levels [0, last) are smoothed by launches
+    for (int l = 0; l < last; l++) {
+        // candidates: (np = 1, bs) for each workgroup size, then np = 2 (, 4)
+        std::vector<std::pair<int, int>> cand;
+        static const int sizes[5] = {64, 128, 256, 512, 1024};
+        for (int b : sizes) cand.push_back({1, b});
+        if (!sweep_fixed)
+            for (int np = 2; np <= ncol; np *= 2) cand.push_back({np, 0});
+        else if (npass[l] > 1)
+            cand.assign(1, {npass[l], 0});
+        std::vector<double> t = time_candidates(s, (int)cand.size(), [&](int c) {
+            smooth(s, fem, LV[l], prd, cand[c].first, 1, (KProf *)nullptr, cand[c].second);
+        });
+        int w = argmin(t);
+        npass[l] = cand[w].first;
+        smooth_bs[l] = cand[w].second;
+    }
+    if (fdm) {
+        MGLevel<T> &c = LV[nlev - 1];
+        for (int a = 0; a < 3; a++) {
+            static const int pbs[3] = {2, 4, 8};
+            std::vector<double> t = time_candidates(s, 3, [&](int k) {
+                launch_mg_fdm_pass<T>(s, c, a, pbs[k], F.S[a], a == 2 ? F.dinv : nullptr, c.b, c.r);
+            });
+            F.pb[a] = pbs[argmin(t)];
+        }
+    }
+    for (int l = 0; l < nlev; l++) {
+        hipMemsetAsync(LV[l].x, 0, sizeof(T) * LV[l].nn, s);
+        hipMemsetAsync(LV[l].b, 0, sizeof(T) * LV[l].nn, s);
+        hipMemsetAsync(LV[l].r, 0, sizeof(T) * LV[l].nn, s);
+    }
+}
+
+void MGSolver::tune(hipStream_t s) {
+    const char *tv = getenv("RMT3D_TUNE");
+    if (tv && tv[0] == '0') return;
+    bool sweep_fixed = getenv("RMT3D_SWEEP") != nullptr;
+    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, nlev, fdm ? 1 : 0, prd[0], prd[1], prd[2]};
+    for (int l = 0; l < nlev; l++)
+        for (int a = 0; a < 3; a++) key.push_back(L[l].d[a]);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            for (int l = 0; l < 16; l++) { npass[l] = it->second.npass[l]; smooth_bs[l] = it->second.bs[l]; }
+            for (int a = 0; a < 3; a++) fdm64.pb[a] = fdm32.pb[a] = it->second.pb[a];
+            tuned = 2;
+            return;
+        }
+    }
+    if (mixed) {
+        tune_levels<float>(s, Lf, fdm32, sweep_fixed);
+        for (int a = 0; a < 3; a++) fdm64.pb[a] = fdm32.pb[a];
+    } else {
+        tune_levels<double>(s, L, fdm64, sweep_fixed);
+        for (int a = 0; a < 3; a++) fdm32.pb[a] = fdm64.pb[a];
+    }
+    hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    TuneChoice ch;
+    for (int l = 0; l < 16; l++) { ch.npass[l] = npass[l]; ch.bs[l] = smooth_bs[l]; }
+    for (int a = 0; a < 3; a++) ch.pb[a] = fdm64.pb[a];
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = ch;
+    tuned = 1;
+}
+
 void MGSolver::vcycle64(hipStream_t s, int lev) {
     if (fdm && lev == nlev - 1) {
         launch_mg_fdm(s, L[lev], fdm64);
@@ -407,12 +537,12 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
-    smooth(s, fem, L[lev], prd, npass[lev], v_down, kp);
+    smooth(s, fem, L[lev], prd, npass[lev], v_down, kp, smooth_bs[lev]);
     launch_mg_resid(s, fem, L[lev], prd);
     launch_mg_restrict(s, fem, L[lev], L[lev + 1], prd);
     vcycle64(s, lev + 1);
     launch_mg_prolong(s, fem, L[lev], L[lev + 1], prd);
-    smooth(s, fem, L[lev], prd, npass[lev], v_up, kp);
+    smooth(s, fem, L[lev], prd, npass[lev], v_up, kp, smooth_bs[lev]);
 }
 
 void MGSolver::vcycle32(hipStream_t s, int lev) {
@@ -425,12 +555,12 @@ void MGSolver::vcycle32(hipStream_t s, int lev) {
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
-    smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp);
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp, smooth_bs[lev]);
     launch_mg_resid(s, fem, Lf[lev], prd);
     launch_mg_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
     vcycle32(s, lev + 1);
     launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
-    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp);
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev]);
 }
 
 // one iteration: an FP64 V-cycle, or (mixed) one FP64 defect-correction step
@@ -867,7 +997,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     P.counters = d_status + 1;
     d_solid_list = (int *)dalloc(sizeof(int) * Ng);
     d_solid_n = (int *)dalloc(sizeof(int));
-    d_u64 = (unsigned long long *)dalloc(sizeof(unsigned long long) * 4);
+    d_u64 = (unsigned long long *)dalloc(sizeof(unsigned long long) * 16 * INIT_VMAX_SLOTS);
     d_partial = (double *)dalloc(sizeof(double) * 1024);
     d_diag = (double *)dalloc(sizeof(double) * diag_scratch_doubles(n_obj));
     d_mean = (double *)dalloc(sizeof(double) * 4);
@@ -882,7 +1012,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     if (dalloc_failed) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
     HCHK(hipHostMalloc((void **)&h_status, sizeof(int) * 4, hipHostMallocDefault));
     HCHK(hipHostMalloc((void **)&h_int, sizeof(int) * 16, hipHostMallocDefault));
-    HCHK(hipHostMalloc((void **)&h_dbl, sizeof(double) * 16, hipHostMallocDefault));
+    HCHK(hipHostMalloc((void **)&h_dbl, sizeof(double) * 16 * INIT_VMAX_SLOTS, hipHostMallocDefault));
     // initial gphi = -1e200 for all primary maps (ref_map ctor, fields.hh:324-352)
     {
         std::vector<double> neg((size_t)Ng, -1.e200);
@@ -894,6 +1024,10 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     fem.setup(1, df, g.prd, mixed);
     if (dalloc_failed) { err = "hipMalloc failed"; return RMT3D_ERR_NOMEM; }
     if (mac.alloc_failed || fem.alloc_failed) { err = "hipMalloc failed (multigrid levels)"; return RMT3D_ERR_NOMEM; }
+    // the level arrays' zero fills ran on the null stream
+    HCHK(hipStreamSynchronize(nullptr));
+    mac.tune(stream);
+    fem.tune(stream);
     dev_bytes += mac.bytes + fem.bytes;
     if (mixed) {
         // defect correction stops on the FP64 residual relative to the rhs
@@ -986,13 +1120,19 @@ void Sim::cfl_recheck() {
 
 void Sim::init_fluid() {
     P.time = time;
-    launch_zero_u64(stream, d_u64, 1);
+    launch_zero_u64(stream, d_u64, 16 * INIT_VMAX_SLOTS);
     launch_init_fluid(stream, g, P, d_obj, init, U, d_u64);
     comm_fluid(ORC_VEL);
-    unsigned long long bits;
-    hipMemcpyAsync(h_dbl, d_u64, sizeof(bits), hipMemcpyDeviceToHost, stream);
+    hipMemcpyAsync(h_dbl, d_u64, sizeof(unsigned long long) * 16 * INIT_VMAX_SLOTS, hipMemcpyDeviceToHost, stream);
     hipStreamSynchronize(stream);
-    memcpy(&bits, h_dbl, sizeof bits);
+    // the slots hold bit patterns of non-negative doubles (0 where no wavefront
+    // wrote), which order like the values
+    unsigned long long bits = 0;
+    for (int a = 0; a < INIT_VMAX_SLOTS; a++) {
+        unsigned long long b;
+        memcpy(&b, (const char *)h_dbl + sizeof(b) * 16 * a, sizeof b);
+        if (b > bits) bits = b;
+    }
     double m;
     memcpy(&m, &bits, sizeof m);
     if (m > vmax) vmax = m;   // sim_manager::velocity (sim_manager.cc:129-130)

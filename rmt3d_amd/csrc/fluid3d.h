@@ -54,6 +54,8 @@ struct MGSolver {
     int tail_lev = 0;                      // first level run by the one-workgroup tail kernel
     int tail_lds64 = 0, tail_lds32 = 0;    // tail levels held in LDS
     int npass[16];                         // colour passes per smoothing launch, per level
+    int smooth_bs[16] = {};                // workgroup size of single-pass launches (0: default)
+    int tuned = 0;                         // 1 tuned by measurement, 2 taken from the cache
     // mg_pred (mg_probs.hh:37-78)
     int iP = 16, iV = 32;
     double V_avg = 32, V_var = 1e6, alpha = 0;
@@ -75,6 +77,11 @@ struct MGSolver {
     double tol_override = 0;               // > 0 replaces max_norm_tol
 
     void setup(int fem_, const int d[3], const int prd_[3], bool mixed_);
+    // picks launch shapes by timing the candidates on the levels (or from
+    // the per-process cache of that level shape)
+    void tune(hipStream_t s);
+    template <typename T>
+    void tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool sweep_fixed);
     void release();
     double resid_max(hipStream_t s);
     // residual check in two halves: queue the max-norm and its read-back,

@@ -1017,56 +1017,67 @@ __device__ __forceinline__ void obj_rm0(const InitC &I, const ObjC *O, int o, co
         for (int a = 0; a < 3; a++) xi[a] = X[a];
 }
 
-// init_fluid (fluid_3d.cc:292-318); vmax is a max over |u| (atomicMax on bits)
-__global__ void k_init_fluid(Geo g, Phys P, const ObjC *O, InitC I, Fluid U, unsigned long long *vmax_bits) {
+// init_fluid (fluid_3d.cc:292-318); vmax is the maximum of |u| over the cells
+// the reference tracks.  Each wavefront reduces its lanes' values with
+// shuffles (every lane reaches the reduction, out-of-range lanes with no
+// value) and adds one atomicMax on the bit pattern (|u| >= 0, so the bits
+// order like the values) to one of INIT_VMAX_SLOTS words 128 bytes apart,
+// picked by the row; the host takes the maximum of the slots.
+__global__ void k_init_fluid(Geo g, Phys P, const ObjC *O, InitC I, Fluid U, unsigned long long *vmax_slots) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m4) return;
-    int c = IX(g, i, j, k);
-    double x = g.lx[i], y = g.ly[j], z = g.lz[k];
-    double u = 0, v = 0, w = 0;
-    const double PI = 3.14159265358979323846;
-    bool track = true;
-    if (P.sim_type == 0 || P.sim_type == 3) {
-        ftest_vel(x, y, z, u, v, w);
-        double t = sqrt(u * u + v * v + w * w);
-        atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(t));
-        track = false;
-        if (I.has_exact) {
-            u *= det_cos(2 * PI * P.time); v *= det_cos(2 * PI * P.time); w *= det_cos(2 * PI * P.time);
-        }
-    } else if (I.has_exact) {
-        track = false;
-    } else {
-        for (int a = 0; a < I.vel_prof_num; a++) {
-            int vt = (int)I.vel_profile[2 * a];
-            double amp = I.vel_profile[2 * a + 1];
-            if (vt == 1) u += amp;
-            else if (vt == 2) v += amp;
-            else if (vt == 3) w += amp;
-        }
-        double X[3] = {x, y, z};
-        for (int o = 0; o < P.n_obj; o++) {
-            double rx[3];
-            obj_rm0(I, O, o, X, rx);
-            if (obj_phi(O, P.n_obj, o, rx[0], rx[1], rx[2]) < 0.) {
-                double L[3];
-                for (int a = 0; a < 3; a++) L[a] = rx[a] - O[o].c[a];
-                u += I.omega[o][1] * L[2] - I.omega[o][2] * L[1];
-                v -= I.omega[o][0] * L[2] - I.omega[o][2] * L[0];
-                w += I.omega[o][0] * L[1] - I.omega[o][1] * L[0];
+    bool has = false;
+    double speed = 0;
+    if (i < g.m4) {
+        int c = IX(g, i, j, k);
+        double x = g.lx[i], y = g.ly[j], z = g.lz[k];
+        double u = 0, v = 0, w = 0;
+        const double PI = 3.14159265358979323846;
+        if (P.sim_type == 0 || P.sim_type == 3) {
+            ftest_vel(x, y, z, u, v, w);
+            speed = sqrt(u * u + v * v + w * w);
+            has = true;
+            if (I.has_exact) {
+                u *= det_cos(2 * PI * P.time); v *= det_cos(2 * PI * P.time); w *= det_cos(2 * PI * P.time);
             }
+        } else if (!I.has_exact) {
+            for (int a = 0; a < I.vel_prof_num; a++) {
+                int vt = (int)I.vel_profile[2 * a];
+                double amp = I.vel_profile[2 * a + 1];
+                if (vt == 1) u += amp;
+                else if (vt == 2) v += amp;
+                else if (vt == 3) w += amp;
+            }
+            double X[3] = {x, y, z};
+            for (int o = 0; o < P.n_obj; o++) {
+                double rx[3];
+                obj_rm0(I, O, o, X, rx);
+                if (obj_phi(O, P.n_obj, o, rx[0], rx[1], rx[2]) < 0.) {
+                    double L[3];
+                    for (int a = 0; a < 3; a++) L[a] = rx[a] - O[o].c[a];
+                    u += I.omega[o][1] * L[2] - I.omega[o][2] * L[1];
+                    v -= I.omega[o][0] * L[2] - I.omega[o][2] * L[0];
+                    w += I.omega[o][0] * L[1] - I.omega[o][1] * L[0];
+                }
+            }
+            speed = sqrt(u * u + v * v + w * w);
+            has = true;
         }
+        U.vel[0][c] = u; U.vel[1][c] = v; U.vel[2][c] = w;
+        U.q[c] = 0; U.dp[c] = 0;
+        for (int a = 0; a < 3; a++) U.dvel[a][c] = 0;
+        for (int a = 0; a < 18; a++) U.fvel[a][c] = 0;
+        for (int a = 0; a < 9; a++) U.sigma[a][c] = 0;
     }
-    U.vel[0][c] = u; U.vel[1][c] = v; U.vel[2][c] = w;
-    U.q[c] = 0; U.dp[c] = 0;
-    for (int a = 0; a < 3; a++) U.dvel[a][c] = 0;
-    for (int a = 0; a < 18; a++) U.fvel[a][c] = 0;
-    for (int a = 0; a < 9; a++) U.sigma[a][c] = 0;
-    if (track) {
-        double t = sqrt(u * u + v * v + w * w);
-        atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(t));
+    double m = has ? speed : -1.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
     }
+    int slot = (blockIdx.y + blockIdx.z * gridDim.y) % INIT_VMAX_SLOTS;
+    if (__lane_id() == 0 && m >= 0)
+        atomicMax(vmax_slots + 16 * slot, (unsigned long long)__double_as_longlong(m));
 }
 
 // init_pres (fluid_3d.cc:326-340)

@@ -981,21 +981,24 @@ static inline Prd mkprd(const int p[3]) { Prd P; P.p[0] = p[0]; P.p[1] = p[1]; P
 
 template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color, hipEvent_t start,
-                      hipEvent_t stop) {
-    // workgroup sizes measured on the 128^3 / 129^3 levels: 1024 for the
-    // red-black pass (7.8 us against 8.1 at 256, FP32; 8.8 against 10.0,
+                      hipEvent_t stop, int bs) {
+    // default workgroup sizes, measured on the 128^3 / 129^3 levels: 1024 for
+    // the red-black pass (7.8 us against 8.1 at 256, FP32; 8.8 against 10.0,
     // FP64); 64 for the FEM colour pass, whose wall-row work-items cost
     // several times an interior one and finish last, so small workgroups
-    // spread them over more CUs (6.9 us against 8.9 at 256, FP32)
+    // spread them over more CUs (6.9 us against 8.9 at 256, FP32).  The
+    // autotuner passes the size it measured fastest on the level at hand.
     Prd P = mkprd(prd);
+    if (bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024) bs = fem ? 64 : 1024;
     if (!fem) {
         int hx = (L.d[0] + 1) / 2, n = hx * L.d[1] * L.d[2];
-        hipExtLaunchKernelGGL(k_mac_smooth<T>, dim3((n + 1023) / 1024), dim3(1024), 0, s, start, stop, 0, L, P,
+        hipExtLaunchKernelGGL(k_mac_smooth<T>, dim3((n + bs - 1) / bs), dim3(bs), 0, s, start, stop, 0, L, P,
                               color, hx, 1.0f / hx, 1.0f / L.d[1]);
     } else {
         FemColour F = fem_colour(L.d, color);
         int n = F.NI + F.NA + F.NB;
-        hipExtLaunchKernelGGL(k_fem_smooth<T>, dim3((n + 63) / 64), dim3(64), 0, s, start, stop, 0, L, P, color, F);
+        hipExtLaunchKernelGGL(k_fem_smooth<T>, dim3((n + bs - 1) / bs), dim3(bs), 0, s, start, stop, 0, L, P, color,
+                              F);
     }
 }
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
@@ -1201,17 +1204,16 @@ void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam) {
 
 // out[.., p, ..] = sum over q of W[q*d + p] in[.., q, ..] along one
 // dimension, each output summed over ascending q from +0.  A wavefront
-// computes FDM_PB consecutive outputs p for 64 pencils (the same p for all
+// computes PB consecutive outputs p for 64 pencils (the same p for all
 // lanes, so its W entries are LDS broadcasts).  The kernels are bound by
 // memory latency, not bandwidth or arithmetic: each workgroup issues all the
-// loads it stages into LDS (W's fdm_rows(da) x FDM_PB slices, and for x the
+// loads it stages into LDS (W's fdm_rows(da) x PB slices, and for x the
 // row tile) as one batch; along y and z it streams pencil values from memory
 // FDM_QU at a time, double-buffered.
-constexpr int FDM_PB = 8;
 constexpr int FDM_QU = 32;
 
 // W slice element e of output block p0: W[q*da + p0 + min(r, last)] with
-// q = e / FDM_PB, r = e % FDM_PB (columns past the end repeat column da-1,
+// q = e / PB, r = e % PB (columns past the end repeat column da-1,
 // unused); rows q >= da are zero.  The terms run in branch-free chunks of
 // FDM_QU, then of 4: the slice has fdm_rows(da) rows (da rounded up to a
 // multiple of 4) and pencil values past da are zero too.  The extra terms
@@ -1219,21 +1221,21 @@ constexpr int FDM_QU = 32;
 // never -0).
 __host__ __device__ constexpr int fdm_rows(int da) { return (da + 3) / 4 * 4; }
 
-template <typename T>
+template <int PB, typename T>
 __device__ __forceinline__ T fdm_w_elem(const T *__restrict__ W, int da, int p0, int e) {
-    int q = e / FDM_PB, r = e - q * FDM_PB;
+    int q = e / PB, r = e - q * PB;
     int last = da - 1 - p0;
     T w = W[min(q, da - 1) * da + p0 + (r < last ? r : last)];
     return q < da ? w : T(0);
 }
 
-template <int U, typename T>
-__device__ __forceinline__ void fdm_terms(T (&acc)[FDM_PB], const T *v, const T *wsl, int q0) {
+template <int U, int PB, typename T>
+__device__ __forceinline__ void fdm_terms(T (&acc)[PB], const T *v, const T *wsl, int q0) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        const T *w = wsl + (q0 + u) * FDM_PB;
+        const T *w = wsl + (q0 + u) * PB;
 #pragma unroll
-        for (int r = 0; r < FDM_PB; r++) acc[r] += w[r] * v[u];
+        for (int r = 0; r < PB; r++) acc[r] += w[r] * v[u];
     }
 }
 
@@ -1241,14 +1243,14 @@ __device__ __forceinline__ void fdm_terms(T (&acc)[FDM_PB], const T *v, const T 
 // (coalesced loads and stores), blockIdx.y the output block.  Pencil c
 // starts at (c % inner) + (c / inner) * ostride and steps by stride; SCALE
 // multiplies each output by dinv.
-template <typename T, bool SCALE>
+template <typename T, bool SCALE, int PB>
 __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__restrict__ out,
                                                  const T *__restrict__ W, const T *__restrict__ dinv, int ncols,
                                                  int inner, int ostride, int da, int stride) {
     extern __shared__ unsigned char fdm_lds[];
     T *wsl = reinterpret_cast<T *>(fdm_lds);
-    const int p0 = blockIdx.y * FDM_PB;
-    const int nw = fdm_rows(da) * FDM_PB;
+    const int p0 = blockIdx.y * PB;
+    const int nw = fdm_rows(da) * PB;
     const int c = min((int)(blockIdx.x * blockDim.x + threadIdx.x), ncols - 1);
     const int co = c / inner;
     const int base = (c - co * inner) + co * ostride;
@@ -1263,7 +1265,7 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
     for (int e0 = 0; e0 < nw; e0 += 64 * FDM_QU) {
         T wv[FDM_QU];
 #pragma unroll
-        for (int u = 0; u < FDM_QU; u++) wv[u] = fdm_w_elem(W, da, p0, min(e0 + u * 64 + (int)threadIdx.x, nw - 1));
+        for (int u = 0; u < FDM_QU; u++) wv[u] = fdm_w_elem<PB>(W, da, p0, min(e0 + u * 64 + (int)threadIdx.x, nw - 1));
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) {
             int e = e0 + u * 64 + threadIdx.x;
@@ -1271,14 +1273,14 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
         }
     }
     __syncthreads();
-    T acc[FDM_PB];
+    T acc[PB];
 #pragma unroll
-    for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
+    for (int r = 0; r < PB; r++) acc[r] = T(0);
     for (int q0 = 0; q0 < dfull; q0 += FDM_QU) {
         if (q0 + FDM_QU < dfull)
 #pragma unroll
             for (int u = 0; u < FDM_QU; u++) vn[u] = in[base + (q0 + FDM_QU + u) * stride];
-        fdm_terms<FDM_QU>(acc, v, wsl, q0);
+        fdm_terms<FDM_QU, PB>(acc, v, wsl, q0);
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) v[u] = vn[u];
     }
@@ -1289,11 +1291,11 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
             T x = in[base + min(q0 + u, da - 1) * stride];
             t4[u] = q0 + u < da ? x : T(0);
         }
-        fdm_terms<4>(acc, t4, wsl, q0);
+        fdm_terms<4, PB>(acc, t4, wsl, q0);
     }
     if (blockIdx.x * blockDim.x + threadIdx.x >= (unsigned)ncols) return;
 #pragma unroll
-    for (int r = 0; r < FDM_PB; r++)
+    for (int r = 0; r < PB; r++)
         if (p0 + r < da) {
             int e = base + (p0 + r) * stride;
             T o = acc[r];
@@ -1306,12 +1308,12 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
 // (pitch d0 + 1) and the W slices of nwv output blocks, in one batch of
 // loads; wavefront w computes block blockIdx.y * nwv + w for all rows (lane
 // = row) and stores it straight to the rows.
-template <typename T>
+template <typename T, int PB>
 __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *__restrict__ out,
                                                   const T *__restrict__ W, int d0, int nrows, int rb) {
     extern __shared__ unsigned char fdm_lds[];
-    const int nwv = blockDim.x >> 6, npb = (d0 + FDM_PB - 1) / FDM_PB;
-    const int nw = fdm_rows(d0) * FDM_PB;         // one block's W slice
+    const int nwv = blockDim.x >> 6, npb = (d0 + PB - 1) / PB;
+    const int nw = fdm_rows(d0) * PB;         // one block's W slice
     T *wsl = reinterpret_cast<T *>(fdm_lds);      // nwv slices
     T *tile = wsl + nwv * nw;
     const int ld = d0 + 1;
@@ -1328,7 +1330,7 @@ __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *_
             int e = min(e0 + u * (int)blockDim.x + (int)threadIdx.x, ntot - 1);
             if (e < nbw * nw) {
                 int b = e / nw;
-                v[u] = fdm_w_elem(W, d0, (pb0 + b) * FDM_PB, e - b * nw);
+                v[u] = fdm_w_elem<PB>(W, d0, (pb0 + b) * PB, e - b * nw);
             } else {
                 v[u] = src[e - nbw * nw];
             }
@@ -1347,13 +1349,13 @@ __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *_
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wv >= nbw) return;
-    const int p0 = (pb0 + wv) * FDM_PB;
+    const int p0 = (pb0 + wv) * PB;
     const T *ws = wsl + wv * nw;
     const int rl = lane < nr ? lane : nr - 1;     // idle lanes repeat the last row, unstored
     const T *row = tile + rl * ld;
-    T acc[FDM_PB];
+    T acc[PB];
 #pragma unroll
-    for (int r = 0; r < FDM_PB; r++) acc[r] = T(0);
+    for (int r = 0; r < PB; r++) acc[r] = T(0);
     // row values come from LDS; here a guarded chunk of FDM_QU terms
     // measured faster than the zero-padded chunks of the y and z passes
     for (int q0 = 0; q0 < d0; q0 += FDM_QU) {
@@ -1363,65 +1365,82 @@ __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *_
 #pragma unroll
         for (int u = 0; u < FDM_QU; u++) {
             if (q0 + u < d0) {
-                const T *w = ws + (q0 + u) * FDM_PB;
+                const T *w = ws + (q0 + u) * PB;
 #pragma unroll
-                for (int r = 0; r < FDM_PB; r++) acc[r] += w[r] * v[u];
+                for (int r = 0; r < PB; r++) acc[r] += w[r] * v[u];
             }
         }
     }
     if (lane < nr) {
         T *dst = out + (size_t)(r0 + lane) * d0;
 #pragma unroll
-        for (int r = 0; r < FDM_PB; r++)
+        for (int r = 0; r < PB; r++)
             if (p0 + r < d0) dst[p0 + r] = acc[r];
     }
 }
 
-template <typename T>
-static void fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, const T *W, const T *dinv, const T *in, T *out) {
+template <typename T, int PB>
+static void fdm_pass_pb(hipStream_t s, const MGLevel<T> &L, int a, const T *W, const T *dinv, const T *in, T *out) {
     int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2];
     int da = L.d[a];
-    size_t wbytes = sizeof(T) * (size_t)fdm_rows(da) * FDM_PB;
+    size_t wbytes = sizeof(T) * (size_t)fdm_rows(da) * PB;
     if (a == 0) {
         // up to 4 output blocks and 64 rows per workgroup, within 64 KiB of LDS
-        int npb = (d0 + FDM_PB - 1) / FDM_PB;
+        int npb = (d0 + PB - 1) / PB;
         int nwv = npb < 4 ? npb : 4;
         while (nwv > 1 && nwv * wbytes > 32768) nwv--;
         int rb = 64;
         while (rb > 1 && nwv * wbytes + sizeof(T) * (size_t)rb * (d0 + 1) > 65536) rb /= 2;
         int nrows = d1 * d2;
         size_t lds = nwv * wbytes + sizeof(T) * (size_t)rb * (d0 + 1);
-        hipLaunchKernelGGL(k_fdm_rows<T>, dim3((nrows + rb - 1) / rb, (npb + nwv - 1) / nwv), dim3(64 * nwv), lds,
+        hipLaunchKernelGGL((k_fdm_rows<T, PB>), dim3((nrows + rb - 1) / rb, (npb + nwv - 1) / nwv), dim3(64 * nwv), lds,
                            s, in, out, W, d0, nrows, rb);
         return;
     }
     int ncols = L.nn / da;
     int inner = a == 1 ? d0 : d0 * d1, ostride = a == 1 ? d0 * d1 : 0, stride = a == 1 ? d0 : d0 * d1;
-    dim3 g((ncols + 63) / 64, (da + FDM_PB - 1) / FDM_PB);
+    dim3 g((ncols + 63) / 64, (da + PB - 1) / PB);
     if (dinv)
-        hipLaunchKernelGGL((k_fdm_cols<T, true>), g, dim3(64), wbytes, s, in, out, W, dinv, ncols, inner, ostride,
+        hipLaunchKernelGGL((k_fdm_cols<T, true, PB>), g, dim3(64), wbytes, s, in, out, W, dinv, ncols, inner, ostride,
                            da, stride);
     else
-        hipLaunchKernelGGL((k_fdm_cols<T, false>), g, dim3(64), wbytes, s, in, out, W, (const T *)nullptr, ncols,
+        hipLaunchKernelGGL((k_fdm_cols<T, false, PB>), g, dim3(64), wbytes, s, in, out, W, (const T *)nullptr, ncols,
                            inner, ostride, da, stride);
+}
+
+// pb: outputs per work-item (2, 4 or 8); every choice sums each output's
+// terms in the same order, so they give identical results
+template <typename T>
+static void fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, int pb, const T *W, const T *dinv, const T *in, T *out) {
+    if (pb == 2) fdm_pass_pb<T, 2>(s, L, a, W, dinv, in, out);
+    else if (pb == 4) fdm_pass_pb<T, 4>(s, L, a, W, dinv, in, out);
+    else fdm_pass_pb<T, 8>(s, L, a, W, dinv, in, out);
+}
+
+template <typename T>
+void launch_mg_fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, int pb, const T *W, const T *dinv, const T *in,
+                        T *out) {
+    fdm_pass<T>(s, L, a, pb, W, dinv, in, out);
 }
 
 // forward along x, y, z (the last scaled by D^+), back along z, y, x,
 // through the level's r and x arrays (b is left as it is)
 template <typename T>
 void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
-    fdm_pass<T>(s, L, 0, F.S[0], nullptr, L.b, L.r);
-    fdm_pass<T>(s, L, 1, F.S[1], nullptr, L.r, L.x);
-    fdm_pass<T>(s, L, 2, F.S[2], F.dinv, L.x, L.r);
-    fdm_pass<T>(s, L, 2, F.ST[2], nullptr, L.r, L.x);
-    fdm_pass<T>(s, L, 1, F.ST[1], nullptr, L.x, L.r);
-    fdm_pass<T>(s, L, 0, F.ST[0], nullptr, L.r, L.x);
+    fdm_pass<T>(s, L, 0, F.pb[0], F.S[0], nullptr, L.b, L.r);
+    fdm_pass<T>(s, L, 1, F.pb[1], F.S[1], nullptr, L.r, L.x);
+    fdm_pass<T>(s, L, 2, F.pb[2], F.S[2], F.dinv, L.x, L.r);
+    fdm_pass<T>(s, L, 2, F.pb[2], F.ST[2], nullptr, L.r, L.x);
+    fdm_pass<T>(s, L, 1, F.pb[1], F.ST[1], nullptr, L.x, L.r);
+    fdm_pass<T>(s, L, 0, F.pb[0], F.ST[0], nullptr, L.r, L.x);
 }
 
 #define INST(T)                                                                                              \
     template void launch_mg_fdm<T>(hipStream_t, const MGLevel<T> &, const FdmDev<T> &);                     \
     template void launch_mg_smooth<T>(hipStream_t, int, const MGLevel<T> &, const int *, int, hipEvent_t,    \
-                                      hipEvent_t);                                                           \
+                                      hipEvent_t, int);                                                      \
+    template void launch_mg_fdm_pass<T>(hipStream_t, const MGLevel<T> &, int, int, const T *, const T *,    \
+                                        const T *, T *);                                                     \
     template void launch_mg_sweep<T>(hipStream_t, int, const MGLevel<T> &, const int *, const SweepPlan &,   \
                                      const T *, T *, hipEvent_t, hipEvent_t);                                \
     template void launch_mg_copy<T>(hipStream_t, const T *, T *, int);                                      \

@@ -74,6 +74,7 @@ constexpr int MG_FDM_DMAX = 512;      // and at most this many per dimension
 template <typename T>
 struct FdmDev {
     const T *S[3], *ST[3], *dinv;
+    int pb[3] = {8, 8, 8};   // outputs per work-item of each dimension's passes (2, 4 or 8)
 };
 // host: the 1D basis of one dimension (fem 1: Q1 nodes, 0: MAC cells) and
 // its eigenvalues
@@ -81,6 +82,10 @@ void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam);
 // x = S D^+ S^T b on level L (six contractions through L.r and L.x)
 template <typename T>
 void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F);
+// one of those contractions: along dimension a, pb outputs per work-item
+template <typename T>
+void launch_mg_fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, int pb, const T *W, const T *dinv, const T *in,
+                        T *out);
 
 // the levels below a size threshold, processed by one workgroup
 constexpr int MG_TAIL_MAXLEV = 8;       // a level of <= 6144 unknowns coarsens at most 4 more times
@@ -118,7 +123,7 @@ void launch_mg_copy(hipStream_t s, const T *a, T *b, int n);
 // non-null start/stop events are stamped with the kernel's own start and end
 template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color,
-                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr, int bs = 0);
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]);
 template <typename T>

@@ -148,14 +148,14 @@ def test_projection_error_on_nan(prec):
             assert np.allclose(fg[k], fo[k], rtol=1e-9, atol=1e-12), k
 
 
-def _full_run(maker, n, steps, prec, **kw):
+def _full_run(maker, n, nsteps, prec, **kw):
     cfg, objs = maker(n, **kw)
     g, o, e = _init(cfg, objs, prec)
     _, dtg, _ = g.prepare_run(e)
     _, dto, _ = o.prepare_run(e)
     assert dtg == dto
     flips = 0
-    for s in range(steps):
+    for s in range(nsteps):
         rg, ro = g.step(), o.step()
         assert rg == ro == 0, (s, rg, ro)
         cg, co = g.step_counters(), o.step_counters()

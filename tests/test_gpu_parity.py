@@ -209,3 +209,32 @@ def test_shear_spheres_match_oracle():
     assert r["rm_c_equal"] and r.get("band_equal", True)
     for k in ("vel", "p"):
         assert r[k] < TOL, (k, r[k])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_autotuned_shapes_give_identical_states(prec, monkeypatch):
+    """The multigrid autotuner only changes launch shapes: a 64^3 cavity with
+    a sphere (fused sweeps and a fast-diagonalisation level) steps to the
+    same bits with the tuned shapes and with the built-in defaults."""
+    from rmt3d_amd import _abi
+    from rmt3d_amd.lib import Simulation
+    cfg, objs = configs.cavity_sphere(64, steps=10)
+    p = _abi.PREC_MIXED if prec else _abi.PREC_FP64
+    monkeypatch.setenv("RMT3D_TUNE", "0")
+    a = Simulation(cfg, objs, p)
+    monkeypatch.delenv("RMT3D_TUNE")
+    b = Simulation(cfg, objs, p)
+    print("tuned:", [b.diag(k) for k in (16, 17, 24, 25, 26, 27, 28, 29)],
+          "defaults:", [a.diag(k) for k in (16, 17, 24, 25, 26, 27, 28, 29)], flush=True)
+    assert a.diag(28) == 0 and a.diag(29) == 0
+    assert b.diag(28) in (1, 2) and b.diag(29) in (1, 2)
+    for s in (a, b):
+        e = s.initialize()
+        s.prepare_run(e)
+        for _ in range(3):
+            assert s.step() == 0
+    fa, fb = a.download(), b.download()
+    for k in ("u", "v", "w", "p", "rm_x", "band_x"):
+        assert np.array_equal(fa[k], fb[k]), k
+    a.close()
+    b.close()
