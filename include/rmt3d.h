@@ -202,6 +202,15 @@ int rmt3d_write_params(const rmt3d_config *cfg, const rmt3d_object *objs, const 
  * order (FEM node counts for the corner fields 3,4,5,7,18); *len is the
  * capacity on entry and the count on exit. */
 int rmt3d_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len);
+/* rmt3d_slice without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as rmt3d_slice, except
+ * that the slice is staged in a fixed per-GPU device scratch of 2^19 values
+ * (4 MiB, held until the process ends): a slice of more values returns
+ * RMT3D_ERR_ARG.  Calls on one GPU share that scratch, so calls that overlap
+ * on one GPU (from several threads or contexts) race; serialise them. */
+int rmt3d_slice_noalloc(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len);
 /* fluid_3d::write_slice (fluid_3d_io.cc:3-65): format 0 matrix text
  * (%17.14g), 1 coordinate text, 2 gnuplot binary (float). */
 int rmt3d_write_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, int format,

@@ -137,6 +137,9 @@ int rmt3d_centroids(rmt3d_ctx *ctx, double *out) { GUARD(ctx->sim.centroids(out)
 int rmt3d_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len) {
     GUARD(ctx->sim.slice(dim, point, o_type, obj_id, out, len));
 }
+int rmt3d_slice_noalloc(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, double *out, int *len) {
+    GUARD(ctx->sim.slice_noalloc(dim, point, o_type, obj_id, out, len));
+}
 int rmt3d_write_slice(rmt3d_ctx *ctx, int dim, int point, int o_type, int obj_id, int format, const char *filename) {
     GUARD(ctx->sim.write_slice(dim, point, o_type, obj_id, format, filename));
 }

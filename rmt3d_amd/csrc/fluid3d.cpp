@@ -1919,7 +1919,17 @@ int Sim::slice_dims(int dim, int t, int &d1, int &d2) const {
     return 0;
 }
 
-int Sim::slice(int dim, int point, int t, int obj_id, double *out, int *len) {
+int Sim::slice(int dim, int point, int t, int obj_id, double *out, int *len) { return slice_into(dim, point, t, obj_id, out, len, false); }
+
+int Sim::slice_noalloc(int dim, int point, int t, int obj_id, double *out, int *len) {
+    return slice_into(dim, point, t, obj_id, out, len, true);
+}
+
+// fluid_3d::slice on the device.  The slice is gathered into a device buffer
+// and copied out: a buffer allocated for the call, or (noalloc) the
+// library's fixed per-GPU scratch, in which case nothing is allocated and a
+// slice larger than the scratch is refused.
+int Sim::slice_into(int dim, int point, int t, int obj_id, double *out, int *len, bool noalloc) {
     int d1, d2;
     if (slice_dims(dim, t, d1, d2)) { err = "slice: bad dimension or output type"; return RMT3D_ERR_ARG; }
     size_t n = (size_t)d1 * d2;
@@ -1937,7 +1947,17 @@ int Sim::slice(int dim, int point, int t, int obj_id, double *out, int *len) {
         return 0;
     }
     double *d = nullptr;
-    HCHK(hipMalloc(&d, sizeof(double) * n));
+    if (noalloc) {
+        if (n > SLICE_SCRATCH_MAX) {
+            err = "slice_noalloc: " + std::to_string(n) + " values exceed the fixed scratch of " +
+                  std::to_string(SLICE_SCRATCH_MAX);
+            return RMT3D_ERR_ARG;
+        }
+        d = slice_scratch(err);
+        if (!d) return RMT3D_ERR_HIP;
+    } else {
+        HCHK(hipMalloc(&d, sizeof(double) * n));
+    }
     const double *r0 = nullptr;
     if (t == 18) {
         launch_pp_rhs(stream, g, U, gradphi, wall_faces.data(), (int)wall_faces.size(), d_faces, d_rhs_tmp);
@@ -1946,7 +1966,7 @@ int Sim::slice(int dim, int point, int t, int obj_id, double *out, int *len) {
     launch_slice(stream, g, d_obj, n_obj, U, maps(), r0, t, obj_id, dim, point, d1, d2, d);
     hipError_t he = hipMemcpyAsync(out, d, sizeof(double) * n, hipMemcpyDeviceToHost, stream);
     if (he == hipSuccess) he = hipStreamSynchronize(stream);
-    hipFree(d);
+    if (!noalloc) hipFree(d);
     HCHK(he);
     return 0;
 }

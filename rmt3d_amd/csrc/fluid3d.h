@@ -187,6 +187,8 @@ class Sim {
     int centroids(double *out);
     int slice_dims(int dim, int t, int &d1, int &d2) const;
     int slice(int dim, int point, int t, int obj_id, double *out, int *len);
+    int slice_noalloc(int dim, int point, int t, int obj_id, double *out, int *len);
+    int slice_into(int dim, int point, int t, int obj_id, double *out, int *len, bool noalloc);
     int write_slice(int dim, int point, int t, int obj_id, int format, const char *filename);
 
   private:

@@ -6,6 +6,10 @@
 // every access is a plain coalesced load or store (the work is HBM-bound
 // byte movement).
 #include <hip/hip_ext.h>
+
+#include <mutex>
+#include <string>
+
 #include "launch.h"
 
 namespace rmt {
@@ -213,6 +217,37 @@ __global__ void k_chk_reset(Geo g, Fluid U, RMap R) {
     for (int a = 0; a < 3; a++) { R.x[a][c] = 0; R.xpred[a][c] = 0; R.detF[a][c] = 0; R.en[a][c] = 0; }
     for (int a = 0; a < 18; a++) R.fx[a][c] = 0;
     for (int a = 0; a < 9; a++) R.gphi[a][c] = -1.e200;
+}
+
+// Device scratch of the allocation-free slice (rmt3d_slice_noalloc): one
+// fixed array per GPU, resident until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for
+// the symbol); it asks hipGetSymbolAddress for this device's copy once, with
+// that device current, and uses the returned address from then on.  Every
+// call on one GPU shares its array, so calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[SLICE_SCRATCH_MAX];
+
+double *slice_scratch(std::string &err) {
+    static std::mutex mu;
+    static double *ptr[SLICE_SCRATCH_DEVICES] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SLICE_SCRATCH_DEVICES) {
+        (void)hipGetLastError();
+        err = "slice_noalloc: no usable current device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            err = std::string("slice_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        ptr[dev] = (double *)p;
+    }
+    return ptr[dev];
 }
 
 void launch_slice(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,

@@ -199,6 +199,13 @@ void launch_band_x_eq_xpred(hipStream_t s, Band B);
 // out holds d1*d2 values; r0 is the pressure-Poisson rhs (type 18 only)
 void launch_slice(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,
                   const double *r0, int t, int obj_id, int dim, int point, int d1, int d2, double *out);
+// The allocation-free slice's fixed device scratch: at most SLICE_SCRATCH_MAX
+// values (4 MiB per GPU, a slice of up to 724 x 724 points) on devices
+// 0 .. SLICE_SCRATCH_DEVICES-1.  Returns this device's copy (looked up once
+// per device, with that device current), or null with err set.
+constexpr size_t SLICE_SCRATCH_MAX = size_t(1) << 19;
+constexpr int SLICE_SCRATCH_DEVICES = 64;
+double *slice_scratch(std::string &err);
 // the eight checkpoint files as 8 x o x (m+1)(n+1) doubles; start/stop
 // (may be null) are recorded at the kernel's own start and end
 void launch_chk_pack(hipStream_t s, const Geo &g, const ObjC *O, int nobj, const Fluid &U, const Maps &M,

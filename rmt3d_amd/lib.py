@@ -24,7 +24,7 @@ EXPORTS = [
     "rmt3d_estimate_bytes", "rmt3d_server_create", "rmt3d_server_submit", "rmt3d_server_wait",
     "rmt3d_server_info", "rmt3d_server_error", "rmt3d_server_result", "rmt3d_server_release",
     "rmt3d_server_stats", "rmt3d_server_destroy", "rmt3d_write_checkpoint", "rmt3d_write_params",
-    "rmt3d_slice", "rmt3d_write_slice", "rmt3d_macro_data", "rmt3d_centroids", "rmt3d_write_tracers",
+    "rmt3d_slice", "rmt3d_slice_noalloc", "rmt3d_write_slice", "rmt3d_macro_data", "rmt3d_centroids", "rmt3d_write_tracers",
     "rmt3d_tracer_positions", "rmt3d_write_contours", "rmt3d_contour",
 ]
 
@@ -93,6 +93,7 @@ def lib():
         L.rmt3d_write_checkpoint.argtypes = [P, C.c_int, C.c_int, C.c_char_p]
         L.rmt3d_write_params.argtypes = [C.POINTER(_abi.Config), C.POINTER(_abi.Object), C.c_char_p]
         L.rmt3d_slice.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, ip]
+        L.rmt3d_slice_noalloc.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, dp, ip]
         L.rmt3d_write_slice.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p]
         _lib = L
     return _lib
@@ -328,13 +329,15 @@ class Simulation:
         """fluid_3d::write_chk_pt into the existing directory dirname."""
         self._check(lib().rmt3d_write_checkpoint(self.h, snum, fnum, os.fsencode(dirname)), "write_checkpoint")
 
-    def slice(self, dim, point, o_type, obj_id=-1):
-        """fluid_3d::slice: the global slice as a (d2, d1) array."""
+    def slice(self, dim, point, o_type, obj_id=-1, noalloc=False):
+        """fluid_3d::slice: the global slice as a (d2, d1) array (noalloc:
+        through rmt3d_slice_noalloc, staged in the library's fixed scratch)."""
         m, n, o = self.cfg.sys_size
         cap = (max(m, n, o) + 1) ** 2
         out = np.zeros(cap)
         ln = C.c_int(cap)
-        self._check(lib().rmt3d_slice(self.h, dim, point, o_type, obj_id, _dp(out), C.byref(ln)), "slice")
+        fn = lib().rmt3d_slice_noalloc if noalloc else lib().rmt3d_slice
+        self._check(fn(self.h, dim, point, o_type, obj_id, _dp(out), C.byref(ln)), "slice")
         corner = 3 <= o_type < 6 or o_type in (7, 18)
         g = [self.cfg.sys_size[d] + (1 if corner and not self.cfg.prd[d] else 0) for d in range(3)]
         rest = [d for d in range(3) if d != dim]
