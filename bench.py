@@ -29,11 +29,13 @@ The JSON line carries:
                null if that summary covers another kernel
   cpu_baseline the FP64 oracle (C restatement of the reference) on a bounded
                sample of the same workload on this host
-  config.serving  the serving workload: Poisson arrivals (--serve-rate per
-               second, --serve-requests requests) of independent solves, grid
-               from {32,48,64,96,128}^3, 1-8 spheres, lid speed 0.5-2, 200
-               steps each; p50/p99 time-to-completion and aggregate
-               cell-steps/s (rmt3d_amd/serve.py)
+  config.serving  the serving workload: Poisson arrivals (--serve-rates per
+               second, swept; --serve-requests requests per rate) of
+               independent mixed-precision solves, grid from
+               {32,48,64,96,128}^3, 1-8 spheres, lid speed 0.5-2, 200 steps
+               each, up to --serve-max-active at a time; p50/p99
+               time-to-completion and aggregate cell-steps/s per rate
+               (rmt3d_amd/serve.py)
 """
 import argparse
 import json
@@ -70,8 +72,10 @@ def parse():
     ap.add_argument("--io-reps", type=int, default=5)
     ap.add_argument("--shear", action="store_true",
                     help="time the 256^3 shear workload (64 spheres, periodic x and y) instead")
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[1.0, 4.0, 16.0],
+                    help="Poisson arrival rates (per second) to sweep")
     ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-max-active", type=int, default=6)
     return ap.parse_args()
 
 
@@ -468,10 +472,18 @@ def main():
         "roofline": g["roofline"],
     }
     if not args.no_serve:
-        from rmt3d_amd import serve
-        config["serving"] = serve.poisson_run(rate=args.serve_rate, n_requests=args.serve_requests)
-        print("bench: serving p50 %.2f s p99 %.2f s" % (config["serving"]["p50_s"], config["serving"]["p99_s"]),
-              file=sys.stderr, flush=True)
+        from rmt3d_amd import _abi, serve
+        # Poisson arrivals of mixed-precision requests at a swept rate, from
+        # a light load to past saturation (requests queue for a slot)
+        sweep = []
+        for rate in args.serve_rates:
+            r = serve.poisson_run(rate=rate, n_requests=args.serve_requests, precision=_abi.PREC_MIXED,
+                                  max_active=args.serve_max_active)
+            sweep.append(r)
+            print("bench: serving rate %g/s p50 %.2f s p99 %.2f s, %.3g cell-steps/s, peak %d active" %
+                  (rate, r["p50_s"], r["p99_s"], r["aggregate_cell_steps_per_s"], r["peak_active"]),
+                  file=sys.stderr, flush=True)
+        config["serving"] = {"precision": "mixed", "sweep": sweep}
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu(args)
     print(json.dumps(line), flush=True)

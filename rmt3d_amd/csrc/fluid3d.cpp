@@ -287,6 +287,8 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
     // tracing tools that do not follow graph launches)
     const char *ng = getenv("RMT3D_NO_GRAPH");
     no_graph = ng && ng[0] == '1';
+    const char *fo = getenv("RMT3D_MIXED_FUSE");
+    fuse_off = fo && fo[0] == '0';
     // colour passes per launch above the tail: one per launch on the finest
     // level (bandwidth-bound, where a halo's redundant loads cost more than
     // launches save), a whole sweep per launch below it (latency-bound);
@@ -372,9 +374,13 @@ double MGSolver::resid_max(hipStream_t s) {
 // them to another, so the launches alternate between x and the level's
 // residual array (free while smoothing); an odd count ends with a copy back
 // into x.  Per-kernel profiling stamps each launch's own start and end.
+// first_col / x64 (single-colour launches only, the mixed path's finest
+// level): the first sweep starts at colour first_col (the colours before it
+// were applied by the residual kernel), and every pass of the last sweep adds
+// its nodes' new values to the FP64 solution x64.
 template <typename T>
 static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3], int np, int nsweeps, KProf *kp,
-                   int bs = 0) {
+                   int bs = 0, int first_col = 0, double *x64 = nullptr) {
     int ncol = fem ? 4 : 2;
     // a halo no wider than a periodic extent wraps at most once
     for (int a = 0; a < 3; a++)
@@ -382,11 +388,11 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
     if (np < 1) np = 1;
     int m = 0;
     for (int sw = 0; sw < nsweeps; sw++)
-        for (int c0 = 0; c0 < ncol; c0 += np) {
+        for (int c0 = (sw == 0 && np == 1) ? first_col : 0; c0 < ncol; c0 += np) {
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (kp) kp->pair(e0, e1);
             if (np == 1) {
-                launch_mg_smooth(s, fem, l, prd, c0, e0, e1, bs);
+                launch_mg_smooth(s, fem, l, prd, c0, e0, e1, bs, sw == nsweeps - 1 ? x64 : nullptr);
                 continue;
             }
             SweepPlan P = mg_sweep_plan(fem, l.d, prd, (int)sizeof(T), std::min(np, ncol - c0));
@@ -413,10 +419,10 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
 //     (2, 4, 8) of its contractions.
 // Every candidate performs the same operations in the same order on every
 // unknown (the fused sweeps are bit-identical to single-colour passes), so a
-// choice changes time only, never results.  Each candidate runs a few times
-// on the level's own arrays (zero at set-up, zeroed again afterwards), timed
-// with events on the context stream; the minimum over three interleaved
-// rounds decides.  Choices are cached per operator, level shape and
+// choice changes time only, never results.  Each candidate is timed over a
+// few whole iterations of the solve on the solver's own arrays (zero at
+// set-up, zeroed again afterwards), with events on the context stream; the
+// minimum over three interleaved rounds decides.  Choices are cached per operator, level shape and
 // precision for the life of the process, so contexts of one shape (a serving
 // mix) tune once.  RMT3D_TUNE=0 keeps the built-in defaults; RMT3D_SWEEP, when
 // set, fixes the passes per launch.
@@ -460,44 +466,68 @@ template <typename T>
 void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool sweep_fixed) {
     const int ncol = fem ? 4 : 2;
     int last = fdm ? nlev - 1 : tail_lev;   // levels [0, last) are smoothed by launches
-    for (int l = 0; l < last; l++) {
+    // Each candidate is timed inside whole iterations of the solve (the
+    // V-cycle, or the defect-correction step around it), so that it meets
+    // the caches the other kernels of the cycle leave it: a colour pass timed
+    // alone on data that stays in L2 prefers other workgroup sizes than it
+    // does inside the cycle.  Choices are made from the coarsest level up,
+    // every other level keeping its current (default or chosen) shape.
+    auto iteration = [&]() { cycle_body(s); };
+    if (fdm) {
+        for (int a = 0; a < 3; a++) {
+            static const int pbs[3] = {2, 4, 8};
+            int keep = F.pb[a];
+            std::vector<double> t = time_candidates(s, 3, [&](int k) {
+                F.pb[a] = pbs[k];
+                iteration();
+            });
+            F.pb[a] = t[argmin(t)] < 1e29 ? pbs[argmin(t)] : keep;
+        }
+    }
+    for (int l = last - 1; l >= 0; l--) {
         // candidates: (np = 1, bs) for each workgroup size, then np = 2 (, 4)
         std::vector<std::pair<int, int>> cand;
         static const int sizes[5] = {64, 128, 256, 512, 1024};
-        for (int b : sizes) cand.push_back({1, b});
+        for (int bsz : sizes) cand.push_back({1, bsz});
         if (!sweep_fixed)
             for (int np = 2; np <= ncol; np *= 2) cand.push_back({np, 0});
         else if (npass[l] > 1)
             cand.assign(1, {npass[l], 0});
+        int keep_np = npass[l], keep_bs = smooth_bs[l];
         std::vector<double> t = time_candidates(s, (int)cand.size(), [&](int c) {
-            smooth(s, fem, LV[l], prd, cand[c].first, 1, (KProf *)nullptr, cand[c].second);
+            npass[l] = cand[c].first;
+            smooth_bs[l] = cand[c].second;
+            iteration();
         });
         int w = argmin(t);
-        npass[l] = cand[w].first;
-        smooth_bs[l] = cand[w].second;
-    }
-    if (fdm) {
-        MGLevel<T> &c = LV[nlev - 1];
-        for (int a = 0; a < 3; a++) {
-            static const int pbs[3] = {2, 4, 8};
-            std::vector<double> t = time_candidates(s, 3, [&](int k) {
-                launch_mg_fdm_pass<T>(s, c, a, pbs[k], F.S[a], a == 2 ? F.dinv : nullptr, c.b, c.r);
-            });
-            F.pb[a] = pbs[argmin(t)];
+        if (t[w] < 1e29) {
+            npass[l] = cand[w].first;
+            smooth_bs[l] = cand[w].second;
+        } else {
+            npass[l] = keep_np;
+            smooth_bs[l] = keep_bs;
         }
     }
+    // the iterations ran on zero arrays; leave every level as set-up left it
     for (int l = 0; l < nlev; l++) {
-        hipMemsetAsync(LV[l].x, 0, sizeof(T) * LV[l].nn, s);
-        hipMemsetAsync(LV[l].b, 0, sizeof(T) * LV[l].nn, s);
-        hipMemsetAsync(LV[l].r, 0, sizeof(T) * LV[l].nn, s);
+        hipMemsetAsync(L[l].x, 0, sizeof(double) * L[l].nn, s);
+        hipMemsetAsync(L[l].b, 0, sizeof(double) * L[l].nn, s);
+        hipMemsetAsync(L[l].r, 0, sizeof(double) * L[l].nn, s);
+        if (mixed) {
+            hipMemsetAsync(Lf[l].x, 0, sizeof(float) * Lf[l].nn, s);
+            hipMemsetAsync(Lf[l].b, 0, sizeof(float) * Lf[l].nn, s);
+            hipMemsetAsync(Lf[l].r, 0, sizeof(float) * Lf[l].nn, s);
+        }
     }
+    (void)LV;
 }
 
 void MGSolver::tune(hipStream_t s) {
     const char *tv = getenv("RMT3D_TUNE");
     if (tv && tv[0] == '0') return;
     bool sweep_fixed = getenv("RMT3D_SWEEP") != nullptr;
-    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, nlev, fdm ? 1 : 0, prd[0], prd[1], prd[2]};
+    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, nlev, fdm ? 1 : 0,
+                            prd[0], prd[1], prd[2]};
     for (int l = 0; l < nlev; l++)
         for (int a = 0; a < 3; a++) key.push_back(L[l].d[a]);
     {
@@ -545,7 +575,9 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
     smooth(s, fem, L[lev], prd, npass[lev], v_up, kp, smooth_bs[lev]);
 }
 
-void MGSolver::vcycle32(hipStream_t s, int lev) {
+// fused (finest level only): the first colour pass came with the residual,
+// and the last sweep adds the correction into the FP64 solution
+void MGSolver::vcycle32(hipStream_t s, int lev, bool fused) {
     if (fdm && lev == nlev - 1) {
         launch_mg_fdm(s, Lf[lev], fdm32);
         return;
@@ -555,12 +587,23 @@ void MGSolver::vcycle32(hipStream_t s, int lev) {
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
-    smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp, smooth_bs[lev]);
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp, smooth_bs[lev], fused ? 1 : 0);
     launch_mg_resid(s, fem, Lf[lev], prd);
     launch_mg_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
-    vcycle32(s, lev + 1);
+    vcycle32(s, lev + 1, false);
     launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
-    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev]);
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev], 0, fused ? L[0].x : nullptr);
+}
+
+// The defect-correction step's FP64 pieces ride on the FP32 V-cycle's own
+// launches when the finest level is smoothed one colour per launch (and
+// has smoothing sweeps before and after): the first colour pass is written
+// by the residual kernel, and the last sweep adds the correction into the
+// FP64 solution, so neither a separate colour pass nor an update pass runs.
+// The values are the same as those of the separate launches, bit for bit.
+// RMT3D_MIXED_FUSE=0 runs the separate launches.
+bool MGSolver::fused_finest() const {
+    return mixed && !fuse_off && tail_lev > 0 && !(fdm && nlev == 1) && npass[0] == 1 && v_down > 0 && v_up > 0;
 }
 
 // one iteration: an FP64 V-cycle, or (mixed) one FP64 defect-correction step
@@ -570,9 +613,10 @@ void MGSolver::cycle_body(hipStream_t s) {
         vcycle64(s, 0);
         return;
     }
-    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, Lf[0].x);
-    vcycle32(s, 0);
-    launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
+    bool fused = fused_finest();
+    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, Lf[0].x, fused ? 1 : 0);
+    vcycle32(s, 0, fused);
+    if (!fused) launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
 }
 
 // The kernel sequence of one iteration and all its arguments are fixed for
@@ -974,6 +1018,7 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     W.mask = (uint32_t *)dalloc(sizeof(uint32_t) * Ng);
     W.flag = (unsigned char *)dalloc(Ng);
     W.lid = (signed char *)dalloc(Ng);
+    W.box = (int *)dalloc(sizeof(int) * 8 * std::max(1, n_obj));
     for (int a = 0; a < 3; a++) W.val[a] = (double *)dalloc(sizeof(double) * Ng);
     W.list[0] = (int *)dalloc(sizeof(int) * Ng);
     W.list[1] = (int *)dalloc(sizeof(int) * Ng);
@@ -1287,7 +1332,10 @@ int Sim::update_reference_map() {
     extrapolate();
     launch_zero_i32(stream, W.err, 1);
     launch_corrector(stream, g, P, d_obj, R, ext, tmp, W.err);
+    // the status word travels with the error flag, so that a fatal flag of an
+    // earlier stage of this step takes precedence over the error code
     hipMemcpyAsync(h_int, W.err, sizeof(int), hipMemcpyDeviceToHost, stream);
+    hipMemcpyAsync(h_status, d_status, sizeof(int), hipMemcpyDeviceToHost, stream);
     hipStreamSynchronize(stream);
     return *h_int;
 }
@@ -1327,10 +1375,12 @@ int Sim::step(int debug) {
     mark(2);
     fill_bc_cf();
     mark_end(2);
-    int e = check_status("step(godunov)");
-    if (e) return e;
+    // The fatal flags of the advection and upwinding stages are read with
+    // the step's final status instead of here: after such a flag the
+    // remaining stages still index only valid cells and maps, and the step
+    // returns the same fatal error, one host round trip fewer per step.
     mark(4);
-    e = mac_solve();
+    int e = mac_solve();
     mark_end(4);
     if (e) return e;
     mark(2);
@@ -1340,6 +1390,7 @@ int Sim::step(int debug) {
     int prob = update_reference_map();
     mark_end(0);
     M = maps();
+    if (*h_status) return check_status("step");
     if (prob) {
         launch_revert(stream, g, R, ext, tmp);
         hipStreamSynchronize(stream);
@@ -1360,7 +1411,10 @@ int Sim::step(int debug) {
     mark(1);
     int pe = pp_solve();
     mark_end(1);
-    if (pe) return RMT3D_PROJECTION_ERR;
+    if (pe) {
+        e = check_status("step");
+        return e ? e : RMT3D_PROJECTION_ERR;
+    }
     mark(0);
     launch_full_update(stream, g, R, ext);
     launch_redraw(stream, g, P, d_obj, R, ext, W, cfg.nlayers, d_status);

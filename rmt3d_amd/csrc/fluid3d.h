@@ -90,12 +90,14 @@ struct MGSolver {
     double check_wait();
     hipEvent_t ev_check = nullptr;
     void vcycle64(hipStream_t s, int lev);
-    void vcycle32(hipStream_t s, int lev);
+    void vcycle32(hipStream_t s, int lev, bool fused = false);
+    bool fused_finest() const;
     void cycle(hipStream_t s);              // one V-cycle (or one defect-correction step)
     void cycle_body(hipStream_t s);         // its launches
     hipGraphExec_t gexec = nullptr;         // the launches captured once
     bool graph_failed = false;
     bool no_graph = false;
+    bool fuse_off = false;                  // RMT3D_MIXED_FUSE=0 at set-up: separate launches
     void timed_cycle(hipStream_t s);
     double run(hipStream_t s);
 };

@@ -16,6 +16,8 @@
 //    over the search cube (means, then centred moments) in the reference's
 //    summation order.
 #include <limits.h>
+
+#include <algorithm>
 #include "dev.h"
 #include "launch.h"
 
@@ -432,6 +434,178 @@ __global__ __launch_bounds__(1024) void k_stage_obj(Geo g, Scratch W, uint32_t o
     for (int a = 0; a < 3; a++) { W.st_x[a][p] = W.val[a][t]; W.st_xpred[a][p] = W.val[a][t]; }
 }
 
+// ---- per-object boxes ----
+// Every per-object pass of the extrapolation touches only the object's
+// primary cells dilated by the layers and the search radius: the BFS marks
+// cells within nlayers of the primaries, a list cell reads masks one cell
+// further and least-squares data max_rs cells further, and prev_normal one
+// more.  So each object's passes run over the bounding box of its primary
+// cells (padded index space) grown by nlayers + max_rs + 2 and clamped to
+// the padded grid, instead of over the whole grid.  Outside the box the
+// scratch keeps whatever earlier objects left, which no pass of this object
+// reads: masks there are below this object's mask values, and flags are
+// zero (every flag a pass sets inside the box is cleared by the compaction
+// in the box).  In a periodic dimension where the grown box reaches the ghost
+// frame or the strip that feeds the opposite ghost frame, the box takes the
+// whole padded extent, so that the periodic copies find their sources and
+// targets inside it.  bx[8 o .. 8 o + 7] = lo[3], ext[3], volume.
+constexpr int BOX_INTS = 8;
+
+__global__ void k_box_init(int *bb, int n_obj) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_obj) return;
+    for (int d = 0; d < 3; d++) { bb[BOX_INTS * t + d] = INT_MAX; bb[BOX_INTS * t + 3 + d] = -1; }
+}
+
+// index extremes of each object's primary cells: a workgroup reduces its
+// cells in LDS, then adds one global atomic per extreme and object it saw
+__global__ __launch_bounds__(256) void k_box_reduce(Geo g, RMap R, int *bb, int n_obj) {
+    __shared__ int sm[6 * MAXOBJ];
+    for (int e = threadIdx.x; e < 6 * n_obj; e += blockDim.x) sm[e] = (e % 6) < 3 ? INT_MAX : -1;
+    __syncthreads();
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.Ng; t += gridDim.x * blockDim.x) {
+        uint32_t c = R.c[t];
+        if (c >= 1 && c <= (uint32_t)n_obj) {
+            int ijk[3];
+            unpack(g, t, ijk[0], ijk[1], ijk[2]);
+            int *o = sm + 6 * (c - 1);
+            for (int d = 0; d < 3; d++) {
+                atomicMin(o + d, ijk[d]);
+                atomicMax(o + 3 + d, ijk[d]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 6 * n_obj; e += blockDim.x) {
+        int ob = e / 6, d = e % 6;
+        if (sm[6 * ob + 3] < 0) continue;
+        int *dst = bb + BOX_INTS * ob + d;
+        if (d < 3) atomicMin(dst, sm[e]);
+        else atomicMax(dst, sm[e]);
+    }
+}
+
+__global__ void k_box_finish(Geo g, int *bb, int n_obj, int grow) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_obj) return;
+    int *b = bb + BOX_INTS * t;
+    if (b[3] < 0) {   // no primary cell: empty box
+        for (int e = 0; e < BOX_INTS; e++) b[e] = 0;
+        return;
+    }
+    const int s[3] = {g.m, g.n, g.o}, d4[3] = {g.m4, g.n4, g.o4};
+    int lo[3], ext[3];
+    for (int d = 0; d < 3; d++) {
+        int l = b[d] - grow, h = b[3 + d] + grow;
+        if (g.prd[d] && (l <= 3 || h >= s[d])) { l = 0; h = d4[d] - 1; }
+        l = l < 0 ? 0 : l;
+        h = h > d4[d] - 1 ? d4[d] - 1 : h;
+        lo[d] = l;
+        ext[d] = h - l + 1;
+    }
+    for (int d = 0; d < 3; d++) { b[d] = lo[d]; b[3 + d] = ext[d]; }
+    b[6] = ext[0] * ext[1] * ext[2];
+    b[7] = 0;
+}
+
+__device__ __forceinline__ int box_cell(const Geo &g, const int *bx, int t) {
+    int e0 = bx[3], e1 = bx[4];
+    int r = t / e0;
+    int i = t - r * e0;
+    int k = r / e1;
+    int j = r - k * e1;
+    return IXb(g, bx[0] + i, bx[1] + j, bx[2] + k);
+}
+
+#define BOX_LOOP(bx)                                                                       \
+    const int vol_ = (bx)[6];                                                              \
+    for (int t_ = blockIdx.x * blockDim.x + threadIdx.x; t_ < vol_; t_ += gridDim.x * blockDim.x)
+
+__global__ void k_obj_init_box(Geo g, RMap R, Scratch W, const int *bx, uint32_t oid, uint32_t nmask) {
+    BOX_LOOP(bx) {
+        int t = box_cell(g, bx, t_);
+        if (R.c[t] == oid) {
+            W.lid[t] = 0;
+            for (int a = 0; a < 3; a++) W.val[a][t] = R.xpred[a][t];
+            W.mask[t] = nmask;
+        } else {
+            W.lid[t] = -1;
+        }
+        W.flag[t] = 0;
+    }
+}
+__global__ void k_first_push_box(Geo g, Scratch W, const int *bx, uint32_t cur_mask) {
+    BOX_LOOP(bx) {
+        int ind = box_cell(g, bx, t_);
+        if (within_inner_ghost(g, ind) && W.mask[ind] == cur_mask) push6(g, W, ind, cur_mask);
+    }
+}
+__global__ void k_apply_flags_box(Geo g, Scratch W, const int *bx, uint32_t nmask) {
+    BOX_LOOP(bx) {
+        int t = box_cell(g, bx, t_);
+        if (W.flag[t]) W.mask[t] = nmask;
+    }
+}
+__global__ void k_comm_mask_box(Geo g, Scratch W, const int *bx) {
+    BOX_LOOP(bx) {
+        int t = box_cell(g, bx, t_);
+        int i, j, k;
+        unpack(g, t, i, j, k);
+        int s = psrc(g, i, j, k);
+        if (s >= 0) W.mask[t] = W.mask[s];
+    }
+}
+__global__ void k_scan_ghost_box(Geo g, Scratch W, const int *bx, uint32_t nmask) {
+    BOX_LOOP(bx) {
+        int t = box_cell(g, bx, t_);
+        int i, j, k;
+        unpack(g, t, i, j, k);
+        bool ghost = (i < 2 || i > g.m + 1 || j < 2 || j > g.n + 1 || k < 2 || k > g.o + 1);
+        if (ghost && W.mask[t] == nmask) W.flag[t] = 1;
+    }
+}
+__global__ void k_wrap_layer_box(Geo g, Scratch W, const int *bx, int L) {
+    BOX_LOOP(bx) {
+        int t = box_cell(g, bx, t_);
+        int i, j, k;
+        unpack(g, t, i, j, k);
+        int s = psrc(g, i, j, k);
+        if (s < 0 || W.lid[s] != L) continue;
+        W.lid[t] = (signed char)L;
+        for (int a = 0; a < 3; a++) W.val[a][t] = W.val[a][s];
+    }
+}
+// the workgroup-uniform loops keep every thread at each block_append
+__global__ __launch_bounds__(1024) void k_compact_box(Geo g, Scratch W, const int *bx, int *list, int *cnt) {
+    __shared__ int sh[17];
+    const int vol = bx[6];
+    for (int b0 = blockIdx.x * blockDim.x; b0 < vol; b0 += gridDim.x * blockDim.x) {
+        int tt = b0 + threadIdx.x;
+        int t = tt < vol ? box_cell(g, bx, tt) : 0;
+        bool f = tt < vol && W.flag[t] != 0;
+        int p = block_append(f, cnt, sh);
+        if (f) {
+            list[p] = t;
+            W.flag[t] = 0;
+        }
+    }
+}
+__global__ __launch_bounds__(1024) void k_stage_obj_box(Geo g, Scratch W, const int *bx, uint32_t oid, int *status) {
+    __shared__ int sh[17];
+    const int vol = bx[6];
+    for (int b0 = blockIdx.x * blockDim.x; b0 < vol; b0 += gridDim.x * blockDim.x) {
+        int tt = b0 + threadIdx.x;
+        int t = tt < vol ? box_cell(g, bx, tt) : 0;
+        int L = tt < vol ? W.lid[t] : 0;
+        int p = block_append(L >= 1, W.st_n, sh);
+        if (L < 1) continue;
+        if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); continue; }
+        W.st_cell[p] = t;
+        W.st_c[p] = oid | ((uint32_t)L << LDIG);
+        for (int a = 0; a < 3; a++) { W.st_x[a][p] = W.val[a][t]; W.st_xpred[a][p] = W.val[a][t]; }
+    }
+}
+
 // ---- band build: count, scan, place, per-cell order, gather ----
 __global__ void k_count_cells(Geo g, Band B, Scratch W) {
     int n = *W.st_n;
@@ -594,22 +768,28 @@ void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
     // temp band needed by prev_normal is passed in B's place by the caller
     // through W; see fluid3d.hip
     (void)B;
+    if (n_obj < 1) return;
+    // the objects' boxes, from their primary cells
+    hipLaunchKernelGGL(k_box_init, dim3((n_obj + 63) / 64), dim3(64), 0, s, W.box, n_obj);
+    hipLaunchKernelGGL(k_box_reduce, dim3(std::min(1024, (g.Ng + 255) / 256)), dim3(256), 0, s, g, R, W.box, n_obj);
+    hipLaunchKernelGGL(k_box_finish, dim3((n_obj + 63) / 64), dim3(64), 0, s, g, W.box, n_obj, nlayers + max_rs + 2);
+    const dim3 ge(std::min(1024, (g.Ng + 255) / 256)), gc(std::min(256, (g.Ng + 1023) / 1024));
     for (int oid = 1; oid <= n_obj; oid++) {
+        const int *bx = W.box + BOX_INTS * (oid - 1);
         if (nmask > UINT_MAX - 50) {
             nmask = 1;
             hipLaunchKernelGGL(k_mask_reset, g1(g.Ng), dim3(256), 0, s, g, W);
         }
         uint32_t cur_mask = nmask;
-        hipLaunchKernelGGL(k_obj_init, g1(g.Ng), dim3(256), 0, s, g, R, W, (uint32_t)oid, nmask);
+        hipLaunchKernelGGL(k_obj_init_box, ge, dim3(256), 0, s, g, R, W, bx, (uint32_t)oid, nmask);
         nmask++;
-        hipLaunchKernelGGL(k_first_push, dim3((g.m4 - 2 + 63) / 64, g.n4 - 2, g.o4 - 2), dim3(64), 0, s, g, W,
-                           cur_mask);
-        hipLaunchKernelGGL(k_apply_flags, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
-        if (per) hipLaunchKernelGGL(k_comm_mask, g1(g.Ng), dim3(256), 0, s, g, W);
-        hipLaunchKernelGGL(k_scan_ghost, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
+        hipLaunchKernelGGL(k_first_push_box, ge, dim3(256), 0, s, g, W, bx, cur_mask);
+        hipLaunchKernelGGL(k_apply_flags_box, ge, dim3(256), 0, s, g, W, bx, nmask);
+        if (per) hipLaunchKernelGGL(k_comm_mask_box, ge, dim3(256), 0, s, g, W, bx);
+        hipLaunchKernelGGL(k_scan_ghost_box, ge, dim3(256), 0, s, g, W, bx, nmask);
         int cur = 0;
         launch_zero_i32(s, W.cnt + cur, 1);
-        hipLaunchKernelGGL(k_compact, dim3((g.Ng + 1023) / 1024), dim3(1024), 0, s, g, W, W.list[cur], W.cnt + cur);
+        hipLaunchKernelGGL(k_compact_box, gc, dim3(1024), 0, s, g, W, bx, W.list[cur], W.cnt + cur);
         for (int L = 1; L <= nlayers; L++) {
             nmask++;
             LsqArgs A;
@@ -621,15 +801,15 @@ void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O,
             if (L < nlayers) {
                 hipLaunchKernelGGL(k_list_push, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, W, W.list[cur],
                                    W.cnt + cur, cur_mask);
-                hipLaunchKernelGGL(k_apply_flags, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
-                if (per) hipLaunchKernelGGL(k_comm_mask, g1(g.Ng), dim3(256), 0, s, g, W);
-                hipLaunchKernelGGL(k_scan_ghost, g1(g.Ng), dim3(256), 0, s, g, W, nmask);
-                hipLaunchKernelGGL(k_compact, dim3((g.Ng + 1023) / 1024), dim3(1024), 0, s, g, W, W.list[nxt], W.cnt + nxt);
+                hipLaunchKernelGGL(k_apply_flags_box, ge, dim3(256), 0, s, g, W, bx, nmask);
+                if (per) hipLaunchKernelGGL(k_comm_mask_box, ge, dim3(256), 0, s, g, W, bx);
+                hipLaunchKernelGGL(k_scan_ghost_box, ge, dim3(256), 0, s, g, W, bx, nmask);
+                hipLaunchKernelGGL(k_compact_box, gc, dim3(1024), 0, s, g, W, bx, W.list[nxt], W.cnt + nxt);
             }
             cur = nxt;
-            if (per) hipLaunchKernelGGL(k_wrap_layer, g1(g.Ng), dim3(256), 0, s, g, W, L);
+            if (per) hipLaunchKernelGGL(k_wrap_layer_box, ge, dim3(256), 0, s, g, W, bx, L);
         }
-        hipLaunchKernelGGL(k_stage_obj, dim3((g.Ng + 1023) / 1024), dim3(1024), 0, s, g, W, (uint32_t)oid, status);
+        hipLaunchKernelGGL(k_stage_obj_box, gc, dim3(1024), 0, s, g, W, bx, (uint32_t)oid, status);
     }
 }
 

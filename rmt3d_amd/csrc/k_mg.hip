@@ -124,17 +124,22 @@ __device__ __forceinline__ int qdiv(int a, int d, float rd) {
 constexpr double MAC_OMEGA = 1.25;
 
 template <typename T>
-__device__ __forceinline__ void mac_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
+__device__ __forceinline__ T mac_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
     int c = i + L.d[0] * (j + L.d[1] * k);
     T dg;
     T s = mac_off(L, P, L.x, i, j, k, dg);
     T xn = (L.b[c] - s) / dg;
-    L.x[c] = T(1 - MAC_OMEGA) * L.x[c] + T(MAC_OMEGA) * xn;
+    T v = T(1 - MAC_OMEGA) * L.x[c] + T(MAC_OMEGA) * xn;
+    L.x[c] = v;
+    return v;
 }
 
-// one thread per node of the colour: t -> (ih, j, k), i = 2 ih + parity
+// one thread per node of the colour: t -> (ih, j, k), i = 2 ih + parity.
+// x64 (the final sweep of the mixed path's FP32 V-cycle on the finest
+// level): this pass is the colour's last update of the correction, which is
+// added to the FP64 solution at once, x64 += (double) x.
 template <typename T>
-__global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, float rd1) {
+__global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, float rd1, double *x64) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int row = qdiv(t, hx, rhx);
     int ih = t - row * hx;
@@ -143,15 +148,18 @@ __global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, 
     if (k >= L.d[2]) return;
     int i = 2 * ih + ((j + k + color) & 1);
     if (i >= L.d[0]) return;
-    mac_smooth_at(L, P, i, j, k);
+    T v = mac_smooth_at(L, P, i, j, k);
+    if (x64) x64[i + L.d[0] * (j + L.d[1] * k)] += (double)v;
 }
 
 template <typename T>
-__device__ __forceinline__ void fem_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
+__device__ __forceinline__ T fem_smooth_at(const MGLevel<T> &L, const Prd &P, int i, int j, int k) {
     int c = i + L.d[0] * (j + L.d[1] * k);
     T dg, rdg;
     T s = fem_off(L, P, L.x, i, j, k, dg, rdg);
-    L.x[c] = (L.b[c] - s) * rdg;
+    T v = (L.b[c] - s) * rdg;
+    L.x[c] = v;
+    return v;
 }
 
 // Interior node (no wall, no wrap): the 20 non-zero neighbours at fixed
@@ -238,19 +246,32 @@ __device__ __forceinline__ bool fem_colour_node(const MGLevel<T> &L, const FemCo
     return false;
 }
 
+// The launch issues the boundary-capable work-items (end nodes and boundary
+// rows, several times the cost of an interior node) first and the interior
+// after them, so that the expensive workgroups start early instead of
+// forming the tail of the launch: work-item t takes the node of rank
+// (t + NI) mod n of fem_colour_node's order.
+// x64: as in k_mac_smooth, the final sweep adds each node's correction to
+// the FP64 solution
 template <typename T>
-__global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, FemColour F) {
+__global__ void k_fem_smooth(MGLevel<T> L, Prd P, int color, FemColour F, double *x64) {
+    int n = F.NI + F.NA + F.NB;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= F.NI + F.NA + F.NB) return;
+    if (t >= n) return;
+    t += F.NI;
+    if (t >= n) t -= n;
     int i, j, k;
     bool interior = fem_colour_node(L, F, color & 1, t, i, j, k);
     int c = i + L.d[0] * (j + L.d[1] * k);
+    T v;
     if (interior) {
         T s = fem_off_interior(L.x, c, L.d[0], L.d[0] * L.d[1]);
-        L.x[c] = (L.b[c] - s) * T(0.03125);
+        v = (L.b[c] - s) * T(0.03125);
+        L.x[c] = v;
     } else {
-        fem_smooth_at(L, P, i, j, k);
+        v = fem_smooth_at(L, P, i, j, k);
     }
+    if (x64) x64[c] += (double)v;
 }
 
 // host side: the work-item layout of one colour of a level
@@ -354,10 +375,16 @@ static __host__ __device__ NodeMap node_map(const int d[3], int fem) {
     return M;
 }
 
-// returns true for an interior FEM node
+// returns true for an interior FEM node.  Ranks are rotated by NI, so
+// that the boundary nodes (the general stencil) come first in a launch and
+// the interior box after them
 template <typename T>
 __device__ __forceinline__ bool map_node(const MGLevel<T> &L, const NodeMap &M, int t, int &i, int &j, int &k) {
     const int D0 = M.D0, D1 = M.D1;
+    if (!M.plain) {
+        t += M.NI;
+        if (t >= M.total) t -= M.total;
+    }
     if (M.plain) {
         int row = qdiv(t, D0, M.r0);
         i = t - row * D0;
@@ -954,14 +981,45 @@ __global__ __launch_bounds__(1024) void k_tail(TailArgs<T> A, Prd P, int fem, in
 
 // mixed defect correction, first half: FP64 residual of the FP64 iterate
 // straight into the FP32 right-hand side, FP32 correction zeroed
+// FIRST: the FP32 correction starts from zero, so the first colour pass of
+// the V-cycle's first smoothing sweep sees only zero neighbours and sets
+// each node of colour 0 to a function of its own right-hand side alone.  That
+// value is written here with the residual, so the V-cycle starts its first
+// sweep at colour 1.  It equals the colour pass's value bit for bit: the pass
+// sums +-0 terms to +0, so FEM gives (b - 0) / dg = b * rdg and MAC
+// (1 - w) 0 + w (b - 0) / dg = w (b / dg).
+template <bool FIRST>
 __global__ void k_resid_to_f32(MGLevel<double> L, Prd P, NodeMap M, float *bf, float *xf) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= M.total) return;
     int i, j, k;
     bool in = map_node(L, M, t, i, j, k);
     int c = i + L.d[0] * (j + L.d[1] * k);
-    bf[c] = (float)resid_value(L, P, M.fem, in, i, j, k, c);
-    xf[c] = 0.f;
+    float r = (float)resid_value(L, P, M.fem, in, i, j, k, c);
+    bf[c] = r;
+    float x0 = 0.f;
+    if (FIRST) {
+        if (M.fem) {
+            if (((j | k) & 1) == 0) {
+                float rdg = 0.03125f;
+                if (!P.p[0] && (i == 0 || i == L.d[0] - 1)) rdg *= 2.f;
+                if (!P.p[1] && (j == 0 || j == L.d[1] - 1)) rdg *= 2.f;
+                if (!P.p[2] && (k == 0 || k == L.d[2] - 1)) rdg *= 2.f;
+                x0 = r * rdg;
+            }
+        } else if (((i + j + k) & 1) == 0) {
+            int miss = 0;
+            miss += nbw(i, -1, L.d[0], P.p[0]) < 0;
+            miss += nbw(i, 1, L.d[0], P.p[0]) < 0;
+            miss += nbw(j, -1, L.d[1], P.p[1]) < 0;
+            miss += nbw(j, 1, L.d[1], P.p[1]) < 0;
+            miss += nbw(k, -1, L.d[2], P.p[2]) < 0;
+            miss += nbw(k, 1, L.d[2], P.p[2]) < 0;
+            float dg = float(-6.0 + miss);
+            x0 = float(MAC_OMEGA) * (r / dg);
+        }
+    }
+    xf[c] = x0;
 }
 
 __global__ void k_cast_d2f(const double *a, float *b, int n) {
@@ -981,7 +1039,7 @@ static inline Prd mkprd(const int p[3]) { Prd P; P.p[0] = p[0]; P.p[1] = p[1]; P
 
 template <typename T>
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color, hipEvent_t start,
-                      hipEvent_t stop, int bs) {
+                      hipEvent_t stop, int bs, double *x64) {
     // default workgroup sizes, measured on the 128^3 / 129^3 levels: 1024 for
     // the red-black pass (7.8 us against 8.1 at 256, FP32; 8.8 against 10.0,
     // FP64); 64 for the FEM colour pass, whose wall-row work-items cost
@@ -993,12 +1051,12 @@ void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd
     if (!fem) {
         int hx = (L.d[0] + 1) / 2, n = hx * L.d[1] * L.d[2];
         hipExtLaunchKernelGGL(k_mac_smooth<T>, dim3((n + bs - 1) / bs), dim3(bs), 0, s, start, stop, 0, L, P,
-                              color, hx, 1.0f / hx, 1.0f / L.d[1]);
+                              color, hx, 1.0f / hx, 1.0f / L.d[1], x64);
     } else {
         FemColour F = fem_colour(L.d, color);
         int n = F.NI + F.NA + F.NB;
         hipExtLaunchKernelGGL(k_fem_smooth<T>, dim3((n + bs - 1) / bs), dim3(bs), 0, s, start, stop, 0, L, P, color,
-                              F);
+                              F, x64);
     }
 }
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
@@ -1122,9 +1180,13 @@ void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const in
     else
         hipLaunchKernelGGL((k_tail<T, false>), dim3(1), dim3(1024), 0, s, A, mkprd(prd), fem, vdown, vup, csweeps);
 }
-void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf) {
+void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf,
+                         int first_pass) {
     NodeMap M = node_map(L.d, fem);
-    hipLaunchKernelGGL(k_resid_to_f32, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M, bf, xf);
+    if (first_pass)
+        hipLaunchKernelGGL(k_resid_to_f32<true>, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M, bf, xf);
+    else
+        hipLaunchKernelGGL(k_resid_to_f32<false>, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M, bf, xf);
 }
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n) {
     hipLaunchKernelGGL(k_cast_d2f, dim3((n + 255) / 256), dim3(256), 0, s, a, b, n);
@@ -1438,7 +1500,7 @@ void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
 #define INST(T)                                                                                              \
     template void launch_mg_fdm<T>(hipStream_t, const MGLevel<T> &, const FdmDev<T> &);                     \
     template void launch_mg_smooth<T>(hipStream_t, int, const MGLevel<T> &, const int *, int, hipEvent_t,    \
-                                      hipEvent_t, int);                                                      \
+                                      hipEvent_t, int, double *);                                            \
     template void launch_mg_fdm_pass<T>(hipStream_t, const MGLevel<T> &, int, int, const T *, const T *,    \
                                         const T *, T *);                                                     \
     template void launch_mg_sweep<T>(hipStream_t, int, const MGLevel<T> &, const int *, const SweepPlan &,   \

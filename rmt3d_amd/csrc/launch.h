@@ -122,8 +122,10 @@ void launch_mg_copy(hipStream_t s, const T *a, T *b, int n);
 // smooth one colour (MAC: red-black SOR, FEM: four colours by (j,k) parity);
 // non-null start/stop events are stamped with the kernel's own start and end
 template <typename T>
+// x64 (FP32 levels of the mixed path only): also add each updated node's
+// value to the FP64 array x64 (the final sweep of the correction)
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color,
-                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr, int bs = 0);
+                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr, int bs = 0, double *x64 = nullptr);
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]);
 template <typename T>
@@ -146,7 +148,10 @@ int mg_tail_enable_lds(size_t bytes);
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits);
 // mixed precision helpers
-void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf);
+// first_pass: also write the first colour pass of the V-cycle from zero
+// into xf (nodes of colour 0), zero elsewhere
+void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf,
+                         int first_pass = 0);
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n);
 void launch_axpy_f2d(hipStream_t s, double *x, const float *e, int n);
 void launch_zero_f(hipStream_t s, float *a, int n);
@@ -176,6 +181,7 @@ struct Scratch {
     int *fill;
     int *scan_tmp;       // block sums
     int *err;            // corrector problem count
+    int *box;            // per object: the extrapolation's index box (8 ints)
     Band tband;          // previous band (get_refmap_prev)
 };
 
