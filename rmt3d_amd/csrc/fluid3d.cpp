@@ -1166,7 +1166,7 @@ void Sim::cfl_recheck() {
 void Sim::init_fluid() {
     P.time = time;
     launch_zero_u64(stream, d_u64, 16 * INIT_VMAX_SLOTS);
-    launch_init_fluid(stream, g, P, d_obj, init, U, d_u64);
+    launch_init_fluid(stream, g, P, d_obj, init, U, d_u64, fields_written ? 1 : 0);
     comm_fluid(ORC_VEL);
     hipMemcpyAsync(h_dbl, d_u64, sizeof(unsigned long long) * 16 * INIT_VMAX_SLOTS, hipMemcpyDeviceToHost, stream);
     hipStreamSynchronize(stream);
@@ -1235,6 +1235,7 @@ int Sim::init_from_checkpoint() {
     hipError_t he = hipMemcpyAsync(d, img.data(), sizeof(double) * n8, hipMemcpyHostToDevice, stream);
     if (he == hipSuccess) {
         launch_zero_i32(stream, d_status, 1);
+        fields_written = true;
         launch_chk_reset(stream, g, U, R);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         kprof[7].pair(e0, e1);
@@ -1343,6 +1344,7 @@ int Sim::update_reference_map() {
 // fluid_3d::step_forward (fluid_3d.cc:429-577), non-TANG, no tracers
 int Sim::step(int debug) {
     (void)debug;
+    fields_written = true;
     nt++;
     P.dt = dt;
     P.time = time;
@@ -1501,6 +1503,7 @@ int Sim::prepare_run(int init_err, double *dt_out, int *total) {
 // Copies run on the context's own stream and wait only for it, so other
 // contexts on the device are neither waited for nor disturbed.
 int Sim::upload(const rmt3d_state *st) {
+    fields_written = true;
     size_t n = (size_t)g.Ng;
     auto up = [&](void *dst, const void *src, size_t bytes) {
         if (src) hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream);

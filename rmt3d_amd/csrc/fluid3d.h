@@ -129,6 +129,7 @@ class Sim {
     ObjC *d_obj = nullptr;
     std::vector<ObjC> h_obj;
     Fluid U{};
+    bool fields_written = false;       // a step, upload or checkpoint read has written the fields
     RMap R{};
     Band ext{}, tmp{};
     Scratch W{};

@@ -1017,12 +1017,16 @@ __device__ __forceinline__ void obj_rm0(const InitC &I, const ObjC *O, int o, co
         for (int a = 0; a < 3; a++) xi[a] = X[a];
 }
 
-// init_fluid (fluid_3d.cc:292-318); vmax is the maximum of |u| over the cells
+// init_fluid (fluid_3d.cc:292-318).  ZERO also clears every other field of
+// the cell record; a context whose fields have not been written since their
+// zero-filled allocation skips those stores (32 of 37 words per cell).
+// vmax is the maximum of |u| over the cells
 // the reference tracks.  Each wavefront reduces its lanes' values with
 // shuffles (every lane reaches the reduction, out-of-range lanes with no
 // value) and adds one atomicMax on the bit pattern (|u| >= 0, so the bits
 // order like the values) to one of INIT_VMAX_SLOTS words 128 bytes apart,
 // picked by the row; the host takes the maximum of the slots.
+template <bool ZERO>
 __global__ void k_init_fluid(Geo g, Phys P, const ObjC *O, InitC I, Fluid U, unsigned long long *vmax_slots) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y, k = blockIdx.z;
@@ -1064,10 +1068,12 @@ __global__ void k_init_fluid(Geo g, Phys P, const ObjC *O, InitC I, Fluid U, uns
             has = true;
         }
         U.vel[0][c] = u; U.vel[1][c] = v; U.vel[2][c] = w;
-        U.q[c] = 0; U.dp[c] = 0;
-        for (int a = 0; a < 3; a++) U.dvel[a][c] = 0;
-        for (int a = 0; a < 18; a++) U.fvel[a][c] = 0;
-        for (int a = 0; a < 9; a++) U.sigma[a][c] = 0;
+        if (ZERO) {
+            U.q[c] = 0; U.dp[c] = 0;
+            for (int a = 0; a < 3; a++) U.dvel[a][c] = 0;
+            for (int a = 0; a < 18; a++) U.fvel[a][c] = 0;
+            for (int a = 0; a < 9; a++) U.sigma[a][c] = 0;
+        }
     }
     double m = has ? speed : -1.0;
 #pragma unroll
@@ -1242,7 +1248,7 @@ void launch_divu_terms(hipStream_t s, const Geo &g, const double *r, double *out
     hipLaunchKernelGGL(k_divu_terms, rows(g.fm, g.fn, g.fo, 64), dim3(64), 0, s, g, r, out);
 }
 void launch_init_fluid(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const InitSpec &is, const Fluid &U,
-                       unsigned long long *vmax_bits) {
+                       unsigned long long *vmax_bits, int zero_rest) {
     InitC I;
     I.has_exact = is.has_exact;
     I.vel_prof_num = is.vel_prof_num;
@@ -1252,7 +1258,10 @@ void launch_init_fluid(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O
         I.type[o] = is.type[o];
         for (int a = 0; a < 3; a++) { I.eig[o][a] = is.eig[o][a]; I.omega[o][a] = is.omega[o][a]; }
     }
-    hipLaunchKernelGGL(k_init_fluid, rows(g.m4, g.n4, g.o4, 64), dim3(64), 0, s, g, P, O, I, U, vmax_bits);
+    if (zero_rest)
+        hipLaunchKernelGGL(k_init_fluid<true>, rows(g.m4, g.n4, g.o4, 64), dim3(64), 0, s, g, P, O, I, U, vmax_bits);
+    else
+        hipLaunchKernelGGL(k_init_fluid<false>, rows(g.m4, g.n4, g.o4, 64), dim3(64), 0, s, g, P, O, I, U, vmax_bits);
 }
 void launch_init_pres(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U) {
     hipLaunchKernelGGL(k_init_pres, rows(g.m4 - 3, g.n4 - 3, g.o4 - 3, 64), dim3(64), 0, s, g, P, U);

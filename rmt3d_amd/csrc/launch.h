@@ -45,8 +45,10 @@ size_t face_struct_size();
 void launch_pp_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x, const double *mean);
 void launch_sub_pgrad(hipStream_t s, const Geo &g, const Fluid &U, double dt);
 void launch_divu_terms(hipStream_t s, const Geo &g, const double *r, double *out);
+// zero_rest: also clear the cell record's other fields (not needed while
+// they still hold the zeros of their allocation)
 void launch_init_fluid(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const InitSpec &is, const Fluid &U,
-                       unsigned long long *vmax_bits);
+                       unsigned long long *vmax_bits, int zero_rest = 1);
 void launch_init_pres(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U);
 void launch_init_refmap(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const InitSpec &is, const RMap &R,
                         int *status);
