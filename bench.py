@@ -405,7 +405,8 @@ def run_shear(args):
         dt = (time.perf_counter() - t0) / args.steps
         res[name] = {"ms_per_step": round(dt * 1e3, 3), "cell_steps_per_s": round(256 ** 3 / dt, 1),
                      "mac_vcycles_last_step": S.diag(1), "fem_vcycles_last_step": S.diag(2),
-                     "levels_mac_fem": [S.diag(10), S.diag(11)], "fdm_level": [S.diag(18), S.diag(19)]}
+                     "levels_mac_fem": [S.diag(10), S.diag(11)], "fdm_level": [S.diag(18), S.diag(19)],
+                     "extrapolation_object_batches": S.diag(30)}
         print("bench: shear %s %.1f ms/step" % (name, dt * 1e3), file=sys.stderr, flush=True)
         del S
     print(json.dumps({"metric": "FSI cell-steps/s, 256^3 shear workload (64 spheres)",

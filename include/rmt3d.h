@@ -246,7 +246,9 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * counterpart in the reference): 24/25 -> MAC/FEM finest-level colour-pass
  * workgroup size (0 = built-in default), 26/27 -> MAC/FEM fast-diagonalisation
  * outputs per work-item, pb_x + 10 pb_y + 100 pb_z, 28/29 -> 0 not tuned,
- * 1 tuned by measurement, 2 taken from the process's cache of that shape. */
+ * 1 tuned by measurement, 2 taken from the process's cache of that shape;
+ * 30 -> object batches of the last band extrapolation (objects whose
+ * extrapolation boxes are disjoint share one sequence of launches). */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
 /* Tracers (Utilities/tracers.{hh,cc}): with dump_code & 1 and ntracers > 0

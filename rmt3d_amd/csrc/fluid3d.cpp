@@ -836,6 +836,8 @@ Sim::~Sim() {
     if (h_status) hipHostFree(h_status);
     if (h_int) hipHostFree(h_int);
     if (h_dbl) hipHostFree(h_dbl);
+    if (W.h_box) hipHostFree(W.h_box);
+    if (W.h_batch) hipHostFree(W.h_batch);
     if (stream) hipStreamDestroy(stream);
 }
 
@@ -1019,6 +1021,9 @@ int Sim::create(const rmt3d_config *c, const rmt3d_object *o, int prec) {
     W.flag = (unsigned char *)dalloc(Ng);
     W.lid = (signed char *)dalloc(Ng);
     W.box = (int *)dalloc(sizeof(int) * 8 * std::max(1, n_obj));
+    W.batch = (BoxBatch *)dalloc(sizeof(BoxBatch) * std::max(1, n_obj));
+    HCHK(hipHostMalloc((void **)&W.h_box, sizeof(int) * 8 * std::max(1, n_obj), hipHostMallocDefault));
+    HCHK(hipHostMalloc((void **)&W.h_batch, sizeof(BoxBatch) * std::max(1, n_obj), hipHostMallocDefault));
     for (int a = 0; a < 3; a++) W.val[a] = (double *)dalloc(sizeof(double) * Ng);
     W.list[0] = (int *)dalloc(sizeof(int) * Ng);
     W.list[1] = (int *)dalloc(sizeof(int) * Ng);
@@ -1199,7 +1204,8 @@ int Sim::init_refmap() {
 
 void Sim::extrapolate() {
     W.tband = tmp;
-    band_extrapolate(stream, g, P, d_obj, R, ext, W, cfg.nlayers, cfg.max_extrap_rs, d_wpow, nmask, d_status, n_obj);
+    last_batches = band_extrapolate(stream, g, P, d_obj, R, ext, W, cfg.nlayers, cfg.max_extrap_rs, d_wpow, nmask,
+                                    d_status, n_obj);
     band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status);
 }
 

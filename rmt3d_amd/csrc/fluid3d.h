@@ -129,7 +129,8 @@ class Sim {
     ObjC *d_obj = nullptr;
     std::vector<ObjC> h_obj;
     Fluid U{};
-    bool fields_written = false;       // a step, upload or checkpoint read has written the fields
+    bool fields_written = false;
+    int last_batches = 0;              // object batches of the last extrapolation       // a step, upload or checkpoint read has written the fields
     RMap R{};
     Band ext{}, tmp{};
     Scratch W{};

@@ -149,6 +149,15 @@ __global__ __launch_bounds__(1024) void k_compact(Geo g, Scratch W, int *list, i
 }
 
 // ---- per-cell weighted least squares (fluid_3d_ex.cc:94-271, extrap_helper.cc:34-169) ----
+// The average unit normal of the object's previous-step face normals around
+// cell ind (fluid_3d_ex.cc:101-140): the cell's three lower faces and the
+// lower face of its +x, +y and +z neighbours, each from that cell's map of
+// the object in the previous band (get_refmap_prev, fluid_3d_utils.cc:44-55).
+// Called by a whole wavefront for one cell: lanes 0-3 look up the four cells'
+// maps and load their face normals at once (the lookups are chains of
+// dependent loads), then every lane adds them up in the reference's order,
+// cell by cell and face by face, from values broadcast by shuffles, so the
+// sums are the serial loop's.
 __device__ void prev_normal(const Geo &g, const RMap &R, const Band &T, int ind, unsigned oid, double gp[3],
                             double &gnorm) {
     int ii, jj, kk;
@@ -156,36 +165,53 @@ __device__ void prev_normal(const Geo &g, const RMap &R, const Band &T, int ind,
     gp[0] = gp[1] = gp[2] = 0;
     gnorm = 0;
     if (!(ii >= 2 && ii <= g.m4 - 3 && jj >= 2 && jj <= g.n4 - 3 && kk >= 2 && kk <= g.o4 - 3)) return;
-    int cells[4] = {ind, ind + 1, ind + g.m4, ind + g.mn4};
-    int nf = 0;
-    for (int pp = 0; pp < 4; pp++) {
-        int c = cells[pp];
-        // get_refmap_prev (fluid_3d_utils.cc:44-55): primary or temp band
-        const double *row[3];
-        bool found = false;
+    const int lane = __lane_id();
+    const int pp = lane < 4 ? lane : 0;
+    const int c = pp == 0 ? ind : pp == 1 ? ind + 1 : pp == 2 ? ind + g.m4 : ind + g.mn4;
+    int found = 0;
+    double gv[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int f = 0; f < 3; f++) gv[a][f] = 0;
+    if (lane < 4) {
         int ref = -1;
-        if (R.c[c] == oid) { found = true; ref = c; }
-        else {
-            int s = T.start[c], e = s + T.cnt[c];
-            for (int a = s; a < e; a++)
-                if ((T.c[a] & OMASK) == oid) { found = true; ref = -2 - a; break; }
+        if (R.c[c] == oid) {
+            found = 1;
+            ref = c;
+        } else {
+            int s0 = T.start[c], e0 = s0 + T.cnt[c];
+            for (int a = s0; a < e0; a++)
+                if ((T.c[a] & OMASK) == oid) { found = 1; ref = -2 - a; break; }
         }
-        if (!found) continue;
-        int r0 = (pp == 0) ? 0 : pp - 1, r1 = (pp == 0) ? 3 : pp;
+        if (found) {
+            // cell 0 contributes its three lower faces, cells 1-3 the face
+            // dd = pp - 1 (their lower face towards cell 0)
+            for (int sl = 0; sl < 3; sl++) {
+                int dd = pp == 0 ? sl : pp - 1;
+                if (pp != 0 && sl > 0) break;
+                for (int ff = 0; ff < 3; ff++)
+                    gv[sl][ff] = (ref >= 0) ? R.gphi[3 * dd + ff][ref] : T.gphi[3 * dd + ff][-2 - ref];
+            }
+        }
+    }
+    int nf = 0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        if (!__shfl(found, p, 64)) continue;
+        const int r0 = (p == 0) ? 0 : p - 1, r1 = (p == 0) ? 3 : p;
+#pragma unroll
         for (int dd = r0; dd < r1; dd++) {
-            double gv[3];
-            for (int ff = 0; ff < 3; ff++)
-                gv[ff] = (ref >= 0) ? R.gphi[3 * dd + ff][ref] : T.gphi[3 * dd + ff][-2 - ref];
-            if (gv[dd] != -1.e200) {
+            const int sl = (p == 0) ? dd : 0;
+            double v[3];
+            for (int ff = 0; ff < 3; ff++) v[ff] = __shfl(gv[sl][ff], p, 64);
+            if (v[dd] != -1.e200) {
                 bool add = false;
                 for (int ff = 0; ff < 3; ff++) {
-                    if (gv[ff] != 0.0) add = true;
-                    gp[ff] += gv[ff];
+                    if (v[ff] != 0.0) add = true;
+                    gp[ff] += v[ff];
                 }
                 if (add) nf++;
             }
         }
-        (void)row;
     }
     if (nf == 6) {
         double nrm = 0;
@@ -205,7 +231,6 @@ __device__ void prev_normal(const Geo &g, const RMap &R, const Band &T, int ind,
 struct LsqArgs {
     int L;
     int max_rs;
-    uint32_t oid;
     const double *wpow;
 };
 
@@ -249,7 +274,7 @@ constexpr int LSQ_T1 = 7;    // ravg[3], davg[3], wsum
 constexpr int LSQ_T2 = 15;   // am[6], rhs[9]
 
 __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt,
-                                                   LsqArgs A, int *status) {
+                                                   LsqArgs A, const BoxBatch *__restrict__ B, int *status) {
     __shared__ double term[64][LSQ_T2 + 1];
     __shared__ double bc[LSQ_T2];
     const int lane = threadIdx.x;
@@ -259,8 +284,17 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
         if (!within_extrap(g, ind)) continue;
         int ii, jj, kk;
         unpack(g, ind, ii, jj, kk);
+        // the cell's object: the one batch box that holds it (lane b tests box b)
+        bool inb = false;
+        if (lane < B->n) {
+            const int *lo = B->lo[lane], *ex = B->ext[lane];
+            inb = ii >= lo[0] && ii < lo[0] + ex[0] && jj >= lo[1] && jj < lo[1] + ex[1] && kk >= lo[2] &&
+                  kk < lo[2] + ex[2];
+        }
+        unsigned long long hit = __ballot(inb);
+        unsigned oid = hit ? (unsigned)B->oid[__ffsll((long long)hit) - 1] : 0u;
         double gp[3], gnorm;
-        prev_normal(g, R, T, ind, A.oid, gp, gnorm);
+        prev_normal(g, R, T, ind, oid, gp, gnorm);
         bool cosw = (gnorm != 0. && A.L <= 2);
         bool ok = false;
         double davg[3], ravg[3], coeff[9];
@@ -508,23 +542,48 @@ __global__ void k_box_finish(Geo g, int *bb, int n_obj, int grow) {
     b[7] = 0;
 }
 
-__device__ __forceinline__ int box_cell(const Geo &g, const int *bx, int t) {
-    int e0 = bx[3], e1 = bx[4];
-    int r = t / e0;
-    int i = t - r * e0;
-    int k = r / e1;
-    int j = r - k * e1;
-    return IXb(g, bx[0] + i, bx[1] + j, bx[2] + k);
+// A batch is a set of objects whose boxes are pairwise disjoint.  Their
+// extrapolations touch disjoint cells of the scratch (each stays in its own
+// box) and read nothing the others write, so one sequence of launches runs
+// all of them, each cell knowing its object from the box it lies in.  The
+// mask tags of a pass are shared by the batch's objects; a tag only has to
+// differ from every earlier tag within the object's own box.
+// A cell of the batch: work-item t of [0, total) -> (box b, padded index)
+__device__ __forceinline__ int batch_cell(const Geo &g, const BoxBatch *__restrict__ B, int t, int &b) {
+    if (B->n == 1) {   // one box: every index below is uniform
+        b = 0;
+        int e0 = B->ext[0][0], e1 = B->ext[0][1];
+        int q = t / e0;
+        int i = t - q * e0;
+        int k = q / e1;
+        int j = q - k * e1;
+        return IXb(g, B->lo[0][0] + i, B->lo[0][1] + j, B->lo[0][2] + k);
+    }
+    int lo = 0, hi = B->n - 1;
+    while (lo < hi) {   // the last box whose start is <= t
+        int mid = (lo + hi + 1) >> 1;
+        if (B->start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    b = lo;
+    int r = t - B->start[lo];
+    int e0 = B->ext[lo][0], e1 = B->ext[lo][1];
+    int q = r / e0;
+    int i = r - q * e0;
+    int k = q / e1;
+    int j = q - k * e1;
+    return IXb(g, B->lo[lo][0] + i, B->lo[lo][1] + j, B->lo[lo][2] + k);
 }
 
-#define BOX_LOOP(bx)                                                                       \
-    const int vol_ = (bx)[6];                                                              \
-    for (int t_ = blockIdx.x * blockDim.x + threadIdx.x; t_ < vol_; t_ += gridDim.x * blockDim.x)
+#define BATCH_LOOP(B)                                                                      \
+    const int tot_ = (B)->total;                                                           \
+    for (int t_ = blockIdx.x * blockDim.x + threadIdx.x; t_ < tot_; t_ += gridDim.x * blockDim.x)
 
-__global__ void k_obj_init_box(Geo g, RMap R, Scratch W, const int *bx, uint32_t oid, uint32_t nmask) {
-    BOX_LOOP(bx) {
-        int t = box_cell(g, bx, t_);
-        if (R.c[t] == oid) {
+__global__ void k_batch_init(Geo g, RMap R, Scratch W, const BoxBatch *__restrict__ B, uint32_t nmask) {
+    BATCH_LOOP(B) {
+        int b;
+        int t = batch_cell(g, B, t_, b);
+        if (R.c[t] == (uint32_t)B->oid[b]) {
             W.lid[t] = 0;
             for (int a = 0; a < 3; a++) W.val[a][t] = R.xpred[a][t];
             W.mask[t] = nmask;
@@ -534,39 +593,44 @@ __global__ void k_obj_init_box(Geo g, RMap R, Scratch W, const int *bx, uint32_t
         W.flag[t] = 0;
     }
 }
-__global__ void k_first_push_box(Geo g, Scratch W, const int *bx, uint32_t cur_mask) {
-    BOX_LOOP(bx) {
-        int ind = box_cell(g, bx, t_);
+__global__ void k_batch_first_push(Geo g, Scratch W, const BoxBatch *__restrict__ B, uint32_t cur_mask) {
+    BATCH_LOOP(B) {
+        int b;
+        int ind = batch_cell(g, B, t_, b);
         if (within_inner_ghost(g, ind) && W.mask[ind] == cur_mask) push6(g, W, ind, cur_mask);
     }
 }
-__global__ void k_apply_flags_box(Geo g, Scratch W, const int *bx, uint32_t nmask) {
-    BOX_LOOP(bx) {
-        int t = box_cell(g, bx, t_);
+__global__ void k_batch_apply_flags(Geo g, Scratch W, const BoxBatch *__restrict__ B, uint32_t nmask) {
+    BATCH_LOOP(B) {
+        int b;
+        int t = batch_cell(g, B, t_, b);
         if (W.flag[t]) W.mask[t] = nmask;
     }
 }
-__global__ void k_comm_mask_box(Geo g, Scratch W, const int *bx) {
-    BOX_LOOP(bx) {
-        int t = box_cell(g, bx, t_);
+__global__ void k_batch_comm_mask(Geo g, Scratch W, const BoxBatch *__restrict__ B) {
+    BATCH_LOOP(B) {
+        int b;
+        int t = batch_cell(g, B, t_, b);
         int i, j, k;
         unpack(g, t, i, j, k);
         int s = psrc(g, i, j, k);
         if (s >= 0) W.mask[t] = W.mask[s];
     }
 }
-__global__ void k_scan_ghost_box(Geo g, Scratch W, const int *bx, uint32_t nmask) {
-    BOX_LOOP(bx) {
-        int t = box_cell(g, bx, t_);
+__global__ void k_batch_scan_ghost(Geo g, Scratch W, const BoxBatch *__restrict__ B, uint32_t nmask) {
+    BATCH_LOOP(B) {
+        int b;
+        int t = batch_cell(g, B, t_, b);
         int i, j, k;
         unpack(g, t, i, j, k);
         bool ghost = (i < 2 || i > g.m + 1 || j < 2 || j > g.n + 1 || k < 2 || k > g.o + 1);
         if (ghost && W.mask[t] == nmask) W.flag[t] = 1;
     }
 }
-__global__ void k_wrap_layer_box(Geo g, Scratch W, const int *bx, int L) {
-    BOX_LOOP(bx) {
-        int t = box_cell(g, bx, t_);
+__global__ void k_batch_wrap_layer(Geo g, Scratch W, const BoxBatch *__restrict__ B, int L) {
+    BATCH_LOOP(B) {
+        int b;
+        int t = batch_cell(g, B, t_, b);
         int i, j, k;
         unpack(g, t, i, j, k);
         int s = psrc(g, i, j, k);
@@ -576,13 +640,13 @@ __global__ void k_wrap_layer_box(Geo g, Scratch W, const int *bx, int L) {
     }
 }
 // the workgroup-uniform loops keep every thread at each block_append
-__global__ __launch_bounds__(1024) void k_compact_box(Geo g, Scratch W, const int *bx, int *list, int *cnt) {
+__global__ __launch_bounds__(1024) void k_batch_compact(Geo g, Scratch W, const BoxBatch *__restrict__ B, int *list, int *cnt) {
     __shared__ int sh[17];
-    const int vol = bx[6];
-    for (int b0 = blockIdx.x * blockDim.x; b0 < vol; b0 += gridDim.x * blockDim.x) {
-        int tt = b0 + threadIdx.x;
-        int t = tt < vol ? box_cell(g, bx, tt) : 0;
-        bool f = tt < vol && W.flag[t] != 0;
+    const int tot = B->total;
+    for (int b0 = blockIdx.x * blockDim.x; b0 < tot; b0 += gridDim.x * blockDim.x) {
+        int tt = b0 + threadIdx.x, b;
+        int t = tt < tot ? batch_cell(g, B, tt, b) : 0;
+        bool f = tt < tot && W.flag[t] != 0;
         int p = block_append(f, cnt, sh);
         if (f) {
             list[p] = t;
@@ -590,18 +654,18 @@ __global__ __launch_bounds__(1024) void k_compact_box(Geo g, Scratch W, const in
         }
     }
 }
-__global__ __launch_bounds__(1024) void k_stage_obj_box(Geo g, Scratch W, const int *bx, uint32_t oid, int *status) {
+__global__ __launch_bounds__(1024) void k_batch_stage(Geo g, Scratch W, const BoxBatch *__restrict__ B, int *status) {
     __shared__ int sh[17];
-    const int vol = bx[6];
-    for (int b0 = blockIdx.x * blockDim.x; b0 < vol; b0 += gridDim.x * blockDim.x) {
-        int tt = b0 + threadIdx.x;
-        int t = tt < vol ? box_cell(g, bx, tt) : 0;
-        int L = tt < vol ? W.lid[t] : 0;
+    const int tot = B->total;
+    for (int b0 = blockIdx.x * blockDim.x; b0 < tot; b0 += gridDim.x * blockDim.x) {
+        int tt = b0 + threadIdx.x, b = 0;
+        int t = tt < tot ? batch_cell(g, B, tt, b) : 0;
+        int L = tt < tot ? W.lid[t] : 0;
         int p = block_append(L >= 1, W.st_n, sh);
         if (L < 1) continue;
         if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); continue; }
         W.st_cell[p] = t;
-        W.st_c[p] = oid | ((uint32_t)L << LDIG);
+        W.st_c[p] = (uint32_t)B->oid[b] | ((uint32_t)L << LDIG);
         for (int a = 0; a < 3; a++) { W.st_x[a][p] = W.val[a][t]; W.st_xpred[a][p] = W.val[a][t]; }
     }
 }
@@ -760,57 +824,114 @@ void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, in
 // fluid_3d::extrapolate (fluid_3d_ex.cc:31-47); the caller has swapped the
 // previous band into the temp slot.  nmask is the host copy of the mask
 // counter (extrap_helper::nmask), advanced exactly as the reference does.
-void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
-                      int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj) {
+// one object: its batch is built on the device, with no host read-back
+__global__ void k_batch_single(const int *bx, BoxBatch *B) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    B->n = 1;
+    B->oid[0] = 1;
+    for (int d = 0; d < 3; d++) { B->lo[0][d] = bx[d]; B->ext[0][d] = bx[3 + d]; }
+    B->start[0] = 0;
+    B->start[1] = bx[6];
+    B->total = bx[6];
+}
+
+// Objects in index order go to the first batch whose boxes they do not
+// overlap (a box spans [lo, lo + ext) in each dimension); empty boxes (no
+// primary cell) take no part.
+static int make_batches(const int *bx, int n_obj, BoxBatch *out) {
+    int nb = 0;
+    for (int o = 0; o < n_obj; o++) {
+        const int *b = bx + BOX_INTS * o;
+        if (b[6] <= 0) continue;
+        int dst = -1;
+        for (int q = 0; q < nb && dst < 0; q++) {
+            bool clash = false;
+            for (int e = 0; e < out[q].n && !clash; e++) {
+                bool ov = true;
+                for (int d = 0; d < 3; d++)
+                    ov = ov && b[d] < out[q].lo[e][d] + out[q].ext[e][d] && out[q].lo[e][d] < b[d] + b[3 + d];
+                clash = ov;
+            }
+            if (!clash) dst = q;
+        }
+        if (dst < 0) {
+            dst = nb++;
+            out[dst].n = 0;
+            out[dst].start[0] = 0;
+        }
+        BoxBatch &q = out[dst];
+        int e = q.n++;
+        q.oid[e] = o + 1;
+        for (int d = 0; d < 3; d++) { q.lo[e][d] = b[d]; q.ext[e][d] = b[3 + d]; }
+        q.start[e + 1] = q.start[e] + b[6];
+        q.total = q.start[e + 1];
+    }
+    return nb;
+}
+
+int band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
+                     int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj) {
     (void)P; (void)O;
     launch_zero_i32(s, W.st_n, 1);
     bool per = g.prd[0] || g.prd[1] || g.prd[2];
     // temp band needed by prev_normal is passed in B's place by the caller
     // through W; see fluid3d.hip
     (void)B;
-    if (n_obj < 1) return;
-    // the objects' boxes, from their primary cells
+    if (n_obj < 1) return 0;
+    // the objects' boxes, from their primary cells, then (on the host) the
+    // batches of objects with disjoint boxes
     hipLaunchKernelGGL(k_box_init, dim3((n_obj + 63) / 64), dim3(64), 0, s, W.box, n_obj);
     hipLaunchKernelGGL(k_box_reduce, dim3(std::min(1024, (g.Ng + 255) / 256)), dim3(256), 0, s, g, R, W.box, n_obj);
     hipLaunchKernelGGL(k_box_finish, dim3((n_obj + 63) / 64), dim3(64), 0, s, g, W.box, n_obj, nlayers + max_rs + 2);
+    int nb = 1;
+    if (n_obj == 1) {
+        hipLaunchKernelGGL(k_batch_single, dim3(1), dim3(64), 0, s, (const int *)W.box, W.batch);
+    } else {
+        hipMemcpyAsync(W.h_box, W.box, sizeof(int) * BOX_INTS * n_obj, hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        nb = make_batches(W.h_box, n_obj, W.h_batch);
+        if (nb == 0) return 0;
+        hipMemcpyAsync(W.batch, W.h_batch, sizeof(BoxBatch) * nb, hipMemcpyHostToDevice, s);
+    }
     const dim3 ge(std::min(1024, (g.Ng + 255) / 256)), gc(std::min(256, (g.Ng + 1023) / 1024));
-    for (int oid = 1; oid <= n_obj; oid++) {
-        const int *bx = W.box + BOX_INTS * (oid - 1);
+    for (int q = 0; q < nb; q++) {
+        const BoxBatch *bb = W.batch + q;
         if (nmask > UINT_MAX - 50) {
             nmask = 1;
             hipLaunchKernelGGL(k_mask_reset, g1(g.Ng), dim3(256), 0, s, g, W);
         }
         uint32_t cur_mask = nmask;
-        hipLaunchKernelGGL(k_obj_init_box, ge, dim3(256), 0, s, g, R, W, bx, (uint32_t)oid, nmask);
+        hipLaunchKernelGGL(k_batch_init, ge, dim3(256), 0, s, g, R, W, bb, nmask);
         nmask++;
-        hipLaunchKernelGGL(k_first_push_box, ge, dim3(256), 0, s, g, W, bx, cur_mask);
-        hipLaunchKernelGGL(k_apply_flags_box, ge, dim3(256), 0, s, g, W, bx, nmask);
-        if (per) hipLaunchKernelGGL(k_comm_mask_box, ge, dim3(256), 0, s, g, W, bx);
-        hipLaunchKernelGGL(k_scan_ghost_box, ge, dim3(256), 0, s, g, W, bx, nmask);
+        hipLaunchKernelGGL(k_batch_first_push, ge, dim3(256), 0, s, g, W, bb, cur_mask);
+        hipLaunchKernelGGL(k_batch_apply_flags, ge, dim3(256), 0, s, g, W, bb, nmask);
+        if (per) hipLaunchKernelGGL(k_batch_comm_mask, ge, dim3(256), 0, s, g, W, bb);
+        hipLaunchKernelGGL(k_batch_scan_ghost, ge, dim3(256), 0, s, g, W, bb, nmask);
         int cur = 0;
         launch_zero_i32(s, W.cnt + cur, 1);
-        hipLaunchKernelGGL(k_compact_box, gc, dim3(1024), 0, s, g, W, bx, W.list[cur], W.cnt + cur);
+        hipLaunchKernelGGL(k_batch_compact, gc, dim3(1024), 0, s, g, W, bb, W.list[cur], W.cnt + cur);
         for (int L = 1; L <= nlayers; L++) {
             nmask++;
             LsqArgs A;
-            A.L = L; A.max_rs = max_rs; A.oid = (uint32_t)oid; A.wpow = wpow_dev;
+            A.L = L; A.max_rs = max_rs; A.wpow = wpow_dev;
             hipLaunchKernelGGL(k_gather_lsq, dim3(8192), dim3(64), 0, s, g, R, W.tband, W,
-                               W.list[cur], W.cnt + cur, A, status);
+                               W.list[cur], W.cnt + cur, A, bb, status);
             int nxt = cur ^ 1;
             launch_zero_i32(s, W.cnt + nxt, 1);
             if (L < nlayers) {
                 hipLaunchKernelGGL(k_list_push, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, W, W.list[cur],
                                    W.cnt + cur, cur_mask);
-                hipLaunchKernelGGL(k_apply_flags_box, ge, dim3(256), 0, s, g, W, bx, nmask);
-                if (per) hipLaunchKernelGGL(k_comm_mask_box, ge, dim3(256), 0, s, g, W, bx);
-                hipLaunchKernelGGL(k_scan_ghost_box, ge, dim3(256), 0, s, g, W, bx, nmask);
-                hipLaunchKernelGGL(k_compact_box, gc, dim3(1024), 0, s, g, W, bx, W.list[nxt], W.cnt + nxt);
+                hipLaunchKernelGGL(k_batch_apply_flags, ge, dim3(256), 0, s, g, W, bb, nmask);
+                if (per) hipLaunchKernelGGL(k_batch_comm_mask, ge, dim3(256), 0, s, g, W, bb);
+                hipLaunchKernelGGL(k_batch_scan_ghost, ge, dim3(256), 0, s, g, W, bb, nmask);
+                hipLaunchKernelGGL(k_batch_compact, gc, dim3(1024), 0, s, g, W, bb, W.list[nxt], W.cnt + nxt);
             }
             cur = nxt;
-            if (per) hipLaunchKernelGGL(k_wrap_layer_box, ge, dim3(256), 0, s, g, W, bx, L);
+            if (per) hipLaunchKernelGGL(k_batch_wrap_layer, ge, dim3(256), 0, s, g, W, bb, L);
         }
-        hipLaunchKernelGGL(k_stage_obj_box, gc, dim3(1024), 0, s, g, W, bx, (uint32_t)oid, status);
+        hipLaunchKernelGGL(k_batch_stage, gc, dim3(1024), 0, s, g, W, bb, status);
     }
+    return nb;
 }
 
 // ---- predictor, corrector, full update ----

@@ -165,6 +165,16 @@ void launch_zero_u64(hipStream_t s, unsigned long long *p, int n);
 void launch_zero_i32(hipStream_t s, int *p, int n);
 
 // ---- band / extrapolation (k_band.hip) ----
+// objects whose extrapolation boxes are pairwise disjoint, run together
+constexpr int BATCH_MAX = 64;
+struct BoxBatch {
+    int n;                       // objects in the batch
+    int total;                   // cells of all their boxes
+    int oid[BATCH_MAX];          // object ids (1-based)
+    int lo[BATCH_MAX][3], ext[BATCH_MAX][3];
+    int start[BATCH_MAX + 1];    // first work-item of each box
+};
+
 struct Scratch {
     uint32_t *mask;
     unsigned char *flag;
@@ -184,11 +194,15 @@ struct Scratch {
     int *scan_tmp;       // block sums
     int *err;            // corrector problem count
     int *box;            // per object: the extrapolation's index box (8 ints)
+    int *h_box;          // its pinned host copy
+    BoxBatch *batch;     // batches of objects with disjoint boxes (device)
+    BoxBatch *h_batch;   // the same, pinned host
     Band tband;          // previous band (get_refmap_prev)
 };
 
-void band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
-                      int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj);
+// returns the number of object batches it ran
+int band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
+                     int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj);
 // carry: take each entry's face energies and det F from the staging (a
 // redraw moves existing maps) instead of marking them not computed (fresh
 // extrapolated maps)
