@@ -1103,6 +1103,11 @@ Maps Sim::maps() const {
 int Sim::check_status(const char *where) {
     HCHK(hipMemcpyAsync(h_status, d_status, sizeof(int), hipMemcpyDeviceToHost, stream));
     HCHK(hipStreamSynchronize(stream));
+    if (!hip_pending.empty()) {
+        err = std::string(where) + ": " + hip_pending;
+        hip_pending.clear();
+        return RMT3D_ERR_HIP;
+    }
     int s = *h_status;
     if (!s) return 0;
     err = std::string(where) + ": ";
@@ -1206,6 +1211,7 @@ void Sim::extrapolate() {
     W.tband = tmp;
     last_batches = band_extrapolate(stream, g, P, d_obj, R, ext, W, cfg.nlayers, cfg.max_extrap_rs, d_wpow, nmask,
                                     d_status, n_obj);
+    if (last_batches < 0) hip_pending = "band_extrapolate: copying the object boxes failed";
     band_build_from_staging(stream, g, ext, W, cfg.nlayers, d_status);
 }
 

@@ -110,6 +110,8 @@ class Sim {
     std::vector<rmt3d_object> objs_in;
     int precision = RMT3D_PREC_FP64;
     std::string err;
+    // a failed host-device copy outside a checked call, reported by check_status
+    std::string hip_pending;
 
     // geometry and constants
     Geo g{};

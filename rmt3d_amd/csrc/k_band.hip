@@ -169,50 +169,56 @@ __device__ void prev_normal(const Geo &g, const RMap &R, const Band &T, int ind,
     const int pp = lane < 4 ? lane : 0;
     const int c = pp == 0 ? ind : pp == 1 ? ind + 1 : pp == 2 ? ind + g.m4 : ind + g.mn4;
     int found = 0;
-    double gv[3][3];
-    for (int a = 0; a < 3; a++)
-        for (int f = 0; f < 3; f++) gv[a][f] = 0;
+    // face normals of the lane's cell, in scalars (no private arrays): cell 0
+    // contributes its three lower faces (a*, b*, c* = faces 0, 1, 2), cells
+    // 1-3 the face pp - 1, their lower face towards cell 0 (a*)
+    double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, cx = 0, cy = 0, cz = 0;
     if (lane < 4) {
-        int ref = -1;
+        int at = -1;
         if (R.c[c] == oid) {
             found = 1;
-            ref = c;
+            at = c;
         } else {
             int s0 = T.start[c], e0 = s0 + T.cnt[c];
             for (int a = s0; a < e0; a++)
-                if ((T.c[a] & OMASK) == oid) { found = 1; ref = -2 - a; break; }
+                if ((T.c[a] & OMASK) == oid) { found = 1; at = -2 - a; break; }
         }
+        // component f of the cell's previous-step face normals
+        auto ld = [&](int f) { return at >= 0 ? R.gphi[f][at] : T.gphi[f][-2 - at]; };
         if (found) {
-            // cell 0 contributes its three lower faces, cells 1-3 the face
-            // dd = pp - 1 (their lower face towards cell 0)
-            for (int sl = 0; sl < 3; sl++) {
-                int dd = pp == 0 ? sl : pp - 1;
-                if (pp != 0 && sl > 0) break;
-                for (int ff = 0; ff < 3; ff++)
-                    gv[sl][ff] = (ref >= 0) ? R.gphi[3 * dd + ff][ref] : T.gphi[3 * dd + ff][-2 - ref];
+            const int d0 = pp == 0 ? 0 : pp - 1;
+            ax = ld(3 * d0);
+            ay = ld(3 * d0 + 1);
+            az = ld(3 * d0 + 2);
+            if (pp == 0) {
+                bx = ld(3); by = ld(4); bz = ld(5);
+                cx = ld(6); cy = ld(7); cz = ld(8);
             }
         }
     }
     int nf = 0;
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        if (!__shfl(found, p, 64)) continue;
-        const int r0 = (p == 0) ? 0 : p - 1, r1 = (p == 0) ? 3 : p;
-#pragma unroll
-        for (int dd = r0; dd < r1; dd++) {
-            const int sl = (p == 0) ? dd : 0;
-            double v[3];
-            for (int ff = 0; ff < 3; ff++) v[ff] = __shfl(gv[sl][ff], p, 64);
-            if (v[dd] != -1.e200) {
-                bool add = false;
-                for (int ff = 0; ff < 3; ff++) {
-                    if (v[ff] != 0.0) add = true;
-                    gp[ff] += v[ff];
-                }
-                if (add) nf++;
-            }
-        }
+    double sx = 0, sy = 0, sz = 0;
+    // one face normal (x, y, z) of face dd: skipped when its dd component
+    // carries the unset marker, counted when any component is nonzero
+    auto face = [&](double x, double y, double z, int dd) {
+        const double vd = dd == 0 ? x : dd == 1 ? y : z;
+        if (vd == -1.e200) return;
+        sx += x;
+        sy += y;
+        sz += z;
+        if (x != 0.0 || y != 0.0 || z != 0.0) nf++;
+    };
+    if (__shfl(found, 0, 64)) {
+        face(__shfl(ax, 0, 64), __shfl(ay, 0, 64), __shfl(az, 0, 64), 0);
+        face(__shfl(bx, 0, 64), __shfl(by, 0, 64), __shfl(bz, 0, 64), 1);
+        face(__shfl(cx, 0, 64), __shfl(cy, 0, 64), __shfl(cz, 0, 64), 2);
     }
+#pragma unroll
+    for (int p = 1; p < 4; p++)
+        if (__shfl(found, p, 64)) face(__shfl(ax, p, 64), __shfl(ay, p, 64), __shfl(az, p, 64), p - 1);
+    gp[0] = sx;
+    gp[1] = sy;
+    gp[2] = sz;
     if (nf == 6) {
         double nrm = 0;
         for (int a = 0; a < 3; a++) { gp[a] /= nf; nrm += gp[a] * gp[a]; }
@@ -887,11 +893,13 @@ int band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, 
     if (n_obj == 1) {
         hipLaunchKernelGGL(k_batch_single, dim3(1), dim3(64), 0, s, (const int *)W.box, W.batch);
     } else {
-        hipMemcpyAsync(W.h_box, W.box, sizeof(int) * BOX_INTS * n_obj, hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        if (hipMemcpyAsync(W.h_box, W.box, sizeof(int) * BOX_INTS * n_obj, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return -1;
         nb = make_batches(W.h_box, n_obj, W.h_batch);
         if (nb == 0) return 0;
-        hipMemcpyAsync(W.batch, W.h_batch, sizeof(BoxBatch) * nb, hipMemcpyHostToDevice, s);
+        if (hipMemcpyAsync(W.batch, W.h_batch, sizeof(BoxBatch) * nb, hipMemcpyHostToDevice, s) != hipSuccess)
+            return -1;
     }
     const dim3 ge(std::min(1024, (g.Ng + 255) / 256)), gc(std::min(256, (g.Ng + 1023) / 1024));
     for (int q = 0; q < nb; q++) {

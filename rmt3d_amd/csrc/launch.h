@@ -200,7 +200,8 @@ struct Scratch {
     Band tband;          // previous band (get_refmap_prev)
 };
 
-// returns the number of object batches it ran
+// returns the number of object batches it ran, or -1 when copying the
+// object boxes between host and device failed
 int band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W,
                      int nlayers, int max_rs, const double *wpow_dev, uint32_t &nmask, int *status, int n_obj);
 // carry: take each entry's face energies and det F from the staging (a
