@@ -189,7 +189,8 @@ def run_gpu(args, precision):
            "autotuned": {"mac_passes_per_launch": sim.diag(16), "fem_passes_per_launch": sim.diag(17),
                          "mac_colour_pass_workgroup": sim.diag(24), "fem_colour_pass_workgroup": sim.diag(25),
                          "mac_fdm_outputs_per_item_xyz": int(sim.diag(26)),
-                         "fem_fdm_outputs_per_item_xyz": int(sim.diag(27))},
+                         "fem_fdm_outputs_per_item_xyz": int(sim.diag(27)),
+                         "mac_fdm_three_launch": int(sim.diag(31)), "fem_fdm_three_launch": int(sim.diag(32))},
            "stress_ms_per_step": prof["stress"][0] / max(args.kernel_steps, 1), "stage_ms_per_step": stages,
            "fields": {k: fields[k] for k in ("u", "v", "w", "p")}}
     sim.close()

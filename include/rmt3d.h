@@ -248,7 +248,10 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * outputs per work-item, pb_x + 10 pb_y + 100 pb_z, 28/29 -> 0 not tuned,
  * 1 tuned by measurement, 2 taken from the process's cache of that shape;
  * 30 -> object batches of the last band extrapolation (objects whose
- * extrapolation boxes are disjoint share one sequence of launches). */
+ * extrapolation boxes are disjoint share one sequence of launches);
+ * 31/32 -> MAC/FEM fast-diagonalisation form in the V-cycles' precision
+ * (0 one launch per contraction, 1 three launches: x-y planes, z pencils,
+ * planes), picked by the autotuner where the three-launch form fits. */
 int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out);
 
 /* Tracers (Utilities/tracers.{hh,cc}): with dump_code & 1 and ntracers > 0

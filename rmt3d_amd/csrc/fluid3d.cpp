@@ -304,7 +304,17 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         }
     }
     mg_sweep_enable();
-    if (!fdm) {
+    if (fdm) {
+        // the FP32 fast-diagonalisation solve (mixed path) runs in three
+        // MFMA launches when the level's planes, pencil blocks and bases fit
+        // a workgroup's LDS; RMT3D_FDM_FUSED=0 keeps one launch per
+        // contraction.  The FP64 solve always takes the six launches.
+        const char *ff = getenv("RMT3D_FDM_FUSED");
+        bool allow = !(ff && ff[0] == '0');
+        fdm64.fused = 0;
+        fdm32.fused = (allow && mixed) ? mg_fdm_fused_enable(L[nlev - 1].d) : 0;
+        fdm_fused_ok = 2 * fdm32.fused;
+    } else {
         tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
         tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
     }
@@ -416,7 +426,8 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
 //     4 FEM; 1 or 2 MAC) and, for single-pass launches, the workgroup size
 //     (64 ... 1024) of the colour-pass kernel;
 //   - per dimension of a fast-diagonalisation level: outputs per work-item
-//     (2, 4, 8) of its contractions.
+//     (2, 4, 8) of its contractions, and then that six-launch form against
+//     the three-launch one.
 // Every candidate performs the same operations in the same order on every
 // unknown (the fused sweeps are bit-identical to single-colour passes), so a
 // choice changes time only, never results.  Each candidate is timed over a
@@ -432,6 +443,7 @@ struct TuneChoice {
     int npass[16];
     int bs[16];
     int pb[3];
+    int fused;
 };
 std::mutex g_tune_mu;
 std::map<std::vector<int>, TuneChoice> g_tune_cache;
@@ -474,6 +486,10 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
     // every other level keeping its current (default or chosen) shape.
     auto iteration = [&]() { cycle_body(s); };
     if (fdm) {
+        // the six-launch form's outputs per work-item, then that form
+        // against the three-launch one (where it fits)
+        const int fused_ok = F.fused;
+        F.fused = 0;
         for (int a = 0; a < 3; a++) {
             static const int pbs[3] = {2, 4, 8};
             int keep = F.pb[a];
@@ -482,6 +498,13 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
                 iteration();
             });
             F.pb[a] = t[argmin(t)] < 1e29 ? pbs[argmin(t)] : keep;
+        }
+        if (fused_ok) {
+            std::vector<double> t = time_candidates(s, 2, [&](int k) {
+                F.fused = k;
+                iteration();
+            });
+            F.fused = t[argmin(t)] < 1e29 ? argmin(t) : fused_ok;
         }
     }
     for (int l = last - 1; l >= 0; l--) {
@@ -526,7 +549,7 @@ void MGSolver::tune(hipStream_t s) {
     const char *tv = getenv("RMT3D_TUNE");
     if (tv && tv[0] == '0') return;
     bool sweep_fixed = getenv("RMT3D_SWEEP") != nullptr;
-    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, nlev, fdm ? 1 : 0,
+    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, fdm_fused_ok, nlev, fdm ? 1 : 0,
                             prd[0], prd[1], prd[2]};
     for (int l = 0; l < nlev; l++)
         for (int a = 0; a < 3; a++) key.push_back(L[l].d[a]);
@@ -536,6 +559,8 @@ void MGSolver::tune(hipStream_t s) {
         if (it != g_tune_cache.end()) {
             for (int l = 0; l < 16; l++) { npass[l] = it->second.npass[l]; smooth_bs[l] = it->second.bs[l]; }
             for (int a = 0; a < 3; a++) fdm64.pb[a] = fdm32.pb[a] = it->second.pb[a];
+            fdm64.fused = (fdm_fused_ok & 1) ? it->second.fused : 0;
+            fdm32.fused = (fdm_fused_ok & 2) ? it->second.fused : 0;
             tuned = 2;
             return;
         }
@@ -543,15 +568,18 @@ void MGSolver::tune(hipStream_t s) {
     if (mixed) {
         tune_levels<float>(s, Lf, fdm32, sweep_fixed);
         for (int a = 0; a < 3; a++) fdm64.pb[a] = fdm32.pb[a];
+        fdm64.fused = (fdm_fused_ok & 1) ? fdm32.fused : 0;
     } else {
         tune_levels<double>(s, L, fdm64, sweep_fixed);
         for (int a = 0; a < 3; a++) fdm32.pb[a] = fdm64.pb[a];
+        fdm32.fused = (fdm_fused_ok & 2) ? fdm64.fused : 0;
     }
     hipStreamSynchronize(s);
     (void)hipGetLastError();
     TuneChoice ch;
     for (int l = 0; l < 16; l++) { ch.npass[l] = npass[l]; ch.bs[l] = smooth_bs[l]; }
     for (int a = 0; a < 3; a++) ch.pb[a] = fdm64.pb[a];
+    ch.fused = mixed ? fdm32.fused : fdm64.fused;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = ch;
     tuned = 1;

@@ -1150,8 +1150,11 @@ int mg_tail_enable_lds(size_t bytes) {
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
     if (bytes > (size_t)maxb) return 0;
-    // raising the dynamic limit is required on some runtimes and a no-op on others
-    (void)hipFuncSetAttribute((const void *)k_tail<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    // raising the dynamic limit is required on some runtimes and a no-op on
+    // others; it is raised to the most any shape uses, since every context
+    // of the process shares the kernel's attribute
+    (void)hipFuncSetAttribute((const void *)k_tail<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)std::min((size_t)maxb, MG_TAIL_LDS_MAX));
     (void)hipGetLastError();
     return 1;
 }
@@ -1485,10 +1488,227 @@ void launch_mg_fdm_pass(hipStream_t s, const MGLevel<T> &L, int a, int pb, const
     fdm_pass<T>(s, L, a, pb, W, dinv, in, out);
 }
 
+// ---- FP32 (the mixed path's preconditioner): three launches on MFMA ----
+// k_fdm_plane_mx: one workgroup per z-plane, forward: the x then the y
+// contraction, inverse: y then x.  k_fdm_pencils_mx: 64 consecutive
+// z-pencils per workgroup: the forward z contraction, the D^+ scaling and
+// the inverse z contraction, written back in place.  Each contraction is a
+// small GEMM on v_mfma_f32_16x16x4f32 with every extent padded to a multiple
+// of 16 with zeros in LDS.  The sums run in the matrix cores' order, not
+// the six-launch form's: the preconditioner's values differ in the last
+// FP32 bits, and the FP64 defect correction around it converges to the
+// same tolerance.
+typedef float fdm_f4 __attribute__((ext_vector_type(4)));
+constexpr int FDM_MX_BLOCK = 1024;
+
+__host__ __device__ constexpr int fdm_x16(int d) { return (d + 15) & ~15; }
+__host__ __device__ constexpr int fdm_xp(int d) { return fdm_x16(d) + 4; }   // LDS row pitch (floats)
+
+// C[m][n] = sum_k A[m][k] B[k][n] for m < M, n < N, k < K (multiples of 16):
+// A row-major (pitch pa), B given as its transpose BT[n][k] (pitch pb), both
+// in LDS.  A wavefront takes 16 x 16 tiles.  In an MFMA step lane l gives
+// the A entry of row l % 16 and the B entry of column l % 16 at k-slot
+// l / 16; here slot t of the four steps of a 16-wide k chunk is k = kb + 4 t
+// + s (s = step), so each lane reads its four A and four B entries with one
+// 16-byte load each.  The result D[m][n] of a tile lands in lane
+// (n % 16) + 16 (m % 16 / 4), element m % 4.
+template <typename F>
+__device__ __forceinline__ void fdm_mx_gemm(const float *A, int pa, const float *BT, int pb, int M, int N, int K,
+                                            F store) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int r = lane & 15, slot = lane >> 4;
+    const int tn = N >> 4, nt = (M >> 4) * tn;
+    for (int t = wave; t < nt; t += nw) {
+        const int m0 = (t / tn) << 4, n0 = (t - (t / tn) * tn) << 4;
+        const float *ap = A + (m0 + r) * pa + 4 * slot;
+        const float *bp = BT + (n0 + r) * pb + 4 * slot;
+        fdm_f4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < K; kb += 16) {
+            const fdm_f4 av = *reinterpret_cast<const fdm_f4 *>(ap + kb);
+            const fdm_f4 bv = *reinterpret_cast<const fdm_f4 *>(bp + kb);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) store(m0 + 4 * slot + v, n0 + r, acc[v]);
+    }
+}
+
+// Staging of a kernel's LDS operands: element e < total is load(e, o),
+// stored at lds[o]; each work-item issues FDM_MX_U loads before its first
+// store, so the loads of all the operands are in flight together.
+constexpr int FDM_MX_U = 16;
+template <typename L>
+__device__ __forceinline__ void fdm_mx_stage(float *lds, int total, L load) {
+    for (int e0 = threadIdx.x; e0 < total; e0 += FDM_MX_U * blockDim.x) {
+        float v[FDM_MX_U];
+        int o[FDM_MX_U];
+#pragma unroll
+        for (int u = 0; u < FDM_MX_U; u++) {
+            int e = e0 + u * blockDim.x;
+            o[u] = -1;
+            v[u] = e < total ? load(e, o[u]) : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < FDM_MX_U; u++)
+            if (o[u] >= 0) lds[o[u]] = v[u];
+    }
+}
+
+// element e of the rows x cols matrix src (row-major, pitch sp) padded to a
+// rows16 x cols16 block at LDS offset off with pitch dp (zero outside), or
+// transposed (stored at [j][i]); cols16 = C, rC = 1 / C
+struct FdmMxPad {
+    const float *src;
+    int off, sp, dp, rows, cols, C, tr;
+    float rC;
+    __device__ __forceinline__ float load(int e, int &o) const {
+        int i = qdiv(e, C, rC), j = e - i * C;
+        o = off + (tr ? j * dp + i : i * dp + j);
+        return (i < rows && j < cols) ? src[i * sp + j] : 0.f;
+    }
+    __device__ __forceinline__ int count() const { return fdm_x16(rows) * C; }
+};
+__device__ __forceinline__ FdmMxPad fdm_mx_pad(int off, int dp, const float *src, int sp, int rows, int cols,
+                                               int tr = 0) {
+    const int C = fdm_x16(cols);
+    return FdmMxPad{src, off, sp, dp, rows, cols, C, tr, 1.0f / C};
+}
+
+// three padded copies staged together
+__device__ __forceinline__ void fdm_mx_stage3(float *lds, const FdmMxPad &a, const FdmMxPad &b, const FdmMxPad &c) {
+    const int na = a.count(), nb = b.count(), n = na + nb + c.count();
+    fdm_mx_stage(lds, n, [&](int e, int &o) {
+        return e < na ? a.load(e, o) : e < na + nb ? b.load(e - na, o) : c.load(e - na - nb, o);
+    });
+}
+
+// forward: out = (S1^T (S0^T planes)) ; inverse: out = S0 (S1 planes) per
+// plane, with S = the bases (S[q][p] = basis q of mode p) and ST their
+// transposes, all d x d row-major
+template <bool INV>
+__global__ void __launch_bounds__(FDM_MX_BLOCK) k_fdm_plane_mx(const float *__restrict__ in, float *__restrict__ out,
+                                                               const float *__restrict__ S0,
+                                                               const float *__restrict__ ST0,
+                                                               const float *__restrict__ S1,
+                                                               const float *__restrict__ ST1, int d0, int d1) {
+    extern __shared__ unsigned char fdm_lds[];
+    const int X0 = fdm_x16(d0), X1 = fdm_x16(d1), p0 = fdm_xp(d0), p1 = fdm_xp(d1), np = d0 * d1;
+    float *buf = reinterpret_cast<float *>(fdm_lds);
+    const float *src = in + (size_t)blockIdx.x * np;
+    float *dst = out + (size_t)blockIdx.x * np;
+    if (!INV) {
+        float *P = buf;                 // [X1][p0]  plane[j][q]
+        float *B1 = P + X1 * p0;        // [X0][p0]  S0^T[p][q]
+        float *A2 = B1 + X0 * p0;       // [X1][p1]  S1^T[p1][q]
+        float *TT = A2 + X1 * p1;       // [X0][p1]  T[j][p] stored at [p][j]
+        fdm_mx_stage3(buf, fdm_mx_pad(0, p0, src, d0, d1, d0), fdm_mx_pad(B1 - buf, p0, ST0, d0, d0, d0),
+                      fdm_mx_pad(A2 - buf, p1, ST1, d1, d1, d1));
+        __syncthreads();
+        // T[j][p] = sum_q plane[j][q] S0[q][p]
+        fdm_mx_gemm(P, p0, B1, p0, X1, X0, X0, [&](int m, int n, float v) { TT[n * p1 + m] = v; });
+        __syncthreads();
+        // out[p1][i] = sum_q S1[q][p1] T[q][i]
+        fdm_mx_gemm(A2, p1, TT, p1, X1, X0, X1, [&](int m, int n, float v) {
+            if (m < d1 && n < d0) dst[m * d0 + n] = v;
+        });
+    } else {
+        float *XT = buf;                // [X0][p1]  plane[q][i] stored at [i][q]
+        float *A1 = XT + X0 * p1;       // [X1][p1]  S1[p1][q]
+        float *O1 = A1 + X1 * p1;       // [X1][p0]
+        float *B2 = O1 + X1 * p0;       // [X0][p0]  S0[p][q]
+        // the plane is read along i (coalesced) and stored transposed
+        fdm_mx_stage3(buf, fdm_mx_pad(0, p1, src, d0, d1, d0, 1), fdm_mx_pad(A1 - buf, p1, S1, d1, d1, d1),
+                      fdm_mx_pad(B2 - buf, p0, S0, d0, d0, d0));
+        __syncthreads();
+        // O1[p1][i] = sum_q ST1[q][p1] plane[q][i]
+        fdm_mx_gemm(A1, p1, XT, p1, X1, X0, X1, [&](int m, int n, float v) { O1[m * p0 + n] = v; });
+        __syncthreads();
+        // out[j][p] = sum_q O1[j][q] ST0[q][p]
+        fdm_mx_gemm(O1, p0, B2, p0, X1, X0, X0, [&](int m, int n, float v) {
+            if (m < d1 && n < d0) dst[m * d0 + n] = v;
+        });
+    }
+}
+
+__global__ void __launch_bounds__(FDM_MX_BLOCK) k_fdm_pencils_mx(float *__restrict__ io, const float *__restrict__ S2,
+                                                                 const float *__restrict__ ST2,
+                                                                 const float *__restrict__ dinv, int np, int d2) {
+    extern __shared__ unsigned char fdm_lds[];
+    const int X2 = fdm_x16(d2), pz = fdm_xp(d2);
+    float *VT = reinterpret_cast<float *>(fdm_lds);   // [64][pz]  values[q][c] at [c][q]
+    float *UT = VT + 64 * pz;                           // [64][pz]
+    float *Af = UT + 64 * pz;                           // [X2][pz]  S2^T[p][q]
+    float *Ai = Af + X2 * pz;                           // [X2][pz]  S2[p][q]
+    const int c0 = blockIdx.x * 64, nc = min(64, np - c0);
+    // the pencils are read along c (coalesced) and stored transposed
+    FdmMxPad v = fdm_mx_pad(0, pz, io + c0, np, d2, nc, 1);
+    v.C = 64;
+    v.rC = 1.0f / 64;
+    fdm_mx_stage3(VT, v, fdm_mx_pad(Af - VT, pz, ST2, d2, d2, d2), fdm_mx_pad(Ai - VT, pz, S2, d2, d2, d2));
+    __syncthreads();
+    // U[p][c] = D^+ sum_q S2[q][p] values[q][c]
+    fdm_mx_gemm(Af, pz, VT, pz, X2, 64, X2, [&](int m, int n, float v) {
+        if (m < d2 && n < nc) v *= dinv[(size_t)m * np + c0 + n];
+        UT[n * pz + m] = v;
+    });
+    __syncthreads();
+    // values[p][c] = sum_q ST2[q][p] U[q][c]
+    fdm_mx_gemm(Ai, pz, UT, pz, X2, 64, X2, [&](int m, int n, float v) {
+        if (m < d2 && n < nc) io[(size_t)m * np + c0 + n] = v;
+    });
+}
+
+static size_t fdm_mx_plane_bytes(const int d[3]) {
+    size_t X0 = fdm_x16(d[0]), X1 = fdm_x16(d[1]), p0 = fdm_xp(d[0]), p1 = fdm_xp(d[1]);
+    return sizeof(float) * (X1 * p0 + X0 * p0 + X1 * p1 + X0 * p1);
+}
+static size_t fdm_mx_pencil_bytes(const int d[3]) {
+    size_t X2 = fdm_x16(d[2]), pz = fdm_xp(d[2]);
+    return sizeof(float) * (2 * 64 * pz + 2 * X2 * pz);
+}
+
+size_t mg_fdm_fused_bytes(const int d[3]) { return std::max(fdm_mx_plane_bytes(d), fdm_mx_pencil_bytes(d)); }
+
+int mg_fdm_fused_enable(const int d[3]) {
+    int dev = 0, maxb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    size_t bytes = mg_fdm_fused_bytes(d);
+    if (bytes > (size_t)maxb || bytes > MG_TAIL_LDS_MAX) return 0;
+    // every context of the process shares the kernels' attribute: raise it
+    // to the most any shape may use
+    const int lim = (int)std::min((size_t)maxb, MG_TAIL_LDS_MAX);
+    (void)hipFuncSetAttribute((const void *)k_fdm_plane_mx<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_fdm_plane_mx<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_fdm_pencils_mx, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipGetLastError();
+    return 1;
+}
+
+static void fdm_fused(hipStream_t s, const MGLevel<float> &L, const FdmDev<float> &F) {
+    const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2], np = d0 * d1;
+    const size_t bp = fdm_mx_plane_bytes(L.d), bz = fdm_mx_pencil_bytes(L.d);
+    const dim3 blk(FDM_MX_BLOCK);
+    hipLaunchKernelGGL(k_fdm_plane_mx<false>, dim3(d2), blk, bp, s, (const float *)L.b, L.r, F.S[0], F.ST[0], F.S[1],
+                       F.ST[1], d0, d1);
+    hipLaunchKernelGGL(k_fdm_pencils_mx, dim3((np + 63) / 64), blk, bz, s, L.r, F.S[2], F.ST[2], F.dinv, np, d2);
+    hipLaunchKernelGGL(k_fdm_plane_mx<true>, dim3(d2), blk, bp, s, (const float *)L.r, L.x, F.S[0], F.ST[0], F.S[1],
+                       F.ST[1], d0, d1);
+}
+static void fdm_fused(hipStream_t, const MGLevel<double> &, const FdmDev<double> &) {}
+
 // forward along x, y, z (the last scaled by D^+), back along z, y, x,
-// through the level's r and x arrays (b is left as it is)
+// through the level's r and x arrays (b is left as it is): one launch per
+// contraction, or (FP32 with F.fused) the three MFMA launches
 template <typename T>
 void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
+    if (F.fused) {
+        fdm_fused(s, L, F);
+        return;
+    }
     fdm_pass<T>(s, L, 0, F.pb[0], F.S[0], nullptr, L.b, L.r);
     fdm_pass<T>(s, L, 1, F.pb[1], F.S[1], nullptr, L.r, L.x);
     fdm_pass<T>(s, L, 2, F.pb[2], F.S[2], F.dinv, L.x, L.r);

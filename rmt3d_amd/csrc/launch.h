@@ -77,11 +77,17 @@ template <typename T>
 struct FdmDev {
     const T *S[3], *ST[3], *dinv;
     int pb[3] = {8, 8, 8};   // outputs per work-item of each dimension's passes (2, 4 or 8)
+    int fused = 0;           // FP32 only, 1: the three MFMA launches (planes, pencils, planes)
 };
+// LDS bytes of the FP32 three-launch form for a level of dimensions d, and
+// enabling it (1 when that fits a workgroup)
+size_t mg_fdm_fused_bytes(const int d[3]);
+int mg_fdm_fused_enable(const int d[3]);
 // host: the 1D basis of one dimension (fem 1: Q1 nodes, 0: MAC cells) and
 // its eigenvalues
 void mg_fdm_basis(int fem, int d, int prd, double *S, double *lam);
-// x = S D^+ S^T b on level L (six contractions through L.r and L.x)
+// x = S D^+ S^T b on level L (six contractions through L.r and L.x, in
+// six launches or, FP32 with F.fused, three on MFMA)
 template <typename T>
 void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F);
 // one of those contractions: along dimension a, pb outputs per work-item

@@ -67,15 +67,65 @@ def _mg_pair_solve(which, dims, prd, prec, seed):
 @pytest.mark.parametrize("which", [0, 1])
 @pytest.mark.parametrize("dims,prd", [((64, 64, 64), (0, 0, 0)), ((64, 64, 64), (1, 1, 1)),
                                       ((64, 48, 80), (1, 0, 1)), ((96, 64, 56), (0, 1, 0)),
-                                      ((192, 96, 4), (0, 0, 0)), ((4, 96, 192), (0, 0, 0))])
+                                      ((192, 96, 4), (0, 0, 0)), ((4, 96, 192), (0, 0, 0)),
+                                      ((128, 128, 128), (0, 0, 0))])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_mg_solve_fdm_level_matches_oracle(which, dims, prd, prec):
     # the first coarse level ends the hierarchy and is solved by fast
     # diagonalisation (walls, periodic, mixed and anisotropic extents); FP64
     # must match the oracle bit for bit, the mixed path converges to the same
-    # FP64 tolerance
+    # FP64 tolerance (the autotuner picks the solve's launch form)
     g = _mg_pair_solve(which, dims, prd, prec, 3)
     assert g.diag(18 + which) == 1.0 and g.diag(10 + which) == 2.0
+
+
+@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("dims,prd,fits", [((128, 128, 128), (0, 0, 0), (1, 1)), ((64, 64, 64), (1, 1, 1), (1, 1)),
+                                           ((64, 48, 80), (1, 0, 1), (1, 1)), ((192, 96, 4), (0, 0, 0), (1, 1)),
+                                           ((4, 96, 192), (0, 0, 0), (1, 0))])
+def test_mg_solve_fdm_mfma_form_matches_oracle(which, dims, prd, fits, monkeypatch):
+    # with the built-in defaults the mixed path's FP32 fast-diagonalisation
+    # solve runs as three MFMA launches wherever its planes, pencil blocks
+    # and bases fit a workgroup's LDS (fits: MAC, FEM level), and as six
+    # single-contraction launches where they do not (the 3 x 49 x 97 FEM
+    # level: padded 112 x 112 z-bases beside 64 pencils); the FP64 defect
+    # correction around it converges to the oracle's solution
+    monkeypatch.setenv("RMT3D_TUNE", "0")
+    g = _mg_pair_solve(which, dims, prd, 1, 7)
+    assert g.diag(18 + which) == 1.0 and g.diag(31 + which) == float(fits[which])
+
+
+def test_fdm_mfma_form_gives_the_same_states(monkeypatch):
+    """The mixed path's FP32 fast-diagonalisation solve on MFMA (three
+    launches) and as six single-contraction launches (RMT3D_FDM_FUSED=0)
+    precondition the same FP64 defect correction: a 64^3 cavity with a
+    sphere steps to velocities and pressures 1e-10 apart (relative L2; the
+    velocity as one field, since v alone is nearly zero in the lid-driven
+    box) either way."""
+    from rmt3d_amd import _abi
+    from rmt3d_amd.lib import Simulation
+    cfg, objs = configs.cavity_sphere(64, steps=10)
+    monkeypatch.setenv("RMT3D_TUNE", "0")
+    monkeypatch.setenv("RMT3D_FDM_FUSED", "0")
+    a = Simulation(cfg, objs, _abi.PREC_MIXED)
+    monkeypatch.delenv("RMT3D_FDM_FUSED")
+    b = Simulation(cfg, objs, _abi.PREC_MIXED)
+    assert a.diag(18) == 1.0 and a.diag(19) == 1.0
+    assert a.diag(31) == 0.0 and a.diag(32) == 0.0
+    assert b.diag(31) == 1.0 and b.diag(32) == 1.0
+    for s in (a, b):
+        e = s.initialize()
+        s.prepare_run(e)
+        for _ in range(3):
+            assert s.step() == 0
+    fa, fb = a.download(), b.download()
+    va = np.concatenate([fa[k] for k in ("u", "v", "w")])
+    vb = np.concatenate([fb[k] for k in ("u", "v", "w")])
+    for k, d in (("vel", rel_l2(va, vb)), ("p", rel_l2(fa["p"], fb["p"]))):
+        print(k, d, flush=True)
+        assert d < TOL, (k, d)
+    a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("which", [0, 1])
