@@ -91,12 +91,16 @@ def sweep_words(np_):
     return 1.0 + np_ / 4.0 + np_ / 4.0
 
 
-# One V-cycle, per finest unknown, at 2 pre + 2 post sweeps of the finest
-# level (red-black: 4 sweeps of 3 words, residual 3, restriction 1.25,
-# prolongation 2.125 = 18.4 words) and the fast-diagonalisation solve of the
-# first coarse level, which ends the hierarchy of the 128^3 workload (six
-# passes reading and writing 1/8 as many unknowns: 1.5 words).
-VCYCLE_WORDS = 19.9
+# One V-cycle, per finest unknown: the finest level's smoothing sweeps
+# (red-black MAC: 2 colour passes of 1.5 words; FEM: 4 colour passes of
+# FEM_COLOUR_WORDS), residual 3, restriction 1.25, prolongation 2.125, and
+# the fast-diagonalisation solve of the first coarse level, which ends the
+# hierarchy of the 128^3 workload (six contractions reading and writing 1/8
+# as many unknowns: 1.5 words).  At the reference's 2 + 2 sweeps: MAC 19.9,
+# FEM 27.9 words.
+def vcycle_words(fem, sweeps):
+    per_sweep = 4 * FEM_COLOUR_WORDS if fem else 2 * 1.5
+    return sweeps * per_sweep + 3.0 + 1.25 + 2.125 + 1.5
 # One FP64 defect-correction step of the mixed path adds the FP64 residual
 # (read x, b, write r) per finest unknown.
 DEFECT_WORDS64 = 3.0
@@ -180,7 +184,9 @@ def run_gpu(args, precision):
     for key, N in (("fem", nodes), ("mac", cells)):
         ms_v, nv = prof[key + "_vcycle"]
         t_v = ms_v / max(nv, 1) * 1e-3
-        b = VCYCLE_WORDS * N * word + (DEFECT_WORDS64 * N * 8 if precision == "mixed" else 0)
+        sched = int(sim.diag(34 if key == "fem" else 33))
+        words = vcycle_words(key == "fem", sched // 10 + sched % 10)
+        b = words * N * word + (DEFECT_WORDS64 * N * 8 if precision == "mixed" else 0)
         vc[key] = {"us": round(t_v * 1e6, 2), "gbs": round(b / t_v / 1e9, 1) if t_v > 0 else 0.0,
                    "frac": round(b / t_v / 1e9 / HBM_PEAK_GBS, 4) if t_v > 0 else 0.0, "cycles": nv}
     fields = sim.download(band=False)
@@ -190,7 +196,9 @@ def run_gpu(args, precision):
                          "mac_colour_pass_workgroup": sim.diag(24), "fem_colour_pass_workgroup": sim.diag(25),
                          "mac_fdm_outputs_per_item_xyz": int(sim.diag(26)),
                          "fem_fdm_outputs_per_item_xyz": int(sim.diag(27)),
-                         "mac_fdm_three_launch": int(sim.diag(31)), "fem_fdm_three_launch": int(sim.diag(32))},
+                         "mac_fdm_three_launch": int(sim.diag(31)), "fem_fdm_three_launch": int(sim.diag(32)),
+                         "mac_vcycle_sweeps_down_up": int(sim.diag(33)),
+                         "fem_vcycle_sweeps_down_up": int(sim.diag(34))},
            "stress_ms_per_step": prof["stress"][0] / max(args.kernel_steps, 1), "stage_ms_per_step": stages,
            "fields": {k: fields[k] for k in ("u", "v", "w", "p")}}
     sim.close()

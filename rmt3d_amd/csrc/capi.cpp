@@ -183,6 +183,8 @@ int rmt3d_diag(rmt3d_ctx *ctx, int which, double *out) {
         case 27: *out = S.fem.fdm64.pb[0] + 10 * S.fem.fdm64.pb[1] + 100 * S.fem.fdm64.pb[2]; return 0;
         case 31: *out = S.mac.mixed ? S.mac.fdm32.fused : S.mac.fdm64.fused; return 0;
         case 32: *out = S.fem.mixed ? S.fem.fdm32.fused : S.fem.fdm64.fused; return 0;
+        case 33: *out = 10 * S.mac.v_down + S.mac.v_up; return 0;
+        case 34: *out = 10 * S.fem.v_down + S.fem.v_up; return 0;
         case 30: *out = S.last_batches; return 0;
         case 28: *out = S.mac.tuned; return 0;
         case 29: *out = S.fem.tuned; return 0;

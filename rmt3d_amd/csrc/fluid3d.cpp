@@ -444,8 +444,10 @@ struct TuneChoice {
     int bs[16];
     int pb[3];
     int fused;
+    int vd, vu;
 };
 std::mutex g_tune_mu;
+std::mutex g_tune_run_mu;
 std::map<std::vector<int>, TuneChoice> g_tune_cache;
 
 // minimum over rounds of the time of `reps` runs of op(c), per candidate c
@@ -545,14 +547,33 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
     (void)LV;
 }
 
+// the schedule RMT3D_MIXED_VCYCLE fixes, as 10 down + up, or 0
+static int mixed_vcycle_env() {
+    const char *vc = getenv("RMT3D_MIXED_VCYCLE");
+    int d = 0, u = 0;
+    if (vc && sscanf(vc, "%d,%d", &d, &u) == 2 && d >= 1 && u >= 1 && d <= 4 && u <= 4) return 10 * d + u;
+    return 0;
+}
+
 void MGSolver::tune(hipStream_t s) {
     const char *tv = getenv("RMT3D_TUNE");
-    if (tv && tv[0] == '0') return;
+    if (tv && tv[0] == '0') {
+        if (mixed)
+            if (int f = mixed_vcycle_env()) {
+                v_down = f / 10;
+                v_up = f % 10;
+            }
+        return;
+    }
     bool sweep_fixed = getenv("RMT3D_SWEEP") != nullptr;
-    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, fdm_fused_ok, nlev, fdm ? 1 : 0,
+    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, fdm_fused_ok,
+                            mixed ? mixed_vcycle_env() : 0, nlev, fdm ? 1 : 0,
                             prd[0], prd[1], prd[2]};
     for (int l = 0; l < nlev; l++)
         for (int a = 0; a < 3; a++) key.push_back(L[l].d[a]);
+    // one tuning at a time in the process: contexts of one shape created
+    // together (a serving mix) wait for the first one's choice and share it
+    std::lock_guard<std::mutex> run_lk(g_tune_run_mu);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -561,6 +582,8 @@ void MGSolver::tune(hipStream_t s) {
             for (int a = 0; a < 3; a++) fdm64.pb[a] = fdm32.pb[a] = it->second.pb[a];
             fdm64.fused = (fdm_fused_ok & 1) ? it->second.fused : 0;
             fdm32.fused = (fdm_fused_ok & 2) ? it->second.fused : 0;
+            v_down = it->second.vd;
+            v_up = it->second.vu;
             tuned = 2;
             return;
         }
@@ -569,6 +592,7 @@ void MGSolver::tune(hipStream_t s) {
         tune_levels<float>(s, Lf, fdm32, sweep_fixed);
         for (int a = 0; a < 3; a++) fdm64.pb[a] = fdm32.pb[a];
         fdm64.fused = (fdm_fused_ok & 1) ? fdm32.fused : 0;
+        tune_schedule(s);
     } else {
         tune_levels<double>(s, L, fdm64, sweep_fixed);
         for (int a = 0; a < 3; a++) fdm32.pb[a] = fdm64.pb[a];
@@ -580,9 +604,85 @@ void MGSolver::tune(hipStream_t s) {
     for (int l = 0; l < 16; l++) { ch.npass[l] = npass[l]; ch.bs[l] = smooth_bs[l]; }
     for (int a = 0; a < 3; a++) ch.pb[a] = fdm64.pb[a];
     ch.fused = mixed ? fdm32.fused : fdm64.fused;
+    ch.vd = v_down;
+    ch.vu = v_up;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = ch;
     tuned = 1;
+}
+
+// Smoothing sweeps of the mixed path's FP32 V-cycle.  The reference picks
+// (v_up, v_down) by measurement when its mg_opt search is on (mg_probs.hh:
+// 80-208: digits of residual reduction per second over neighbouring
+// schedules).  The FP64 path keeps the reference's default 2 + 2, so that
+// its iterates are the oracle's.  The mixed path's FP32 V-cycle is a
+// preconditioner inside the FP64 defect correction, whose stopping rule and
+// converged solution do not depend on it, so there the schedule is picked
+// here: for 2 + 2, 2 + 1 and 1 + 2 sweeps, the iterations that take a fixed
+// zero-mean right-hand side (a 64-bit LCG) from zero below the solver's
+// FP64 tolerance, plus the two insurance iterations every solve runs after
+// that (mg_probs.cc:43-47), times the event-timed cost of one iteration;
+// the least product wins, 2 + 2 on ties.  RMT3D_MIXED_VCYCLE="down,up"
+// fixes it.
+void MGSolver::tune_schedule(hipStream_t s) {
+    if (int f = mixed_vcycle_env()) {
+        v_down = f / 10;
+        v_up = f % 10;
+        return;
+    }
+    static const int cand[3][2] = {{2, 2}, {2, 1}, {1, 2}};
+    const size_t n = L[0].nn;
+    std::vector<double> h(n);
+    unsigned long long st = 0x9E3779B97F4A7C15ull;
+    double mean = 0;
+    for (size_t i = 0; i < n; i++) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        h[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        mean += h[i];
+    }
+    mean /= (double)n;
+    for (size_t i = 0; i < n; i++) h[i] -= mean;
+    if (hipMemcpy(L[0].b, h.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    const double thr = max_norm_tol;
+    double best = 1e300;
+    int bd = v_down, bu = v_up;
+    for (const auto &c : cand) {
+        v_down = c[0];
+        v_up = c[1];
+        hipMemsetAsync(L[0].x, 0, sizeof(double) * n, s);
+        int it = 0;
+        double res = resid_max(s);
+        while (res > thr && it < 40) {
+            cycle_body(s);
+            it++;
+            res = resid_max(s);
+        }
+        if (!(res <= thr)) continue;
+        std::vector<double> t = time_candidates(s, 1, [&](int) { cycle_body(s); });
+        double cost = (it + 2) * t[0];
+        if (t[0] < 1e29 && cost < best * (1 - 1e-3)) {
+            best = cost;
+            bd = c[0];
+            bu = c[1];
+        }
+    }
+    v_down = bd;
+    v_up = bu;
+    for (int l = 0; l < nlev; l++) {
+        hipMemsetAsync(L[l].x, 0, sizeof(double) * L[l].nn, s);
+        hipMemsetAsync(L[l].b, 0, sizeof(double) * L[l].nn, s);
+        hipMemsetAsync(L[l].r, 0, sizeof(double) * L[l].nn, s);
+        hipMemsetAsync(Lf[l].x, 0, sizeof(float) * Lf[l].nn, s);
+        hipMemsetAsync(Lf[l].b, 0, sizeof(float) * Lf[l].nn, s);
+        hipMemsetAsync(Lf[l].r, 0, sizeof(float) * Lf[l].nn, s);
+    }
+    if (gexec) {
+        hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
 }
 
 void MGSolver::vcycle64(hipStream_t s, int lev) {

@@ -80,6 +80,7 @@ struct MGSolver {
     // picks launch shapes by timing the candidates on the levels (or from
     // the per-process cache of that level shape)
     void tune(hipStream_t s);
+    void tune_schedule(hipStream_t s);
     template <typename T>
     void tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool sweep_fixed);
     void release();

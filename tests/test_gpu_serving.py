@@ -25,8 +25,10 @@ def _alone(cfg, objs, steps, precision=_abi.PREC_FP64):
 @pytest.mark.parametrize("precision", [_abi.PREC_FP64, _abi.PREC_MIXED])
 def test_results_same_alone_and_in_the_mix(precision):
     """Bitwise on both paths: a request's stage order and arithmetic do not
-    depend on the other requests (the mixed path's launch shapes may differ
-    with the load during tuning, which changes no result)."""
+    depend on the other requests.  The autotuner's choices (launch shapes,
+    and on the mixed path the V-cycle schedule and fast-diagonalisation
+    form) are made once per grid shape in the process and shared, so a
+    request alone and in the mix runs the same schedule."""
     rng = np.random.default_rng(7)
     reqs = [serve.mixed_request(rng, steps=6, grids=(24, 32), max_spheres=4) for _ in range(5)]
     srv = serve.Server(16e9, max_active=3)
