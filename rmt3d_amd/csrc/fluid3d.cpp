@@ -1279,11 +1279,11 @@ void Sim::comm_rm(int flags) {
 
 // fluid_3d::set_bc_cc / set_bc_cf (fluid_3d.cc:145-165)
 void Sim::fill_bc_cc() {
-    for (auto &f : wall_faces) launch_bc_cc(stream, g, f, U);
+    launch_bc_cc_faces(stream, g, wall_faces.data(), (int)wall_faces.size(), U);
     comm_fluid(ORC_VEL);
 }
 void Sim::fill_bc_cf() {
-    for (auto &f : wall_faces) launch_bc_cf(stream, g, f, U);
+    launch_bc_cf_faces(stream, g, wall_faces.data(), (int)wall_faces.size(), U);
     comm_fluid(ORC_FVEL);
 }
 

@@ -651,13 +651,21 @@ __device__ __forceinline__ int face_node(const Geo &g, const FaceC &F, int a, in
     return g.G0 + F.u0_off + a * F.len4[0] + b * F.len4[1] + c * F.len4[2];
 }
 
-__global__ void k_bc_cc(Geo g, FaceC F, Fluid U) {
+// Several faces in one launch: work-items [start[q], start[q + 1]) serve
+// face q.  The faces of one launch write disjoint cells and none reads what
+// another writes (the caller groups them so), so the values are those of
+// one launch per face in order.
+constexpr int FACES_MAX = 6;
+struct FaceSet {
+    int nf;
+    int start[FACES_MAX + 1];
+    FaceC F[FACES_MAX];
+};
+
+__device__ __forceinline__ void bc_cc_at(const Geo &g, const FaceC &F, const Fluid &U, int t) {
     // normal index 1..2, tangential -2..len+1
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
     int ext[3];
     for (int d = 0; d < 3; d++) ext[d] = (d == F.n) ? 2 : F.len[d] + 4;
-    int total = ext[0] * ext[1] * ext[2];
-    if (t >= total) return;
     int in[3];
     in[0] = t % ext[0]; in[1] = (t / ext[0]) % ext[1]; in[2] = t / (ext[0] * ext[1]);
     for (int d = 0; d < 3; d++) in[d] += (d == F.n) ? 1 : -2;
@@ -674,12 +682,9 @@ __global__ void k_bc_cc(Geo g, FaceC F, Fluid U) {
     }
 }
 
-__global__ void k_bc_cf(Geo g, FaceC F, Fluid U) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bc_cf_at(const Geo &g, const FaceC &F, const Fluid &U, int t) {
     int ext[3];
     for (int d = 0; d < 3; d++) ext[d] = (d == F.n) ? 1 : F.len[d] + 4;
-    int total = ext[0] * ext[1] * ext[2];
-    if (t >= total) return;
     int in[3];
     in[0] = t % ext[0]; in[1] = (t / ext[0]) % ext[1]; in[2] = t / (ext[0] * ext[1]);
     for (int d = 0; d < 3; d++) in[d] += (d == F.n) ? 1 : -2;
@@ -695,6 +700,16 @@ __global__ void k_bc_cf(Geo g, FaceC F, Fluid U) {
         U.fvel[3 * F.gcf0 + c][y0] = fb;
         U.fvel[3 * F.gcf1 + c][u] = fb;
     }
+}
+
+template <bool CC>
+__global__ void k_bc_faces(Geo g, FaceSet S, Fluid U) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= S.start[S.nf]) return;
+    int q = 0;
+    while (q + 1 < S.nf && t >= S.start[q + 1]) q++;
+    if (CC) bc_cc_at(g, S.F[q], U, t - S.start[q]);
+    else bc_cf_at(g, S.F[q], U, t - S.start[q]);
 }
 
 // periodic source (mpi_table region rule, Utilities/comm_info.hh:91-142)
@@ -951,11 +966,14 @@ __global__ void k_pp_rhs(Geo g, Fluid U, GradPhi gp, Faces6 faces, int nwall, do
     r[a + g.fm * (b + g.fn * cz)] = acc;
 }
 
+// one work-item per node on a flat grid (rows of fm = m + 1 nodes in
+// 64-wide row launches left a nearly empty wavefront per row)
 __global__ void k_pp_extract(Geo g, Fluid U, const double *x, const double *mean) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    int b = blockIdx.y, c = blockIdx.z;
-    if (a >= g.fm) return;
-    double v = x[a + g.fm * (b + g.fn * c)];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.fm * g.fn * g.fo) return;
+    int row = t / g.fm, a = t - row * g.fm;
+    int c = row / g.fn, b = row - c * g.fn;
+    double v = x[t];
     double pm = *mean;
     int e = IX(g, a + 2, b + 2, c + 2);
     U.p[e] += (v - pm);
@@ -1177,17 +1195,45 @@ static FaceC make_face(const Geo &g, const FaceSpec &f) {
     F.h = hs[F.n];
     return F;
 }
-void launch_bc_cc(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U) {
-    FaceC F = make_face(g, f);
+static int face_items(const FaceC &F, int depth) {
     int tot = 1;
-    for (int d = 0; d < 3; d++) tot *= (d == F.n) ? 2 : F.len[d] + 4;
-    hipLaunchKernelGGL(k_bc_cc, dim3((tot + 255) / 256), dim3(256), 0, s, g, F, U);
+    for (int d = 0; d < 3; d++) tot *= (d == F.n) ? depth : F.len[d] + 4;
+    return tot;
 }
-void launch_bc_cf(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U) {
-    FaceC F = make_face(g, f);
-    int tot = 1;
-    for (int d = 0; d < 3; d++) tot *= (d == F.n) ? 1 : F.len[d] + 4;
-    hipLaunchKernelGGL(k_bc_cf, dim3((tot + 255) / 256), dim3(256), 0, s, g, F, U);
+static void launch_faces(hipStream_t s, const Geo &g, const FaceSpec *f, int nf, const Fluid &U, bool cc) {
+    if (nf <= 0) return;
+    FaceSet S{};
+    S.nf = nf;
+    S.start[0] = 0;
+    for (int q = 0; q < nf; q++) {
+        S.F[q] = make_face(g, f[q]);
+        S.start[q + 1] = S.start[q] + face_items(S.F[q], cc ? 2 : 1);
+    }
+    int tot = S.start[nf];
+    if (cc) hipLaunchKernelGGL(k_bc_faces<true>, dim3((tot + 255) / 256), dim3(256), 0, s, g, S, U);
+    else hipLaunchKernelGGL(k_bc_faces<false>, dim3((tot + 255) / 256), dim3(256), 0, s, g, S, U);
+}
+// set_ghost_cc over the wall faces in order.  A face's mirror cells include
+// the ghost rows of the other dimensions, which later faces write, so faces
+// of different dimensions keep their order (one launch per dimension); the
+// two faces of one dimension write disjoint ghost layers and read cells the
+// other does not write (the extent is at least 4), so they share a launch.
+void launch_bc_cc_faces(hipStream_t s, const Geo &g, const FaceSpec *f, int nf, const Fluid &U) {
+    const int dims[3] = {g.m, g.n, g.o};
+    for (int q = 0; q < nf;) {
+        int r = q + 1;
+        const int n = f[q].f / 2;
+        if (dims[n] >= 4)
+            while (r < nf && f[r].f / 2 == n) r++;
+        launch_faces(s, g, f + q, r - q, U, true);
+        q = r;
+    }
+}
+// set_ghost_cf over the wall faces: each face writes its own two face-state
+// components (its own and its neighbour's) in disjoint cells and reads only
+// cell-centred velocities, so all faces share one launch
+void launch_bc_cf_faces(hipStream_t s, const Geo &g, const FaceSpec *f, int nf, const Fluid &U) {
+    for (int q = 0; q < nf; q += FACES_MAX) launch_faces(s, g, f + q, std::min(FACES_MAX, nf - q), U, false);
 }
 static int slab_total(const Geo &g) { return 2 * 2 * g.n4 * g.o4 + 2 * g.m4 * 2 * g.o4 + 2 * g.m4 * g.n4 * 2; }
 void launch_periodic(hipStream_t s, const Geo &g, double *const *d, int nd, uint32_t *const *u, int nu, int corner) {
@@ -1239,7 +1285,8 @@ void launch_pp_rhs(hipStream_t s, const Geo &g, const Fluid &U, const double gra
 }
 size_t face_struct_size() { return sizeof(FaceC) * 6; }
 void launch_pp_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x, const double *mean) {
-    hipLaunchKernelGGL(k_pp_extract, rows(g.fm, g.fn, g.fo, 64), dim3(64), 0, s, g, U, x, mean);
+    int n = g.fm * g.fn * g.fo;
+    hipLaunchKernelGGL(k_pp_extract, dim3((n + 255) / 256), dim3(256), 0, s, g, U, x, mean);
 }
 void launch_sub_pgrad(hipStream_t s, const Geo &g, const Fluid &U, double dt) {
     hipLaunchKernelGGL(k_sub_pgrad, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, U, dt);

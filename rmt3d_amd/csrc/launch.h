@@ -29,8 +29,9 @@ void launch_stress(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, co
 void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                    const int *list, const int *count, int *status);
 void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status);
-void launch_bc_cc(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U);
-void launch_bc_cf(hipStream_t s, const Geo &g, const FaceSpec &f, const Fluid &U);
+// the wall faces' ghost fills, in order (set_ghost_cc / set_ghost_cf)
+void launch_bc_cc_faces(hipStream_t s, const Geo &g, const FaceSpec *f, int nf, const Fluid &U);
+void launch_bc_cf_faces(hipStream_t s, const Geo &g, const FaceSpec *f, int nf, const Fluid &U);
 void launch_periodic(hipStream_t s, const Geo &g, double *const *d, int nd, uint32_t *const *u, int nu, int corner);
 void launch_periodic_band_fx(hipStream_t s, const Geo &g, const Band &B);
 void launch_mac_rhs(hipStream_t s, const Geo &g, const Fluid &U, double *b, double fac);
