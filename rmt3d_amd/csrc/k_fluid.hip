@@ -31,22 +31,64 @@ struct Nb8 { int r1p, r1p1, r1m, r1m1, r2p, r2p1, r2m, r2m1; };
 
 // The face direction F is a template parameter throughout the stress path:
 // every small array is then indexed by constants and stays in registers.
+// get_pos for N cells at once: the same result as N calls, with the
+// loads of each stage (primary ids, then band ranges) issued together so
+// that the lookups overlap instead of running one dependent chain after
+// another
+template <int N>
+__device__ __forceinline__ void get_pos_n(const Geo &g, const Maps &M, const int (&ind)[N], const bool (&inb)[N],
+                                         unsigned oid, int (&out)[N]) {
+    unsigned c[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        bool in = inb[k] && ind[k] >= 0 && ind[k] < M.Ng;
+        out[k] = in ? -2 : -1;
+        c[k] = in ? M.r.c[ind[k]] : 0u;
+    }
+    int s[N], e[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (out[k] == -2 && c[k] == oid) out[k] = ind[k];
+        s[k] = e[k] = 0;
+        if (out[k] == -2) {
+            s[k] = M.b.start[ind[k]];
+            e[k] = s[k] + M.b.cnt[ind[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (out[k] != -2) continue;
+        int r = -1;
+        for (int a = s[k]; a < e[k]; a++)
+            if ((M.b.c[a] & OMASK) == oid) { r = M.Ng + a; break; }
+        out[k] = r;
+    }
+}
+
 template <int F>
 __device__ __forceinline__ bool nbr_set(const Geo &g, const Maps &M, int ind, unsigned oid, const int lp[3], Nb8 &n) {
     constexpr int ad1 = (F + 1) % 3, ad2 = (F + 2) % 3;
     const int str[3] = {1, g.m4, g.mn4};
+    const int lim[3] = {g.m4 - 1, g.n4 - 1, g.o4 - 1};
     int strd = str[F], as1 = str[ad1], as2 = str[ad2];
-    int q[3];
-#define POS(da, va, db, vb) q[0] = lp[0]; q[1] = lp[1]; q[2] = lp[2]; q[da] += va; if (db >= 0) q[db] += vb;
-    POS(ad1, 1, -1, 0);  n.r1p = get_pos(g, M, ind + as1, oid, q[0], q[1], q[2]);
-    POS(ad1, 1, F, -1);  n.r1p1 = get_pos(g, M, ind + as1 - strd, oid, q[0], q[1], q[2]);
-    POS(ad1, -1, -1, 0); n.r1m = get_pos(g, M, ind - as1, oid, q[0], q[1], q[2]);
-    POS(ad1, -1, F, -1); n.r1m1 = get_pos(g, M, ind - as1 - strd, oid, q[0], q[1], q[2]);
-    POS(ad2, 1, -1, 0);  n.r2p = get_pos(g, M, ind + as2, oid, q[0], q[1], q[2]);
-    POS(ad2, 1, F, -1);  n.r2p1 = get_pos(g, M, ind + as2 - strd, oid, q[0], q[1], q[2]);
-    POS(ad2, -1, -1, 0); n.r2m = get_pos(g, M, ind - as2, oid, q[0], q[1], q[2]);
-    POS(ad2, -1, F, -1); n.r2m1 = get_pos(g, M, ind - as2 - strd, oid, q[0], q[1], q[2]);
-#undef POS
+    // the eight cells: +-1 along ad1 and ad2, each also one back along F
+    const int da[8] = {ad1, ad1, ad1, ad1, ad2, ad2, ad2, ad2};
+    const int va[8] = {1, 1, -1, -1, 1, 1, -1, -1};
+    const int back[8] = {0, 1, 0, 1, 0, 1, 0, 1};
+    int cell[8];
+    bool inb[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int q[3] = {lp[0], lp[1], lp[2]};
+        q[da[k]] += va[k];
+        q[F] -= back[k];
+        inb[k] = q[0] >= 0 && q[0] <= lim[0] && q[1] >= 0 && q[1] <= lim[1] && q[2] >= 0 && q[2] <= lim[2];
+        cell[k] = ind + va[k] * (da[k] == ad1 ? as1 : as2) - back[k] * strd;
+    }
+    int r[8];
+    get_pos_n<8>(g, M, cell, inb, oid, r);
+    n.r1p = r[0]; n.r1p1 = r[1]; n.r1m = r[2]; n.r1m1 = r[3];
+    n.r2p = r[4]; n.r2p1 = r[5]; n.r2m = r[6]; n.r2m1 = r[7];
     return n.r1p >= 0 && n.r1p1 >= 0 && n.r1m >= 0 && n.r1m1 >= 0 && n.r2p >= 0 && n.r2p1 >= 0 && n.r2m >= 0 &&
            n.r2m1 >= 0;
 }
