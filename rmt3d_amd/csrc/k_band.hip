@@ -299,8 +299,10 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
         }
         unsigned long long hit = __ballot(inb);
         unsigned oid = hit ? (unsigned)B->oid[__ffsll((long long)hit) - 1] : 0u;
-        double gp[3], gnorm;
-        prev_normal(g, R, T, ind, oid, gp, gnorm);
+        // the previous normal only weights the first two layers
+        // (fluid_3d_ex.cc:196: grad_phi_norm != 0 && n_layers <= 2)
+        double gp[3] = {0, 0, 0}, gnorm = 0;
+        if (A.L <= 2) prev_normal(g, R, T, ind, oid, gp, gnorm);
         bool cosw = (gnorm != 0. && A.L <= 2);
         bool ok = false;
         double davg[3], ravg[3], coeff[9];
@@ -313,48 +315,85 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
             // pass 1: weighted sums (extrap_helper.cc:49-62)
             double acc = 0;
             int npt = 0;
-            for (int k0 = 0, ch = 0; k0 < nc; k0 += 64, ch++) {
-                int k = k0 + lane;
-                double w = -1;
-                int p = 0, q = 0, r = 0;
-                if (k < nc) {
-                    p = k % D - sr; q = (k / D) % D - sr; r = k / (D * D) - sr;
-                    w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
-                }
-                bool v = w >= 0;
-                if (v) {
-                    int nb = ind + p + g.m4 * q + g.mn4 * r;
-                    double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
-                    double dp[3] = {g.lx[ii + p], g.ly[jj + q], g.lz[kk + r]};
-                    term[lane][0] = rv[0] * w;
-                    term[lane][1] = rv[1] * w;
-                    term[lane][2] = rv[2] * w;
-                    term[lane][3] = dp[0] * w;
-                    term[lane][4] = dp[1] * w;
-                    term[lane][5] = dp[2] * w;
-                    term[lane][6] = w;
-                    if (keep) {
-                        // ch is 0 or 1 here; selects keep the arrays in registers
-                        for (int a = 0; a < 3; a++) {
-                            krv[0][a] = ch ? krv[0][a] : rv[a];
-                            kdp[0][a] = ch ? kdp[0][a] : dp[a];
-                            krv[1][a] = ch ? rv[a] : krv[1][a];
-                            kdp[1][a] = ch ? dp[a] : kdp[1][a];
-                        }
-                        kw[0] = ch ? kw[0] : w;
-                        kw[1] = ch ? w : kw[1];
+            if (keep) {
+                // both chunks' weights and data first, so that their
+                // gathers are in flight together, then the two summations
+#pragma unroll
+                for (int ch = 0; ch < 2; ch++) {
+                    int k = ch * 64 + lane;
+                    double w = -1;
+                    int p = 0, q = 0, r = 0;
+                    if (k < nc) {
+                        p = k % D - sr; q = (k / D) % D - sr; r = k / (D * D) - sr;
+                        w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
                     }
-                } else {
-                    for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
+                    kw[ch] = w;
+                    for (int a = 0; a < 3; a++) krv[ch][a] = kdp[ch][a] = 0;
+                    if (w >= 0) {
+                        int nb = ind + p + g.m4 * q + g.mn4 * r;
+                        krv[ch][0] = W.val[0][nb]; krv[ch][1] = W.val[1][nb]; krv[ch][2] = W.val[2][nb];
+                        kdp[ch][0] = g.lx[ii + p]; kdp[ch][1] = g.ly[jj + q]; kdp[ch][2] = g.lz[kk + r];
+                    }
                 }
-                npt += __popcll(__ballot(v));
-                __syncthreads();
-                if (lane < LSQ_T1) {
-                    int ne = min(64, nc - k0);
+#pragma unroll
+                for (int ch = 0; ch < 2; ch++) {
+                    const int k0 = ch * 64;
+                    if (k0 >= nc) break;
+                    const double w = kw[ch];
+                    bool v = w >= 0;
+                    if (v) {
+                        term[lane][0] = krv[ch][0] * w;
+                        term[lane][1] = krv[ch][1] * w;
+                        term[lane][2] = krv[ch][2] * w;
+                        term[lane][3] = kdp[ch][0] * w;
+                        term[lane][4] = kdp[ch][1] * w;
+                        term[lane][5] = kdp[ch][2] * w;
+                        term[lane][6] = w;
+                    } else {
+                        for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
+                    }
+                    npt += __popcll(__ballot(v));
+                    __syncthreads();
+                    if (lane < LSQ_T1) {
+                        int ne = min(64, nc - k0);
 #pragma unroll 8
-                    for (int e = 0; e < ne; e++) acc += term[e][lane];
+                        for (int e = 0; e < ne; e++) acc += term[e][lane];
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
+            } else {
+                for (int k0 = 0; k0 < nc; k0 += 64) {
+                    int k = k0 + lane;
+                    double w = -1;
+                    int p = 0, q = 0, r = 0;
+                    if (k < nc) {
+                        p = k % D - sr; q = (k / D) % D - sr; r = k / (D * D) - sr;
+                        w = nb_weight(g, W, ind, ii, jj, kk, p, q, r, A.L, cosw, gp, A.wpow);
+                    }
+                    bool v = w >= 0;
+                    if (v) {
+                        int nb = ind + p + g.m4 * q + g.mn4 * r;
+                        double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
+                        double dp[3] = {g.lx[ii + p], g.ly[jj + q], g.lz[kk + r]};
+                        term[lane][0] = rv[0] * w;
+                        term[lane][1] = rv[1] * w;
+                        term[lane][2] = rv[2] * w;
+                        term[lane][3] = dp[0] * w;
+                        term[lane][4] = dp[1] * w;
+                        term[lane][5] = dp[2] * w;
+                        term[lane][6] = w;
+                    } else {
+                        for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
+                    }
+                    npt += __popcll(__ballot(v));
+                    __syncthreads();
+                    if (lane < LSQ_T1) {
+                        int ne = min(64, nc - k0);
+#pragma unroll 8
+                        for (int e = 0; e < ne; e++) acc += term[e][lane];
+                    }
+                    __syncthreads();
+                }
             }
             if (npt == 0) continue;
             if (lane < LSQ_T1) bc[lane] = acc;
