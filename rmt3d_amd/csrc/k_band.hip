@@ -273,15 +273,16 @@ __device__ __forceinline__ double nb_weight(const Geo &g, const Scratch &W, int 
 // (bit-identical), while the stencil gathers of a cell run 64-wide.  A point
 // that is not a data point parks +0 terms: the running sums start at +0 and
 // are never -0, so adding them leaves the sums bit-identical and the summing
-// loops are branch-free.  Terms are stored point-major ([point][quantity]),
-// so the summing lanes read adjacent words and their loads can be issued
-// ahead of the dependent additions.
+// loops are branch-free.  Terms are stored quantity-major ([quantity][point],
+// rows padded to 65 doubles): a store puts the 64 lanes' terms in 64
+// adjacent words, and the summing lanes (one per quantity) read rows whose
+// starts fall in different banks, so neither side conflicts on LDS banks.
 constexpr int LSQ_T1 = 7;    // ravg[3], davg[3], wsum
 constexpr int LSQ_T2 = 15;   // am[6], rhs[9]
 
 __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt,
                                                    LsqArgs A, const BoxBatch *__restrict__ B, int *status) {
-    __shared__ double term[64][LSQ_T2 + 1];
+    __shared__ double term[LSQ_T2][65];
     __shared__ double bc[LSQ_T2];
     const int lane = threadIdx.x;
     int n = *cnt;
@@ -342,22 +343,22 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
                     const double w = kw[ch];
                     bool v = w >= 0;
                     if (v) {
-                        term[lane][0] = krv[ch][0] * w;
-                        term[lane][1] = krv[ch][1] * w;
-                        term[lane][2] = krv[ch][2] * w;
-                        term[lane][3] = kdp[ch][0] * w;
-                        term[lane][4] = kdp[ch][1] * w;
-                        term[lane][5] = kdp[ch][2] * w;
-                        term[lane][6] = w;
+                        term[0][lane] = krv[ch][0] * w;
+                        term[1][lane] = krv[ch][1] * w;
+                        term[2][lane] = krv[ch][2] * w;
+                        term[3][lane] = kdp[ch][0] * w;
+                        term[4][lane] = kdp[ch][1] * w;
+                        term[5][lane] = kdp[ch][2] * w;
+                        term[6][lane] = w;
                     } else {
-                        for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
+                        for (int a = 0; a < LSQ_T1; a++) term[a][lane] = 0.0;
                     }
                     npt += __popcll(__ballot(v));
                     __syncthreads();
                     if (lane < LSQ_T1) {
                         int ne = min(64, nc - k0);
 #pragma unroll 8
-                        for (int e = 0; e < ne; e++) acc += term[e][lane];
+                        for (int e = 0; e < ne; e++) acc += term[lane][e];
                     }
                     __syncthreads();
                 }
@@ -375,22 +376,22 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
                         int nb = ind + p + g.m4 * q + g.mn4 * r;
                         double rv[3] = {W.val[0][nb], W.val[1][nb], W.val[2][nb]};
                         double dp[3] = {g.lx[ii + p], g.ly[jj + q], g.lz[kk + r]};
-                        term[lane][0] = rv[0] * w;
-                        term[lane][1] = rv[1] * w;
-                        term[lane][2] = rv[2] * w;
-                        term[lane][3] = dp[0] * w;
-                        term[lane][4] = dp[1] * w;
-                        term[lane][5] = dp[2] * w;
-                        term[lane][6] = w;
+                        term[0][lane] = rv[0] * w;
+                        term[1][lane] = rv[1] * w;
+                        term[2][lane] = rv[2] * w;
+                        term[3][lane] = dp[0] * w;
+                        term[4][lane] = dp[1] * w;
+                        term[5][lane] = dp[2] * w;
+                        term[6][lane] = w;
                     } else {
-                        for (int a = 0; a < LSQ_T1; a++) term[lane][a] = 0.0;
+                        for (int a = 0; a < LSQ_T1; a++) term[a][lane] = 0.0;
                     }
                     npt += __popcll(__ballot(v));
                     __syncthreads();
                     if (lane < LSQ_T1) {
                         int ne = min(64, nc - k0);
 #pragma unroll 8
-                        for (int e = 0; e < ne; e++) acc += term[e][lane];
+                        for (int e = 0; e < ne; e++) acc += term[lane][e];
                     }
                     __syncthreads();
                 }
@@ -427,25 +428,25 @@ __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratc
                 bool v = Q >= 0;
                 if (v) {
                     double dp0 = dpv[0], dp1 = dpv[1], dp2 = dpv[2];
-                    term[lane][0] = Q * (dp0 - davg[0]) * (dp0 - davg[0]);
-                    term[lane][1] = Q * (dp1 - davg[1]) * (dp1 - davg[1]);
-                    term[lane][2] = Q * (dp2 - davg[2]) * (dp2 - davg[2]);
-                    term[lane][3] = Q * (dp0 - davg[0]) * (dp1 - davg[1]);
-                    term[lane][4] = Q * (dp0 - davg[0]) * (dp2 - davg[2]);
-                    term[lane][5] = Q * (dp1 - davg[1]) * (dp2 - davg[2]);
+                    term[0][lane] = Q * (dp0 - davg[0]) * (dp0 - davg[0]);
+                    term[1][lane] = Q * (dp1 - davg[1]) * (dp1 - davg[1]);
+                    term[2][lane] = Q * (dp2 - davg[2]) * (dp2 - davg[2]);
+                    term[3][lane] = Q * (dp0 - davg[0]) * (dp1 - davg[1]);
+                    term[4][lane] = Q * (dp0 - davg[0]) * (dp2 - davg[2]);
+                    term[5][lane] = Q * (dp1 - davg[1]) * (dp2 - davg[2]);
                     double dd[3] = {dp0 - davg[0], dp1 - davg[1], dp2 - davg[2]};
                     for (int i = 0; i < 3; i++) {
                         double lr = Q * (rv[i] - ravg[i]);
-                        for (int j = 0; j < 3; j++) term[lane][6 + 3 * i + j] = dd[j] * lr;
+                        for (int j = 0; j < 3; j++) term[6 + 3 * i + j][lane] = dd[j] * lr;
                     }
                 } else {
-                    for (int a = 0; a < LSQ_T2; a++) term[lane][a] = 0.0;
+                    for (int a = 0; a < LSQ_T2; a++) term[a][lane] = 0.0;
                 }
                 __syncthreads();
                 if (lane < LSQ_T2) {
                     int ne = min(64, nc - k0);
 #pragma unroll 8
-                    for (int e = 0; e < ne; e++) acc += term[e][lane];
+                    for (int e = 0; e < ne; e++) acc += term[lane][e];
                 }
                 __syncthreads();
             }
