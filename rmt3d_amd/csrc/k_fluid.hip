@@ -481,6 +481,7 @@ __device__ __forceinline__ bool refmap_nvd(const Geo &g, const Maps &M, int rf, 
     const int str[3] = {1, g.m4, g.mn4};
     const double dhsp[3] = {g.dxsp, g.dysp, g.dzsp};
     unsigned oid = M.c(rf) & OMASK;
+#pragma unroll
     for (int d = 0; d < 3; d++) {
         int base[3] = {i + 2, j + 2, k + 2};
         int m1p[3] = {base[0], base[1], base[2]}, p1p[3] = {base[0], base[1], base[2]};
@@ -497,20 +498,31 @@ __device__ __forceinline__ bool refmap_nvd(const Geo &g, const Maps &M, int rf, 
     return true;
 }
 
-// ref_map::godunov_extrap (fields.hh:572-592)
-__device__ __forceinline__ void refmap_faces(const Geo &g, const Phys &P, const Maps &M, const Fluid &U, int rf, int c,
-                             const double nvd[3][3]) {
+// ref_map::godunov_extrap (fields.hh:572-592).  The 18 face states go to
+// the primary or the band arrays through one branch on the map's kind, with
+// constant component indices in each (a per-store choice between the two
+// pointer tables made the compiler keep both tables in scratch memory).
+template <typename FX>
+__device__ __forceinline__ void refmap_faces_to(const Geo &g, const Phys &P, const Fluid &U, const FX &fx, int off,
+                                                const double xv[3], int c, const double nvd[3][3]) {
     const double dh2[3] = {g.dx / 2, g.dy / 2, g.dz / 2};
     double dt2 = P.dt / 2.0;
-    double xv[3] = {M.x(rf, 0), M.x(rf, 1), M.x(rf, 2)};
+#pragma unroll
     for (int fc = 0; fc < 6; fc++) {
-        int dim = fc / 2, a1 = (dim + 1) % 3, a2 = (dim + 2) % 3;
+        const int dim = fc / 2, a1 = (dim + 1) % 3, a2 = (dim + 2) % 3;
         double nv = U.vel[dim][c];
         double d = dh2[dim] * (2 * (fc % 2) - 1);
         double v1 = U.vel[a1][c], v2 = U.vel[a2][c];
+#pragma unroll
         for (int b = 0; b < 3; b++)
-            M.fxr(rf, 3 * fc + b) = xv[b] + (d - dt2 * nv) * nvd[dim][b] - dt2 * (v1 * nvd[a1][b] + v2 * nvd[a2][b]);
+            fx[3 * fc + b][off] = xv[b] + (d - dt2 * nv) * nvd[dim][b] - dt2 * (v1 * nvd[a1][b] + v2 * nvd[a2][b]);
     }
+}
+__device__ __forceinline__ void refmap_faces(const Geo &g, const Phys &P, const Maps &M, const Fluid &U, int rf, int c,
+                             const double nvd[3][3]) {
+    double xv[3] = {M.x(rf, 0), M.x(rf, 1), M.x(rf, 2)};
+    if (rf < M.Ng) refmap_faces_to(g, P, U, M.r.fx, rf, xv, c, nvd);
+    else refmap_faces_to(g, P, U, M.b.fx, rf - M.Ng, xv, c, nvd);
 }
 
 __device__ __forceinline__ bool abut_primary(const Geo &g, const Maps &M, unsigned o, int ind) {
@@ -569,15 +581,31 @@ __global__ void k_advect(Geo g, Phys P, const ObjC *O, Fluid U, Maps M) {
 // acceleration, then their maps' normal derivatives and face states
 // (normal_derivatives_refmap + compute_half_time_edge_refmap), which read
 // only the cell-centred velocity
+// The solid cells' advection runs as two launches over the same list: the
+// velocity face states (advect_cell) and the reference-map face states of
+// the cell's primary and band maps.  They write disjoint fields and read
+// none the other writes; apart, each needs about half the registers of the
+// combined cell, so more wavefronts hide the latency of the map lookups.
 template <bool FORCING>
 __global__ __launch_bounds__(64) void k_advect_solid(Geo g, Phys P, const ObjC *O, Fluid U, Maps M, const int *list, const int *count,
                                int *status) {
     int n = *count;
+    (void)status;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         int c = list[t];
         int i = c % g.m4 - 2, j = (c / g.m4) % g.n4 - 2, k = c / g.mn4 - 2;
         if (i < 0 || i >= g.m || j < 0 || j >= g.n || k < 0 || k >= g.o) continue;
         advect_cell<FORCING, true>(g, P, O, U, M, c, i, j, k);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_advect_solid_maps(Geo g, Phys P, Fluid U, Maps M, const int *list,
+                                                          const int *count, int *status) {
+    int n = *count;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int c = list[t];
+        int i = c % g.m4 - 2, j = (c / g.m4) % g.n4 - 2, k = c / g.mn4 - 2;
+        if (i < 0 || i >= g.m || j < 0 || j >= g.n || k < 0 || k >= g.o) continue;
         double rn[3][3];
         if ((M.r.c[c] & OMASK) > 0) {
             if (!refmap_nvd(g, M, c, c, i, j, k, rn)) { atomicOr(status, ST_FATAL_NVD); continue; }
@@ -1217,6 +1245,7 @@ void launch_advect(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, co
         hipLaunchKernelGGL(k_advect<false>, dim3((n + 255) / 256), dim3(256), 0, s, g, P, O, U, M);
         hipLaunchKernelGGL(k_advect_solid<false>, dim3(2048), dim3(64), 0, s, g, P, O, U, M, list, count, status);
     }
+    hipLaunchKernelGGL(k_advect_solid_maps, dim3(2048), dim3(64), 0, s, g, P, U, M, list, count, status);
 }
 void launch_upwind(hipStream_t s, const Geo &g, const Fluid &U, const Maps &M, int *status) {
     int nup = (g.m + 1) * (g.n + 1) * (g.o + 1);
