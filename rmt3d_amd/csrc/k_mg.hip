@@ -1370,7 +1370,9 @@ __global__ void __launch_bounds__(64) k_fdm_cols(const T *__restrict__ in, T *__
 }
 
 // x: a workgroup of nwv wavefronts stages rb (<= 64) consecutive rows in LDS
-// (pitch d0 + 1) and the W slices of nwv output blocks, in one batch of
+// (pitch d0 | 1: an odd number of words or doublewords, so the 64 lanes'
+// rows start in different LDS banks) and the W slices of nwv output
+// blocks, in one batch of
 // loads; wavefront w computes block blockIdx.y * nwv + w for all rows (lane
 // = row) and stores it straight to the rows.
 template <typename T, int PB>
@@ -1381,7 +1383,7 @@ __global__ void __launch_bounds__(256) k_fdm_rows(const T *__restrict__ in, T *_
     const int nw = fdm_rows(d0) * PB;         // one block's W slice
     T *wsl = reinterpret_cast<T *>(fdm_lds);      // nwv slices
     T *tile = wsl + nwv * nw;
-    const int ld = d0 + 1;
+    const int ld = d0 | 1;
     const int r0 = blockIdx.x * rb;
     const int nr = min(rb, nrows - r0);
     const int pb0 = blockIdx.y * nwv;
@@ -1455,9 +1457,9 @@ static void fdm_pass_pb(hipStream_t s, const MGLevel<T> &L, int a, const T *W, c
         int nwv = npb < 4 ? npb : 4;
         while (nwv > 1 && nwv * wbytes > 32768) nwv--;
         int rb = 64;
-        while (rb > 1 && nwv * wbytes + sizeof(T) * (size_t)rb * (d0 + 1) > 65536) rb /= 2;
+        while (rb > 1 && nwv * wbytes + sizeof(T) * (size_t)rb * (d0 | 1) > 65536) rb /= 2;
         int nrows = d1 * d2;
-        size_t lds = nwv * wbytes + sizeof(T) * (size_t)rb * (d0 + 1);
+        size_t lds = nwv * wbytes + sizeof(T) * (size_t)rb * (d0 | 1);
         hipLaunchKernelGGL((k_fdm_rows<T, PB>), dim3((nrows + rb - 1) / rb, (npb + nwv - 1) / nwv), dim3(64 * nwv), lds,
                            s, in, out, W, d0, nrows, rb);
         return;
