@@ -128,6 +128,52 @@ def test_fdm_mfma_form_gives_the_same_states(monkeypatch):
     b.close()
 
 
+@pytest.mark.parametrize("dims,prd", [((64, 64, 64), (0, 0, 0)), ((64, 48, 80), (1, 0, 1)),
+                                      ((160, 128, 192), (1, 1, 1))])
+def test_mg_solve_fem_pair_form_matches_oracle(dims, prd, monkeypatch):
+    # two FEM colours per launch, plane by plane, on the finest level
+    # (RMT3D_FEM_PAIR=1): the FP64 solve is the oracle's bit for bit, walls,
+    # mixed periodic and all-periodic, with and without an intermediate
+    # level of fused sweeps
+    monkeypatch.setenv("RMT3D_TUNE", "0")
+    monkeypatch.setenv("RMT3D_FEM_PAIR", "1")
+    g = _mg_pair_solve(1, dims, prd, 0, 11)
+    assert g.diag(25) == -1.0
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("maker,n", [("cavity", 64), ("tg", 32)])
+def test_fem_pair_form_gives_identical_states(prec, maker, n, monkeypatch):
+    """Two FEM colours per launch (k_fem_pair: a workgroup per row band of a
+    plane, the band's plane rows in LDS) give every node the bits of two
+    single-colour launches: a wall-bounded cavity with a sphere and a
+    periodic Taylor-Green box step to the same states either way, on both
+    paths (the mixed path's fused first colour and FP64 update included)."""
+    from rmt3d_amd import _abi
+    from rmt3d_amd.lib import Simulation
+    cfg, objs = (configs.cavity_sphere(n, steps=10) if maker == "cavity" else configs.taylor_green(n))
+    p = _abi.PREC_MIXED if prec else _abi.PREC_FP64
+    monkeypatch.setenv("RMT3D_TUNE", "0")
+    a = Simulation(cfg, objs, p)
+    monkeypatch.setenv("RMT3D_FEM_PAIR", "1")
+    b = Simulation(cfg, objs, p)
+    assert a.diag(25) != -1.0 and b.diag(25) == -1.0
+    for s in (a, b):
+        e = s.initialize()
+        if maker == "tg":
+            u, v, w = configs.taylor_green_field(cfg)
+            s.upload(u=u, v=v, w=w)
+            e = 0
+        s.prepare_run(e)
+        for _ in range(3):
+            assert s.step() == 0
+    fa, fb = a.download(), b.download()
+    for k in ("u", "v", "w", "p"):
+        assert np.array_equal(fa[k], fb[k]), k
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("which", [0, 1])
 @pytest.mark.parametrize("dims,prd", [((160, 160, 160), (0, 0, 0)), ((160, 128, 192), (1, 1, 1))])
 @pytest.mark.parametrize("prec", [0, 1])
