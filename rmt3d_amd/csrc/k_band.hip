@@ -1076,21 +1076,39 @@ void launch_band_x_eq_xpred(hipStream_t s, Band B) {
 
 // ---- redraw_boundary (fluid_3d_ex.cc:336-394) ----
 constexpr int RD_MAX = 32;
-__global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.Ng) return;
-    uint32_t lc[RD_MAX];
-    double lx[RD_MAX][3], lp[RD_MAX][3];
+constexpr int RD_LIGHT = 4;    // list entries kept in registers
+// One cell's redraw.  Its list (the band maps, then the primary if it
+// leaves) holds up to CAP entries; with UNR = CAP every loop over it is
+// unrolled to CAP with guards on the list length, so the list lives in
+// registers (a private array indexed at run time lives in scratch memory,
+// whose per-wavefront allocation caps how many wavefronts run).  k_redraw
+// takes the cells with at most RD_LIGHT - 1 band maps that way and lists
+// the others, which k_redraw_heavy takes with CAP = RD_MAX.  Every work-item
+// of k_redraw reaches the staging append at the end, where a wavefront
+// reserves its cells' slots with one atomic (the counts scanned across the
+// wavefront).  The staging order does not matter: the band build sorts
+// each cell's entries.
+template <int CAP, int UNR, bool HEAVY>
+__device__ __forceinline__ void redraw_cell(const Geo &g, const Phys &P, const ObjC *O, const RMap &R, const Band &B,
+                                            const Scratch &W, int nlayers, int *status, int t, bool live) {
+    bool fatal = false;
+    uint32_t lc[CAP];
+    double lx[CAP][3], lp[CAP][3];
     // where each listed map came from: its band entry, or -1 for this
     // cell's primary; the whole record moves in the reference
     // (fluid_3d_ex.cc:345-389), so face energies and det F go with it
-    int lsrc[RD_MAX];
+    int lsrc[CAP];
     int nl = 0;
-    int s = B.start[t], n = B.cnt[t];
-    for (int e = s; e < s + n && nl < RD_MAX - 1; e++, nl++) {
-        lc[nl] = B.c[e];
-        lsrc[nl] = e;
-        for (int a = 0; a < 3; a++) { lx[nl][a] = B.x[a][e]; lp[nl][a] = B.xpred[a][e]; }
+    int s = B.start[t], n = live ? B.cnt[t] : 0;
+    if (n > CAP - 1) n = CAP - 1;
+#pragma unroll UNR
+    for (int q = 0; q < CAP - 1; q++) {
+        if (q >= n) break;
+        const int e = s + q;
+        lc[q] = B.c[e];
+        lsrc[q] = e;
+        for (int a = 0; a < 3; a++) { lx[q][a] = B.x[a][e]; lp[q][a] = B.xpred[a][e]; }
+        nl = q + 1;
     }
     double pen[3], pdet[3];
     uint32_t lflag = (uint32_t)(nlayers + 1) << LDIG;
@@ -1099,18 +1117,18 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
     double pp[3] = {R.xpred[0][t], R.xpred[1][t], R.xpred[2][t]};
     bool occupied = false, changed = false, cleared = false;
     int take = -1;   // band entry that became the primary
-    int oidp = (int)(pc & OMASK);
+    int oidp = live ? (int)(pc & OMASK) : 0;
     if (oidp > 0) {
         double pphi = obj_phi(O, P.n_obj, oidp - 1, px[0], px[1], px[2]);
         if (pphi > 0) {
-            lc[nl] = lflag | pc;
-            lsrc[nl] = -1;
-            for (int a = 0; a < 3; a++) {
-                lx[nl][a] = px[a];
-                lp[nl][a] = pp[a];
-                pen[a] = R.en[a][t];
-                pdet[a] = R.detF[a][t];
-            }
+#pragma unroll UNR
+            for (int q = 0; q < CAP; q++)
+                if (q == nl) {
+                    lc[q] = lflag | pc;
+                    lsrc[q] = -1;
+                    for (int a = 0; a < 3; a++) { lx[q][a] = px[a]; lp[q][a] = pp[a]; }
+                }
+            for (int a = 0; a < 3; a++) { pen[a] = R.en[a][t]; pdet[a] = R.detF[a][t]; }
             nl++;
             pc = 0;
             for (int a = 0; a < 3; a++) { px[a] = 0; pp[a] = 0; }
@@ -1121,8 +1139,10 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             occupied = true;
         }
     }
-    int N = nl;
-    for (int l = 0; l < N; l++) {
+    const int N = nl;
+#pragma unroll UNR
+    for (int l = 0; l < CAP; l++) {
+        if (l >= N || fatal) break;
         if (l >= nl) continue;
         uint32_t oid = lc[l] & OMASK;
         double sphi = obj_phi(O, P.n_obj, (int)oid - 1, lx[l][0], lx[l][1], lx[l][2]);
@@ -1131,7 +1151,11 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
         int po = (int)(pc & OMASK);
         double pphi = po > 0 ? obj_phi(O, P.n_obj, po - 1, px[0], px[1], px[2]) : __builtin_nan("");
         if ((occupied && sphi < pphi) || !occupied) {
-            if (pphi < -g.dx) { atomicOr(status, ST_FATAL_REDRAW); return; }
+            if (pphi < -g.dx) {
+                atomicOr(status, ST_FATAL_REDRAW);
+                fatal = true;
+                break;
+            }
             pc = oid;
             for (int a = 0; a < 3; a++) { px[a] = lx[l][a]; pp[a] = lp[l][a]; }
             take = lsrc[l];
@@ -1139,20 +1163,26 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             atomicAdd(&P.counters[2], 1);
             changed = true;
             cleared = false;
+            // remove the first entry of that object, shifting the rest down
             int j0 = -1;
-            for (int b = 0; b < nl; b++)
-                if ((lc[b] & OMASK) == oid && j0 == -1) j0 = b;
+#pragma unroll UNR
+            for (int b = 0; b < CAP; b++)
+                if (b < nl && (lc[b] & OMASK) == oid && j0 == -1) j0 = b;
             if (j0 >= 0) {
-                for (int b = j0; b < nl - 1; b++) {
-                    lc[b] = lc[b + 1];
-                    lsrc[b] = lsrc[b + 1];
-                    for (int a = 0; a < 3; a++) { lx[b][a] = lx[b + 1][a]; lp[b][a] = lp[b + 1][a]; }
+#pragma unroll UNR
+                for (int b = 0; b < CAP - 1; b++) {
+                    if (b >= j0 && b < nl - 1) {
+                        lc[b] = lc[b + 1];
+                        lsrc[b] = lsrc[b + 1];
+                        for (int a = 0; a < 3; a++) { lx[b][a] = lx[b + 1][a]; lp[b][a] = lp[b + 1][a]; }
+                    }
                 }
                 nl--;
             }
         }
     }
-    if (changed) {
+    if (fatal) nl = 0;
+    if (changed && !fatal) {
         R.c[t] = pc;
         for (int a = 0; a < 3; a++) { R.x[a][t] = px[a]; R.xpred[a][t] = pp[a]; }
         if (cleared) {
@@ -1163,11 +1193,28 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
             for (int a = 0; a < 3; a++) { R.detF[a][t] = B.detF[a][take]; R.en[a][t] = B.en[a][take]; }
         }
     }
-    if (nl == 0) return;
-    int base = atomicAdd(W.st_n, nl);
-    for (int e = 0; e < nl; e++) {
+    int base;
+    if (HEAVY) {
+        base = nl > 0 ? atomicAdd(W.st_n, nl) : 0;
+    } else {
+        // wavefront scan of the counts, one reservation per wavefront
+        const int lane = __lane_id();
+        int incl = nl;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63, 64);
+        int b0 = 0;
+        if (lane == 63 && total > 0) b0 = atomicAdd(W.st_n, total);
+        base = __shfl(b0, 63, 64) + incl - nl;
+    }
+#pragma unroll UNR
+    for (int e = 0; e < CAP; e++) {
+        if (e >= nl) break;
         int p = base + e;
-        if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); return; }
+        if (p >= W.st_cap) { atomicOr(status, ST_BAND_OVERFLOW); break; }
         W.st_cell[p] = t;
         W.st_c[p] = lc[e];
         for (int a = 0; a < 3; a++) { W.st_x[a][p] = lx[e][a]; W.st_xpred[a][p] = lp[e][a]; }
@@ -1178,10 +1225,32 @@ __global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W
         }
     }
 }
+
+// cells with at most RD_LIGHT - 1 band maps; the others go to hlist
+__global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status,
+                         int *hlist, int *hcount) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = t < g.Ng;
+    if (!live) t = 0;
+    if (live && B.cnt[t] > RD_LIGHT - 1) {
+        hlist[atomicAdd(hcount, 1)] = t;
+        live = false;
+    }
+    redraw_cell<RD_LIGHT, RD_LIGHT, false>(g, P, O, R, B, W, nlayers, status, t, live);
+}
+__global__ void k_redraw_heavy(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status,
+                               const int *hlist, const int *hcount) {
+    int n = *hcount;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        redraw_cell<RD_MAX, 1, true>(g, P, O, R, B, W, nlayers, status, hlist[q], true);
+}
 void launch_redraw(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W, int nlayers,
                    int *status) {
     launch_zero_i32(s, W.st_n, 1);
-    hipLaunchKernelGGL(k_redraw, g1(g.Ng), dim3(256), 0, s, g, P, O, R, B, W, nlayers, status);
+    launch_zero_i32(s, W.cnt + 1, 1);
+    hipLaunchKernelGGL(k_redraw, g1(g.Ng), dim3(256), 0, s, g, P, O, R, B, W, nlayers, status, W.list[1], W.cnt + 1);
+    hipLaunchKernelGGL(k_redraw_heavy, dim3(64), dim3(64), 0, s, g, P, O, R, B, W, nlayers, status,
+                       (const int *)W.list[1], (const int *)(W.cnt + 1));
 }
 
 // ---- reductions ----
