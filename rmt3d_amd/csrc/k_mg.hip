@@ -16,6 +16,7 @@
 
 #include "dev.h"
 #include "launch.h"
+#include <type_traits>
 
 namespace rmt {
 
@@ -1871,16 +1872,16 @@ static void fdm_fused(hipStream_t s, const MGLevel<float> &L, const FdmDev<float
     hipLaunchKernelGGL(k_fdm_plane_mx<true>, dim3(d2), blk, bp, s, (const float *)L.r, L.x, F.S[0], F.ST[0], F.S[1],
                        F.ST[1], d0, d1);
 }
-static void fdm_fused(hipStream_t, const MGLevel<double> &, const FdmDev<double> &) {}
-
 // forward along x, y, z (the last scaled by D^+), back along z, y, x,
 // through the level's r and x arrays (b is left as it is): one launch per
 // contraction, or (FP32 with F.fused) the three MFMA launches
 template <typename T>
 void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
-    if (F.fused) {
-        fdm_fused(s, L, F);
-        return;
+    if constexpr (std::is_same<T, float>::value) {
+        if (F.fused) {
+            fdm_fused(s, L, F);
+            return;
+        }
     }
     fdm_pass<T>(s, L, 0, F.pb[0], F.S[0], nullptr, L.b, L.r);
     fdm_pass<T>(s, L, 1, F.pb[1], F.S[1], nullptr, L.r, L.x);
