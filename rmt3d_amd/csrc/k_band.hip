@@ -280,8 +280,12 @@ __device__ __forceinline__ double nb_weight(const Geo &g, const Scratch &W, int 
 constexpr int LSQ_T1 = 7;    // ravg[3], davg[3], wsum
 constexpr int LSQ_T2 = 15;   // am[6], rhs[9]
 
+// cnt_next: the next layer's list counter, cleared here (nothing reads it
+// during this launch)
 __global__ __launch_bounds__(64) void k_gather_lsq(Geo g, RMap R, Band T, Scratch W, const int *list, const int *cnt,
-                                                   LsqArgs A, const BoxBatch *__restrict__ B, int *status) {
+                                                   LsqArgs A, const BoxBatch *__restrict__ B, int *status,
+                                                   int *cnt_next) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cnt_next = 0;
     __shared__ double term[LSQ_T2][65];
     __shared__ double bc[LSQ_T2];
     const int lane = threadIdx.x;
@@ -625,7 +629,9 @@ __device__ __forceinline__ int batch_cell(const Geo &g, const BoxBatch *__restri
     const int tot_ = (B)->total;                                                           \
     for (int t_ = blockIdx.x * blockDim.x + threadIdx.x; t_ < tot_; t_ += gridDim.x * blockDim.x)
 
-__global__ void k_batch_init(Geo g, RMap R, Scratch W, const BoxBatch *__restrict__ B, uint32_t nmask) {
+// also clears the counter of the first layer's cell list
+__global__ void k_batch_init(Geo g, RMap R, Scratch W, const BoxBatch *__restrict__ B, uint32_t nmask, int *cnt0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cnt0 = 0;
     BATCH_LOOP(B) {
         int b;
         int t = batch_cell(g, B, t_, b);
@@ -663,16 +669,6 @@ __global__ void k_batch_comm_mask(Geo g, Scratch W, const BoxBatch *__restrict__
         if (s >= 0) W.mask[t] = W.mask[s];
     }
 }
-__global__ void k_batch_scan_ghost(Geo g, Scratch W, const BoxBatch *__restrict__ B, uint32_t nmask) {
-    BATCH_LOOP(B) {
-        int b;
-        int t = batch_cell(g, B, t_, b);
-        int i, j, k;
-        unpack(g, t, i, j, k);
-        bool ghost = (i < 2 || i > g.m + 1 || j < 2 || j > g.n + 1 || k < 2 || k > g.o + 1);
-        if (ghost && W.mask[t] == nmask) W.flag[t] = 1;
-    }
-}
 __global__ void k_batch_wrap_layer(Geo g, Scratch W, const BoxBatch *__restrict__ B, int L) {
     BATCH_LOOP(B) {
         int b;
@@ -685,16 +681,32 @@ __global__ void k_batch_wrap_layer(Geo g, Scratch W, const BoxBatch *__restrict_
         for (int a = 0; a < 3; a++) W.val[a][t] = W.val[a][s];
     }
 }
-// the workgroup-uniform loops keep every thread at each block_append
-__global__ __launch_bounds__(1024) void k_batch_compact(Geo g, Scratch W, const BoxBatch *__restrict__ B, int *list, int *cnt) {
+// The next layer's cell list: the flagged cells, appended and unflagged.
+// The workgroup-uniform loops keep every thread at each block_append.  Each
+// cell's decision reads only its own state, so the passes before it fold
+// in: APPLY (no periodic dimension) first gives each flagged cell the new
+// mask tag (as k_batch_apply_flags does; without periodic images a ghost
+// cell carries the new tag only if it was flagged), and SCAN (periodic)
+// first flags the ghost cells that carry the tag after the periodic mask
+// copy.
+template <bool APPLY, bool SCAN>
+__global__ __launch_bounds__(1024) void k_batch_compact(Geo g, Scratch W, const BoxBatch *__restrict__ B, int *list,
+                                                        int *cnt, uint32_t nmask) {
     __shared__ int sh[17];
     const int tot = B->total;
     for (int b0 = blockIdx.x * blockDim.x; b0 < tot; b0 += gridDim.x * blockDim.x) {
         int tt = b0 + threadIdx.x, b;
         int t = tt < tot ? batch_cell(g, B, tt, b) : 0;
         bool f = tt < tot && W.flag[t] != 0;
+        if (SCAN && tt < tot && !f) {
+            int i, j, k;
+            unpack(g, t, i, j, k);
+            bool ghost = (i < 2 || i > g.m + 1 || j < 2 || j > g.n + 1 || k < 2 || k > g.o + 1);
+            f = ghost && W.mask[t] == nmask;
+        }
         int p = block_append(f, cnt, sh);
         if (f) {
+            if (APPLY) W.mask[t] = nmask;
             list[p] = t;
             W.flag[t] = 0;
         }
@@ -949,30 +961,32 @@ int band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, 
             hipLaunchKernelGGL(k_mask_reset, g1(g.Ng), dim3(256), 0, s, g, W);
         }
         uint32_t cur_mask = nmask;
-        hipLaunchKernelGGL(k_batch_init, ge, dim3(256), 0, s, g, R, W, bb, nmask);
+        int cur = 0;
+        hipLaunchKernelGGL(k_batch_init, ge, dim3(256), 0, s, g, R, W, bb, nmask, W.cnt + cur);
         nmask++;
         hipLaunchKernelGGL(k_batch_first_push, ge, dim3(256), 0, s, g, W, bb, cur_mask);
-        hipLaunchKernelGGL(k_batch_apply_flags, ge, dim3(256), 0, s, g, W, bb, nmask);
-        if (per) hipLaunchKernelGGL(k_batch_comm_mask, ge, dim3(256), 0, s, g, W, bb);
-        hipLaunchKernelGGL(k_batch_scan_ghost, ge, dim3(256), 0, s, g, W, bb, nmask);
-        int cur = 0;
-        launch_zero_i32(s, W.cnt + cur, 1);
-        hipLaunchKernelGGL(k_batch_compact, gc, dim3(1024), 0, s, g, W, bb, W.list[cur], W.cnt + cur);
+        // flags -> mask tag, periodic mask copy, ghost scan, compaction
+        auto next_list = [&](int *list, int *cnt) {
+            if (per) {
+                hipLaunchKernelGGL(k_batch_apply_flags, ge, dim3(256), 0, s, g, W, bb, nmask);
+                hipLaunchKernelGGL(k_batch_comm_mask, ge, dim3(256), 0, s, g, W, bb);
+                hipLaunchKernelGGL((k_batch_compact<false, true>), gc, dim3(1024), 0, s, g, W, bb, list, cnt, nmask);
+            } else {
+                hipLaunchKernelGGL((k_batch_compact<true, false>), gc, dim3(1024), 0, s, g, W, bb, list, cnt, nmask);
+            }
+        };
+        next_list(W.list[cur], W.cnt + cur);
         for (int L = 1; L <= nlayers; L++) {
             nmask++;
             LsqArgs A;
             A.L = L; A.max_rs = max_rs; A.wpow = wpow_dev;
-            hipLaunchKernelGGL(k_gather_lsq, dim3(8192), dim3(64), 0, s, g, R, W.tband, W,
-                               W.list[cur], W.cnt + cur, A, bb, status);
             int nxt = cur ^ 1;
-            launch_zero_i32(s, W.cnt + nxt, 1);
+            hipLaunchKernelGGL(k_gather_lsq, dim3(8192), dim3(64), 0, s, g, R, W.tband, W,
+                               W.list[cur], W.cnt + cur, A, bb, status, W.cnt + nxt);
             if (L < nlayers) {
                 hipLaunchKernelGGL(k_list_push, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, W, W.list[cur],
                                    W.cnt + cur, cur_mask);
-                hipLaunchKernelGGL(k_batch_apply_flags, ge, dim3(256), 0, s, g, W, bb, nmask);
-                if (per) hipLaunchKernelGGL(k_batch_comm_mask, ge, dim3(256), 0, s, g, W, bb);
-                hipLaunchKernelGGL(k_batch_scan_ghost, ge, dim3(256), 0, s, g, W, bb, nmask);
-                hipLaunchKernelGGL(k_batch_compact, gc, dim3(1024), 0, s, g, W, bb, W.list[nxt], W.cnt + nxt);
+                next_list(W.list[nxt], W.cnt + nxt);
             }
             cur = nxt;
             if (per) hipLaunchKernelGGL(k_batch_wrap_layer, ge, dim3(256), 0, s, g, W, bb, L);
