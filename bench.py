@@ -196,7 +196,7 @@ def run_gpu(args, precision):
                          "mac_colour_pass_workgroup": sim.diag(24), "fem_colour_pass_workgroup": sim.diag(25),
                          "mac_fdm_outputs_per_item_xyz": int(sim.diag(26)),
                          "fem_fdm_outputs_per_item_xyz": int(sim.diag(27)),
-                         "mac_fdm_three_launch": int(sim.diag(31)), "fem_fdm_three_launch": int(sim.diag(32)),
+                         "mac_fdm_form": int(sim.diag(31)), "fem_fdm_form": int(sim.diag(32)),
                          "mac_vcycle_sweeps_down_up": int(sim.diag(33)),
                          "fem_vcycle_sweeps_down_up": int(sim.diag(34))},
            "stress_ms_per_step": prof["stress"][0] / max(args.kernel_steps, 1), "stage_ms_per_step": stages,

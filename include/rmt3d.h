@@ -251,8 +251,10 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * 30 -> object batches of the last band extrapolation (objects whose
  * extrapolation boxes are disjoint share one sequence of launches);
  * 31/32 -> MAC/FEM fast-diagonalisation form in the V-cycles' precision
- * (0 one launch per contraction, 1 three launches: x-y planes, z pencils,
- * planes), picked by the autotuner where the three-launch form fits;
+ * (0 one launch per contraction; three MFMA launches, x-y planes, z
+ * pencils, planes: 1 with each plane and its bases staged in LDS, 2 with
+ * direct operands, one workgroup per 16-wide output block), picked by the
+ * autotuner among the forms that fit the level;
  * 33/34 -> MAC/FEM V-cycle smoothing sweeps, 10 v_down + v_up (2 + 2 on the
  * FP64 path, the reference's default; picked by the autotuner on the mixed
  * path, whose FP32 V-cycle only preconditions the FP64 iteration). */

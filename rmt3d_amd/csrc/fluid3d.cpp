@@ -321,8 +321,9 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
         const char *ff = getenv("RMT3D_FDM_FUSED");
         bool allow = !(ff && ff[0] == '0');
         fdm64.fused = 0;
-        fdm32.fused = (allow && mixed) ? mg_fdm_fused_enable(L[nlev - 1].d) : 0;
-        fdm_fused_ok = 2 * fdm32.fused;
+        fdm32.forms = (allow && mixed) ? mg_fdm_fused_enable(L[nlev - 1].d) : 0;
+        fdm32.fused = (fdm32.forms & 2) ? 2 : (fdm32.forms & 1);
+        fdm_fused_ok = fdm32.forms ? 2 : 0;
     } else {
         tail_lds64 = mg_tail_enable_lds<double>(mg_tail_lds_bytes(L + tail_lev, nlev - tail_lev));
         tail_lds32 = mixed ? mg_tail_enable_lds<float>(mg_tail_lds_bytes(Lf + tail_lev, nlev - tail_lev)) : 0;
@@ -518,7 +519,7 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
     auto iteration = [&]() { cycle_body(s); };
     if (fdm) {
         // the six-launch form's outputs per work-item, then that form
-        // against the three-launch one (where it fits)
+        // against the three-launch ones that fit
         const int fused_ok = F.fused;
         F.fused = 0;
         for (int a = 0; a < 3; a++) {
@@ -531,11 +532,14 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
             F.pb[a] = t[argmin(t)] < 1e29 ? pbs[argmin(t)] : keep;
         }
         if (fused_ok) {
-            std::vector<double> t = time_candidates(s, 2, [&](int k) {
-                F.fused = k;
+            std::vector<int> forms = {0};
+            for (int f = 1; f <= 2; f++)
+                if (F.forms & f) forms.push_back(f);
+            std::vector<double> t = time_candidates(s, (int)forms.size(), [&](int k) {
+                F.fused = forms[k];
                 iteration();
             });
-            F.fused = t[argmin(t)] < 1e29 ? argmin(t) : fused_ok;
+            F.fused = t[argmin(t)] < 1e29 ? forms[argmin(t)] : fused_ok;
         }
     }
     for (int l = last - 1; l >= 0; l--) {
@@ -597,7 +601,8 @@ void MGSolver::tune(hipStream_t s) {
         return;
     }
     bool sweep_fixed = getenv("RMT3D_SWEEP") != nullptr;
-    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, fdm_fused_ok, pair_fixed ? 1 : 0,
+    std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, fdm_fused_ok, fdm32.forms,
+                            pair_fixed ? 1 : 0,
                             mixed ? mixed_vcycle_env() : 0, nlev, fdm ? 1 : 0,
                             prd[0], prd[1], prd[2]};
     for (int l = 0; l < nlev; l++)

@@ -1835,6 +1835,208 @@ __global__ void __launch_bounds__(FDM_MX_BLOCK) k_fdm_pencils_mx(float *__restri
     });
 }
 
+// ---- the same three launches with direct operands (every extent <= 128) --
+// A staged workgroup holds a whole plane (or 64 pencils) and its bases in
+// LDS, so one CU's matrix cores run all of a plane's tiles and the level
+// occupies only d2 CUs.  Here a workgroup computes one 16-wide block of a
+// plane's output (or 16 pencils), one wavefront per 16-row tile: the
+// operands are read straight from memory (the bases stay in L1/L2) into
+// registers, every chunk of a tile at once, and only the 16-wide
+// intermediate product passes through LDS.  Operands, k-slot order and
+// roundings are those of the staged kernels, so the results are identical.
+constexpr int FDM_DX_MAXD = 128;
+constexpr int FDM_DX_PITCH = FDM_DX_MAXD + 4;
+
+// One 16 x 16 tile over nc <= NC 16-wide chunks of the contraction:
+// ld_a(c) and ld_b(c) give this lane's A[m0 + r][16 c + 4 slot + s] and
+// B[16 c + 4 slot + s][n0 + r] for s = 0..3 (r = lane % 16, slot = lane / 16).
+// All NC chunks are loaded unconditionally (the loaders return zeros past
+// the extents), so the loads form one batch; the products stop at nc.
+template <int NC, typename LA, typename LB>
+__device__ __forceinline__ fdm_f4 fdm_dx_tile(int nc, LA ld_a, LB ld_b) {
+    fdm_f4 a[NC], b[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        a[c] = ld_a(c);
+        b[c] = ld_b(c);
+    }
+    fdm_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+        if (c < nc) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][0], b[c][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][1], b[c][1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][2], b[c][2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[c][3], b[c][3], acc, 0, 0, 0);
+        }
+    return acc;
+}
+
+// four consecutive entries M[row][k0 .. k0+3] of a rows x cols row-major
+// matrix (pitch ld), zero outside it.  Entries outside are loaded from M[0]
+// and dropped, so that every load is unconditional and a tile's loads are
+// all in flight together.
+__device__ __forceinline__ fdm_f4 fdm_dx_row4(const float *__restrict__ M, int ld, int row, int rows, int k0,
+                                              int cols) {
+    fdm_f4 v;
+    const bool in = row < rows;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const bool ok = in && k0 + s < cols;
+        const float x = M[ok ? (size_t)row * ld + k0 + s : 0];
+        v[s] = ok ? x : 0.f;
+    }
+    return v;
+}
+// M[k0 + s][col], s = 0..3, zero outside
+__device__ __forceinline__ fdm_f4 fdm_dx_col4(const float *__restrict__ M, int ld, int col, int cols, int k0,
+                                              int rows) {
+    fdm_f4 v;
+    const bool in = col < cols;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const bool ok = in && k0 + s < rows;
+        const float x = M[ok ? (size_t)(k0 + s) * ld + col : 0];
+        v[s] = ok ? x : 0.f;
+    }
+    return v;
+}
+
+// forward, plane blockIdx.y, output columns p in [16 blockIdx.x, +16):
+// T[j][p] = sum_q plane[j][q] S0[q][p] (wave w: rows j of tile w), then
+// out[p1][p] = sum_j S1[j][p1] T[j][p] (wave w: rows p1 of tile w);
+// 64 ceil(d1 / 16) work-items
+template <int NC>
+__global__ void __launch_bounds__(64 * NC) k_fdm_plane_dx_fwd(const float *__restrict__ in,
+                                                                     float *__restrict__ out,
+                                                                     const float *__restrict__ ST0,
+                                                                     const float *__restrict__ ST1, int d0, int d1) {
+    __shared__ __align__(16) float TT[16 * FDM_DX_PITCH];   // T[j][n0 + r] at [r][j]
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, slot = lane >> 4;
+    const int n0 = blockIdx.x * 16, m = 16 * w + r;
+    const size_t np = (size_t)d0 * d1;
+    const float *src = in + blockIdx.y * np;
+    float *dst = out + blockIdx.y * np;
+    const int nc0 = fdm_x16(d0) >> 4, nc1 = fdm_x16(d1) >> 4;
+    fdm_f4 t = fdm_dx_tile<NC>(
+        nc0, [&](int c) { return fdm_dx_row4(src, d0, m, d1, 16 * c + 4 * slot, d0); },
+        [&](int c) { return fdm_dx_row4(ST0, d0, n0 + r, d0, 16 * c + 4 * slot, d0); });
+    *reinterpret_cast<fdm_f4 *>(TT + r * FDM_DX_PITCH + 16 * w + 4 * slot) = t;
+    __syncthreads();
+    fdm_f4 o = fdm_dx_tile<NC>(
+        nc1, [&](int c) { return fdm_dx_row4(ST1, d1, m, d1, 16 * c + 4 * slot, d1); },
+        [&](int c) { return *reinterpret_cast<const fdm_f4 *>(TT + r * FDM_DX_PITCH + 16 * c + 4 * slot); });
+    const int p = n0 + r;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int p1 = 16 * w + 4 * slot + v;
+        if (p1 < d1 && p < d0) dst[p1 * d0 + p] = o[v];
+    }
+}
+
+// inverse, plane blockIdx.y, output rows j in [16 blockIdx.x, +16):
+// O1[j][i] = sum_q S1[j][q] plane[q][i] (wave w: columns i of tile w), then
+// out[j][p] = sum_i O1[j][i] S0[p][i] (wave w: columns p of tile w);
+// 64 ceil(d0 / 16) work-items
+template <int NC>
+__global__ void __launch_bounds__(64 * NC) k_fdm_plane_dx_inv(const float *__restrict__ in,
+                                                                     float *__restrict__ out,
+                                                                     const float *__restrict__ S0,
+                                                                     const float *__restrict__ S1, int d0, int d1) {
+    __shared__ __align__(16) float O1[16 * FDM_DX_PITCH];   // O1[m0 + a][i] at [a][i]
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, slot = lane >> 4;
+    const int m0 = blockIdx.x * 16, n = 16 * w + r;
+    const size_t np = (size_t)d0 * d1;
+    const float *src = in + blockIdx.y * np;
+    float *dst = out + blockIdx.y * np;
+    const int nc0 = fdm_x16(d0) >> 4, nc1 = fdm_x16(d1) >> 4;
+    fdm_f4 o = fdm_dx_tile<NC>(
+        nc1, [&](int c) { return fdm_dx_row4(S1, d1, m0 + r, d1, 16 * c + 4 * slot, d1); },
+        [&](int c) { return fdm_dx_col4(src, d0, n, d0, 16 * c + 4 * slot, d1); });
+#pragma unroll
+    for (int v = 0; v < 4; v++) O1[(4 * slot + v) * FDM_DX_PITCH + n] = o[v];
+    __syncthreads();
+    fdm_f4 u = fdm_dx_tile<NC>(
+        nc0, [&](int c) { return *reinterpret_cast<const fdm_f4 *>(O1 + r * FDM_DX_PITCH + 16 * c + 4 * slot); },
+        [&](int c) { return fdm_dx_row4(S0, d0, n, d0, 16 * c + 4 * slot, d0); });
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int j = m0 + 4 * slot + v;
+        if (j < d1 && n < d0) dst[j * d0 + n] = u[v];
+    }
+}
+
+// pencils c in [16 blockIdx.x, +16) of the np = d0 d1 z-pencils, in place:
+// U[p][c] = D^+ sum_q S2[q][p] values[q][c], then values[p][c] =
+// sum_q S2[p][q] U[q][c] (wave w: rows p of tile w); 64 ceil(d2 / 16)
+// work-items
+template <int NC>
+__global__ void __launch_bounds__(64 * NC) k_fdm_pencils_dx(float *__restrict__ io,
+                                                                   const float *__restrict__ S2,
+                                                                   const float *__restrict__ ST2,
+                                                                   const float *__restrict__ dinv, int np, int d2) {
+    __shared__ __align__(16) float UT[16 * FDM_DX_PITCH];   // U[p][c0 + r] at [r][p]
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, slot = lane >> 4;
+    const int c = blockIdx.x * 16 + r, m = 16 * w + r;
+    const int nc2 = fdm_x16(d2) >> 4;
+    fdm_f4 u = fdm_dx_tile<NC>(
+        nc2, [&](int k) { return fdm_dx_row4(ST2, d2, m, d2, 16 * k + 4 * slot, d2); },
+        [&](int k) { return fdm_dx_col4(io, np, c, np, 16 * k + 4 * slot, d2); });
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int p = 16 * w + 4 * slot + v;
+        if (p < d2 && c < np) u[v] *= dinv[(size_t)p * np + c];
+    }
+    *reinterpret_cast<fdm_f4 *>(UT + r * FDM_DX_PITCH + 16 * w + 4 * slot) = u;
+    __syncthreads();
+    fdm_f4 o = fdm_dx_tile<NC>(
+        nc2, [&](int k) { return fdm_dx_row4(S2, d2, m, d2, 16 * k + 4 * slot, d2); },
+        [&](int k) { return *reinterpret_cast<const fdm_f4 *>(UT + r * FDM_DX_PITCH + 16 * k + 4 * slot); });
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int p = 16 * w + 4 * slot + v;
+        if (p < d2 && c < np) io[(size_t)p * np + c] = o[v];
+    }
+}
+
+static bool fdm_dx_fits(const int d[3]) {
+    return d[0] <= FDM_DX_MAXD && d[1] <= FDM_DX_MAXD && d[2] <= FDM_DX_MAXD;
+}
+
+// NC: chunks loaded per tile, at least the chunks of either contraction of
+// the launch (a plane kernel's work-items: 64 per 16-row tile, fewer than
+// 64 NC when the extents differ)
+template <int NC>
+static void fdm_direct_nc(hipStream_t s, const MGLevel<float> &L, const FdmDev<float> &F, int plane) {
+    const int d0 = L.d[0], d1 = L.d[1], d2 = L.d[2], np = d0 * d1;
+    const int nc0 = fdm_x16(d0) >> 4, nc1 = fdm_x16(d1) >> 4, nc2 = fdm_x16(d2) >> 4;
+    if (plane == 0)
+        hipLaunchKernelGGL(k_fdm_plane_dx_fwd<NC>, dim3(nc0, d2), dim3(64 * nc1), 0, s, (const float *)L.b, L.r,
+                           F.ST[0], F.ST[1], d0, d1);
+    else if (plane == 1)
+        hipLaunchKernelGGL(k_fdm_pencils_dx<NC>, dim3((np + 15) / 16), dim3(64 * nc2), 0, s, L.r, F.S[2], F.ST[2],
+                           F.dinv, np, d2);
+    else
+        hipLaunchKernelGGL(k_fdm_plane_dx_inv<NC>, dim3(nc1, d2), dim3(64 * nc0), 0, s, (const float *)L.r, L.x,
+                           F.S[0], F.S[1], d0, d1);
+}
+
+static void fdm_direct(hipStream_t s, const MGLevel<float> &L, const FdmDev<float> &F) {
+    const int nc01 = fdm_x16(std::max(L.d[0], L.d[1])) >> 4, nc2 = fdm_x16(L.d[2]) >> 4;
+    for (int k = 0; k < 3; k++) {
+        switch (k == 1 ? nc2 : nc01) {
+            case 1: fdm_direct_nc<1>(s, L, F, k); break;
+            case 2: fdm_direct_nc<2>(s, L, F, k); break;
+            case 3: fdm_direct_nc<3>(s, L, F, k); break;
+            case 4: fdm_direct_nc<4>(s, L, F, k); break;
+            case 5: fdm_direct_nc<5>(s, L, F, k); break;
+            case 6: fdm_direct_nc<6>(s, L, F, k); break;
+            case 7: fdm_direct_nc<7>(s, L, F, k); break;
+            default: fdm_direct_nc<8>(s, L, F, k); break;
+        }
+    }
+}
+
 static size_t fdm_mx_plane_bytes(const int d[3]) {
     size_t X0 = fdm_x16(d[0]), X1 = fdm_x16(d[1]), p0 = fdm_xp(d[0]), p1 = fdm_xp(d[1]);
     return sizeof(float) * (X1 * p0 + X0 * p0 + X1 * p1 + X0 * p1);
@@ -1847,11 +2049,13 @@ static size_t fdm_mx_pencil_bytes(const int d[3]) {
 size_t mg_fdm_fused_bytes(const int d[3]) { return std::max(fdm_mx_plane_bytes(d), fdm_mx_pencil_bytes(d)); }
 
 int mg_fdm_fused_enable(const int d[3]) {
+    const char *dx = getenv("RMT3D_FDM_DIRECT");
+    const int direct = (fdm_dx_fits(d) && !(dx && dx[0] == '0')) ? 2 : 0;
     int dev = 0, maxb = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    if (hipGetDevice(&dev) != hipSuccess) return direct;
+    if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return direct;
     size_t bytes = mg_fdm_fused_bytes(d);
-    if (bytes > (size_t)maxb || bytes > MG_TAIL_LDS_MAX) return 0;
+    if (bytes > (size_t)maxb || bytes > MG_TAIL_LDS_MAX) return direct;
     // every context of the process shares the kernels' attribute: raise it
     // to the most any shape may use
     const int lim = (int)std::min((size_t)maxb, MG_TAIL_LDS_MAX);
@@ -1859,7 +2063,7 @@ int mg_fdm_fused_enable(const int d[3]) {
     (void)hipFuncSetAttribute((const void *)k_fdm_plane_mx<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)k_fdm_pencils_mx, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipGetLastError();
-    return 1;
+    return 1 | direct;
 }
 
 static void fdm_fused(hipStream_t s, const MGLevel<float> &L, const FdmDev<float> &F) {
@@ -1878,6 +2082,10 @@ static void fdm_fused(hipStream_t s, const MGLevel<float> &L, const FdmDev<float
 template <typename T>
 void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
     if constexpr (std::is_same<T, float>::value) {
+        if (F.fused == 2) {
+            fdm_direct(s, L, F);
+            return;
+        }
         if (F.fused) {
             fdm_fused(s, L, F);
             return;

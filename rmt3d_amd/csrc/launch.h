@@ -78,10 +78,15 @@ template <typename T>
 struct FdmDev {
     const T *S[3], *ST[3], *dinv;
     int pb[3] = {8, 8, 8};   // outputs per work-item of each dimension's passes (2, 4 or 8)
-    int fused = 0;           // FP32 only, 1: the three MFMA launches (planes, pencils, planes)
+    // FP32 only, the three MFMA launches (planes, pencils, planes): 1 with
+    // each plane and its bases staged in LDS, 2 with direct operands (one
+    // workgroup per 16-wide output block); 0: one launch per contraction
+    int fused = 0;
+    int forms = 0;           // bit 0: form 1 fits this level, bit 1: form 2
 };
-// LDS bytes of the FP32 three-launch form for a level of dimensions d, and
-// enabling it (1 when that fits a workgroup)
+// LDS bytes of the staged FP32 three-launch form for a level of dimensions
+// d, and enabling the MFMA forms: returns the forms that fit (bit 0 staged,
+// bit 1 direct: every extent <= 128, unless RMT3D_FDM_DIRECT=0)
 size_t mg_fdm_fused_bytes(const int d[3]);
 int mg_fdm_fused_enable(const int d[3]);
 // host: the 1D basis of one dimension (fem 1: Q1 nodes, 0: MAC cells) and

@@ -80,24 +80,64 @@ def test_mg_solve_fdm_level_matches_oracle(which, dims, prd, prec):
 
 
 @pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("direct", [1, 0])
 @pytest.mark.parametrize("dims,prd,fits", [((128, 128, 128), (0, 0, 0), (1, 1)), ((64, 64, 64), (1, 1, 1), (1, 1)),
                                            ((64, 48, 80), (1, 0, 1), (1, 1)), ((192, 96, 4), (0, 0, 0), (1, 1)),
                                            ((4, 96, 192), (0, 0, 0), (1, 0))])
-def test_mg_solve_fdm_mfma_form_matches_oracle(which, dims, prd, fits, monkeypatch):
+def test_mg_solve_fdm_mfma_form_matches_oracle(which, direct, dims, prd, fits, monkeypatch):
     # with the built-in defaults the mixed path's FP32 fast-diagonalisation
-    # solve runs as three MFMA launches wherever its planes, pencil blocks
-    # and bases fit a workgroup's LDS (fits: MAC, FEM level), and as six
+    # solve runs as three MFMA launches with direct operands (form 2: every
+    # extent of these levels is at most 128); with RMT3D_FDM_DIRECT=0, as
+    # the three launches staged in LDS wherever planes, pencil blocks and
+    # bases fit a workgroup (fits: MAC, FEM level), and as six
     # single-contraction launches where they do not (the 3 x 49 x 97 FEM
     # level: padded 112 x 112 z-bases beside 64 pencils); the FP64 defect
     # correction around it converges to the oracle's solution
     monkeypatch.setenv("RMT3D_TUNE", "0")
+    if not direct:
+        monkeypatch.setenv("RMT3D_FDM_DIRECT", "0")
     g = _mg_pair_solve(which, dims, prd, 1, 7)
-    assert g.diag(18 + which) == 1.0 and g.diag(31 + which) == float(fits[which])
+    assert g.diag(18 + which) == 1.0
+    assert g.diag(31 + which) == (2.0 if direct else float(fits[which]))
+
+
+@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("dims,prd", [((128, 128, 128), (0, 0, 0)), ((64, 48, 80), (1, 0, 1)),
+                                      ((192, 96, 4), (0, 0, 0)), ((96, 64, 56), (0, 1, 0))])
+def test_fdm_direct_form_is_bitwise_the_staged_form(which, dims, prd, monkeypatch):
+    # the direct-operand MFMA kernels take the staged kernels' operands in
+    # the same k-slot order: the mixed solves' iterates, and so their
+    # solutions and cycle counts, are identical bit for bit
+    from rmt3d_amd import _abi
+    from rmt3d_amd.lib import Simulation
+    cfg, _ = configs.taylor_green(8)
+    cfg.sys_size[:] = list(dims)
+    cfg.prd[:] = list(prd)
+    for a in range(3):
+        cfg.xyz_bounds[2 * a] = 0.0
+        cfg.xyz_bounds[2 * a + 1] = dims[a] / 64.0
+    monkeypatch.setenv("RMT3D_TUNE", "0")
+    monkeypatch.setenv("RMT3D_FDM_DIRECT", "0")
+    a = Simulation(cfg, [], _abi.PREC_MIXED)
+    monkeypatch.delenv("RMT3D_FDM_DIRECT")
+    b = Simulation(cfg, [], _abi.PREC_MIXED)
+    assert a.diag(31 + which) == 1.0 and b.diag(31 + which) == 2.0
+    n = int(np.prod(a.mg_dims(which)))
+    rng = np.random.default_rng(17 + which)
+    rhs = rng.standard_normal(n)
+    rhs -= rhs.mean()
+    xa, na, ra = a.mg_solve(which, rhs, np.zeros(n))
+    xb, nb, rb = b.mg_solve(which, rhs, np.zeros(n))
+    print("direct vs staged", dims, prd, which, "cycles", na, nb, "res", ra, rb, flush=True)
+    assert na == nb and ra == rb
+    assert np.array_equal(xa, xb)
+    a.close()
+    b.close()
 
 
 def test_fdm_mfma_form_gives_the_same_states(monkeypatch):
     """The mixed path's FP32 fast-diagonalisation solve on MFMA (three
-    launches) and as six single-contraction launches (RMT3D_FDM_FUSED=0)
+    launches, direct operands) and as six single-contraction launches (RMT3D_FDM_FUSED=0)
     precondition the same FP64 defect correction: a 64^3 cavity with a
     sphere steps to velocities and pressures 1e-10 apart (relative L2; the
     velocity as one field, since v alone is nearly zero in the lid-driven
@@ -112,7 +152,7 @@ def test_fdm_mfma_form_gives_the_same_states(monkeypatch):
     b = Simulation(cfg, objs, _abi.PREC_MIXED)
     assert a.diag(18) == 1.0 and a.diag(19) == 1.0
     assert a.diag(31) == 0.0 and a.diag(32) == 0.0
-    assert b.diag(31) == 1.0 and b.diag(32) == 1.0
+    assert b.diag(31) == 2.0 and b.diag(32) == 2.0
     for s in (a, b):
         e = s.initialize()
         s.prepare_run(e)
