@@ -250,12 +250,33 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
             fdm64.dinv = d_fdm + off[6];
         }
         if (ok && mixed) {
+            // FP32: the same arrays, then zero-padded copies of the bases
+            // (rows of 16-float multiples, each copy 64-float aligned) for
+            // the direct-operand MFMA form
             std::vector<float> hf(h.begin(), h.end());
-            ok = hipMalloc(&f_fdm, sizeof(float) * tot) == hipSuccess &&
-                 hipMemcpy(f_fdm, hf.data(), sizeof(float) * tot, hipMemcpyHostToDevice) == hipSuccess;
-            bytes += sizeof(float) * tot;
+            size_t offp[6];
+            for (int a = 0; a < 3; a++) {
+                const size_t d = c.d[a], X = (d + 15) & ~(size_t)15;
+                for (int t = 0; t < 2; t++) {
+                    hf.resize((hf.size() + 63) & ~(size_t)63, 0.f);
+                    offp[2 * a + t] = hf.size();
+                    hf.resize(hf.size() + X * X, 0.f);
+                    const double *src = h.data() + off[t ? 3 + a : a];
+                    float *dst = hf.data() + offp[2 * a + t];
+                    for (size_t i = 0; i < d; i++)
+                        for (size_t j = 0; j < d; j++) dst[i * X + j] = (float)src[i * d + j];
+                }
+            }
+            ok = hipMalloc(&f_fdm, sizeof(float) * hf.size()) == hipSuccess &&
+                 hipMemcpy(f_fdm, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice) == hipSuccess;
+            bytes += sizeof(float) * hf.size();
             if (ok) {
-                for (int a = 0; a < 3; a++) { fdm32.S[a] = f_fdm + off[a]; fdm32.ST[a] = f_fdm + off[3 + a]; }
+                for (int a = 0; a < 3; a++) {
+                    fdm32.S[a] = f_fdm + off[a];
+                    fdm32.ST[a] = f_fdm + off[3 + a];
+                    fdm32.SP[a] = f_fdm + offp[2 * a];
+                    fdm32.STP[a] = f_fdm + offp[2 * a + 1];
+                }
                 fdm32.dinv = f_fdm + off[6];
             }
         }

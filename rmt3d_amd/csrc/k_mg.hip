@@ -1888,6 +1888,13 @@ __device__ __forceinline__ fdm_f4 fdm_dx_row4(const float *__restrict__ M, int l
     }
     return v;
 }
+// four consecutive entries of row `row` of a zero-padded X x X basis
+// (X = 16 nc): chunk c < nc, the chunks past nc read chunk nc - 1 (their
+// products are not taken)
+__device__ __forceinline__ fdm_f4 fdm_dx_pad4(const float *__restrict__ M, int X, int row, int c, int nc, int slot) {
+    const int cc = c < nc ? c : nc - 1;
+    return *reinterpret_cast<const fdm_f4 *>(M + row * X + 16 * cc + 4 * slot);
+}
 // M[k0 + s][col], s = 0..3, zero outside
 __device__ __forceinline__ fdm_f4 fdm_dx_col4(const float *__restrict__ M, int ld, int col, int cols, int k0,
                                               int rows) {
@@ -1911,6 +1918,7 @@ __global__ void __launch_bounds__(64 * NC) k_fdm_plane_dx_fwd(const float *__res
                                                                      float *__restrict__ out,
                                                                      const float *__restrict__ ST0,
                                                                      const float *__restrict__ ST1, int d0, int d1) {
+    // ST0, ST1: padded (pitch 16 nc0, 16 nc1)
     __shared__ __align__(16) float TT[16 * FDM_DX_PITCH];   // T[j][n0 + r] at [r][j]
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, slot = lane >> 4;
     const int n0 = blockIdx.x * 16, m = 16 * w + r;
@@ -1920,11 +1928,11 @@ __global__ void __launch_bounds__(64 * NC) k_fdm_plane_dx_fwd(const float *__res
     const int nc0 = fdm_x16(d0) >> 4, nc1 = fdm_x16(d1) >> 4;
     fdm_f4 t = fdm_dx_tile<NC>(
         nc0, [&](int c) { return fdm_dx_row4(src, d0, m, d1, 16 * c + 4 * slot, d0); },
-        [&](int c) { return fdm_dx_row4(ST0, d0, n0 + r, d0, 16 * c + 4 * slot, d0); });
+        [&](int c) { return fdm_dx_pad4(ST0, 16 * nc0, n0 + r, c, nc0, slot); });
     *reinterpret_cast<fdm_f4 *>(TT + r * FDM_DX_PITCH + 16 * w + 4 * slot) = t;
     __syncthreads();
     fdm_f4 o = fdm_dx_tile<NC>(
-        nc1, [&](int c) { return fdm_dx_row4(ST1, d1, m, d1, 16 * c + 4 * slot, d1); },
+        nc1, [&](int c) { return fdm_dx_pad4(ST1, 16 * nc1, m, c, nc1, slot); },
         [&](int c) { return *reinterpret_cast<const fdm_f4 *>(TT + r * FDM_DX_PITCH + 16 * c + 4 * slot); });
     const int p = n0 + r;
 #pragma unroll
@@ -1943,6 +1951,7 @@ __global__ void __launch_bounds__(64 * NC) k_fdm_plane_dx_inv(const float *__res
                                                                      float *__restrict__ out,
                                                                      const float *__restrict__ S0,
                                                                      const float *__restrict__ S1, int d0, int d1) {
+    // S0, S1: padded (pitch 16 nc0, 16 nc1)
     __shared__ __align__(16) float O1[16 * FDM_DX_PITCH];   // O1[m0 + a][i] at [a][i]
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, slot = lane >> 4;
     const int m0 = blockIdx.x * 16, n = 16 * w + r;
@@ -1951,14 +1960,14 @@ __global__ void __launch_bounds__(64 * NC) k_fdm_plane_dx_inv(const float *__res
     float *dst = out + blockIdx.y * np;
     const int nc0 = fdm_x16(d0) >> 4, nc1 = fdm_x16(d1) >> 4;
     fdm_f4 o = fdm_dx_tile<NC>(
-        nc1, [&](int c) { return fdm_dx_row4(S1, d1, m0 + r, d1, 16 * c + 4 * slot, d1); },
+        nc1, [&](int c) { return fdm_dx_pad4(S1, 16 * nc1, m0 + r, c, nc1, slot); },
         [&](int c) { return fdm_dx_col4(src, d0, n, d0, 16 * c + 4 * slot, d1); });
 #pragma unroll
     for (int v = 0; v < 4; v++) O1[(4 * slot + v) * FDM_DX_PITCH + n] = o[v];
     __syncthreads();
     fdm_f4 u = fdm_dx_tile<NC>(
         nc0, [&](int c) { return *reinterpret_cast<const fdm_f4 *>(O1 + r * FDM_DX_PITCH + 16 * c + 4 * slot); },
-        [&](int c) { return fdm_dx_row4(S0, d0, n, d0, 16 * c + 4 * slot, d0); });
+        [&](int c) { return fdm_dx_pad4(S0, 16 * nc0, n, c, nc0, slot); });
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         const int j = m0 + 4 * slot + v;
@@ -1975,12 +1984,13 @@ __global__ void __launch_bounds__(64 * NC) k_fdm_pencils_dx(float *__restrict__ 
                                                                    const float *__restrict__ S2,
                                                                    const float *__restrict__ ST2,
                                                                    const float *__restrict__ dinv, int np, int d2) {
+    // S2, ST2: padded (pitch 16 nc2)
     __shared__ __align__(16) float UT[16 * FDM_DX_PITCH];   // U[p][c0 + r] at [r][p]
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, slot = lane >> 4;
     const int c = blockIdx.x * 16 + r, m = 16 * w + r;
     const int nc2 = fdm_x16(d2) >> 4;
     fdm_f4 u = fdm_dx_tile<NC>(
-        nc2, [&](int k) { return fdm_dx_row4(ST2, d2, m, d2, 16 * k + 4 * slot, d2); },
+        nc2, [&](int k) { return fdm_dx_pad4(ST2, 16 * nc2, m, k, nc2, slot); },
         [&](int k) { return fdm_dx_col4(io, np, c, np, 16 * k + 4 * slot, d2); });
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -1990,7 +2000,7 @@ __global__ void __launch_bounds__(64 * NC) k_fdm_pencils_dx(float *__restrict__ 
     *reinterpret_cast<fdm_f4 *>(UT + r * FDM_DX_PITCH + 16 * w + 4 * slot) = u;
     __syncthreads();
     fdm_f4 o = fdm_dx_tile<NC>(
-        nc2, [&](int k) { return fdm_dx_row4(S2, d2, m, d2, 16 * k + 4 * slot, d2); },
+        nc2, [&](int k) { return fdm_dx_pad4(S2, 16 * nc2, m, k, nc2, slot); },
         [&](int k) { return *reinterpret_cast<const fdm_f4 *>(UT + r * FDM_DX_PITCH + 16 * k + 4 * slot); });
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -2012,13 +2022,13 @@ static void fdm_direct_nc(hipStream_t s, const MGLevel<float> &L, const FdmDev<f
     const int nc0 = fdm_x16(d0) >> 4, nc1 = fdm_x16(d1) >> 4, nc2 = fdm_x16(d2) >> 4;
     if (plane == 0)
         hipLaunchKernelGGL(k_fdm_plane_dx_fwd<NC>, dim3(nc0, d2), dim3(64 * nc1), 0, s, (const float *)L.b, L.r,
-                           F.ST[0], F.ST[1], d0, d1);
+                           F.STP[0], F.STP[1], d0, d1);
     else if (plane == 1)
-        hipLaunchKernelGGL(k_fdm_pencils_dx<NC>, dim3((np + 15) / 16), dim3(64 * nc2), 0, s, L.r, F.S[2], F.ST[2],
+        hipLaunchKernelGGL(k_fdm_pencils_dx<NC>, dim3((np + 15) / 16), dim3(64 * nc2), 0, s, L.r, F.SP[2], F.STP[2],
                            F.dinv, np, d2);
     else
         hipLaunchKernelGGL(k_fdm_plane_dx_inv<NC>, dim3(nc1, d2), dim3(64 * nc0), 0, s, (const float *)L.r, L.x,
-                           F.S[0], F.S[1], d0, d1);
+                           F.SP[0], F.SP[1], d0, d1);
 }
 
 static void fdm_direct(hipStream_t s, const MGLevel<float> &L, const FdmDev<float> &F) {

@@ -83,6 +83,8 @@ struct FdmDev {
     // workgroup per 16-wide output block); 0: one launch per contraction
     int fused = 0;
     int forms = 0;           // bit 0: form 1 fits this level, bit 1: form 2
+    // form 2: S and ST zero-padded to X x X, X = d rounded up to 16
+    const T *SP[3] = {nullptr, nullptr, nullptr}, *STP[3] = {nullptr, nullptr, nullptr};
 };
 // LDS bytes of the staged FP32 three-launch form for a level of dimensions
 // d, and enabling the MFMA forms: returns the forms that fit (bit 0 staged,
