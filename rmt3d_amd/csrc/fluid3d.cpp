@@ -483,8 +483,9 @@ static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3]
 // unknown (the fused sweeps are bit-identical to single-colour passes), so a
 // choice changes time only, never results.  Each candidate is timed over a
 // few whole iterations of the solve on the solver's own arrays (zero at
-// set-up, zeroed again afterwards), with events on the context stream; the
-// minimum over three interleaved rounds decides.  Choices are cached per operator, level shape and
+// set-up, zeroed again afterwards), replayed from a graph as the solve runs
+// them, with events on the context stream; the minimum over three
+// interleaved rounds decides.  Choices are cached per operator, level shape and
 // precision for the life of the process, so contexts of one shape (a serving
 // mix) tune once.  RMT3D_TUNE=0 keeps the built-in defaults; RMT3D_SWEEP, when
 // set, fixes the passes per launch.
@@ -527,6 +528,55 @@ std::vector<double> time_candidates(hipStream_t s, int ncand, Op op) {
 int argmin(const std::vector<double> &v) { return (int)(std::min_element(v.begin(), v.end()) - v.begin()); }
 }  // namespace
 
+// Times one iteration of the solve per candidate as the solve will run it:
+// each candidate's iteration (after configure(c)) is captured into a graph
+// of its own and the graphs are replayed, or, where the solve runs eagerly
+// (RMT3D_NO_GRAPH, per-kernel profiling, a failed capture), the launches
+// are issued eagerly.  Eager timing adds each launch's host cost, which
+// favours the candidates with fewer launches.  Runs inside Sim::create,
+// which holds hip_setup_mutex, so no other context captures meanwhile.
+template <typename Cfg>
+std::vector<double> MGSolver::time_iterations(hipStream_t s, int ncand, Cfg configure) {
+    bool eager = graph_failed || no_graph || (prof && prof->enabled);
+    std::vector<hipGraphExec_t> ex(ncand, nullptr);
+    for (int c = 0; c < ncand && !eager; c++) {
+        configure(c);
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            eager = true;
+            break;
+        }
+        cycle_body(s);
+        hipError_t e1 = hipStreamEndCapture(s, &gr);
+        hipError_t e2 = (e1 == hipSuccess) ? hipGraphInstantiate(&ex[c], gr, nullptr, nullptr, 0) : e1;
+        if (gr) hipGraphDestroy(gr);
+        if (e2 != hipSuccess) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                hipGraph_t g2 = nullptr;
+                (void)hipStreamEndCapture(s, &g2);
+                if (g2) hipGraphDestroy(g2);
+            }
+            (void)hipGetLastError();
+            ex[c] = nullptr;
+            eager = true;
+        }
+    }
+    std::vector<double> t;
+    if (eager)
+        t = time_candidates(s, ncand, [&](int c) {
+            configure(c);
+            cycle_body(s);
+        });
+    else
+        t = time_candidates(s, ncand, [&](int c) { hipGraphLaunch(ex[c], s); });
+    hipStreamSynchronize(s);
+    for (hipGraphExec_t e : ex)
+        if (e) hipGraphExecDestroy(e);
+    return t;
+}
+
 template <typename T>
 void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool sweep_fixed) {
     const int ncol = fem ? 4 : 2;
@@ -537,7 +587,6 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
     // alone on data that stays in L2 prefers other workgroup sizes than it
     // does inside the cycle.  Choices are made from the coarsest level up,
     // every other level keeping its current (default or chosen) shape.
-    auto iteration = [&]() { cycle_body(s); };
     if (fdm) {
         // the six-launch form's outputs per work-item, then that form
         // against the three-launch ones that fit
@@ -546,20 +595,14 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
         for (int a = 0; a < 3; a++) {
             static const int pbs[3] = {2, 4, 8};
             int keep = F.pb[a];
-            std::vector<double> t = time_candidates(s, 3, [&](int k) {
-                F.pb[a] = pbs[k];
-                iteration();
-            });
+            std::vector<double> t = time_iterations(s, 3, [&](int k) { F.pb[a] = pbs[k]; });
             F.pb[a] = t[argmin(t)] < 1e29 ? pbs[argmin(t)] : keep;
         }
         if (fused_ok) {
             std::vector<int> forms = {0};
             for (int f = 1; f <= 2; f++)
                 if (F.forms & f) forms.push_back(f);
-            std::vector<double> t = time_candidates(s, (int)forms.size(), [&](int k) {
-                F.fused = forms[k];
-                iteration();
-            });
+            std::vector<double> t = time_iterations(s, (int)forms.size(), [&](int k) { F.fused = forms[k]; });
             F.fused = t[argmin(t)] < 1e29 ? forms[argmin(t)] : fused_ok;
         }
     }
@@ -576,10 +619,9 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
             cand.assign(1, {npass[l], 0});
         if (pair_fixed && smooth_bs[l] == -1) continue;
         int keep_np = npass[l], keep_bs = smooth_bs[l];
-        std::vector<double> t = time_candidates(s, (int)cand.size(), [&](int c) {
+        std::vector<double> t = time_iterations(s, (int)cand.size(), [&](int c) {
             npass[l] = cand[c].first;
             smooth_bs[l] = cand[c].second;
-            iteration();
         });
         int w = argmin(t);
         if (t[w] < 1e29) {
@@ -718,7 +760,7 @@ void MGSolver::tune_schedule(hipStream_t s) {
             res = resid_max(s);
         }
         if (!(res <= thr)) continue;
-        std::vector<double> t = time_candidates(s, 1, [&](int) { cycle_body(s); });
+        std::vector<double> t = time_iterations(s, 1, [&](int) {});
         double cost = (it + 2) * t[0];
         if (t[0] < 1e29 && cost < best * (1 - 1e-3)) {
             best = cost;

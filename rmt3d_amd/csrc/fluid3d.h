@@ -81,6 +81,8 @@ struct MGSolver {
     // the per-process cache of that level shape)
     void tune(hipStream_t s);
     void tune_schedule(hipStream_t s);
+    template <typename Cfg>
+    std::vector<double> time_iterations(hipStream_t s, int ncand, Cfg configure);
     template <typename T>
     void tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool sweep_fixed);
     void release();
