@@ -440,6 +440,96 @@ __device__ __forceinline__ T resid_value(const MGLevel<T> &L, const Prd &P, int 
     return L.b[c] - (s + dg * L.x[c]);
 }
 
+// FEM residual with the interior marched along z.  Work-items [0, NB)
+// take the boundary nodes one each (the general stencil, map_node's
+// order); the rest take a column segment (i, j, k0 .. k0 + RES_KC - 1) of
+// the interior box each and walk it in k, holding the 3 x 3 neighbourhoods
+// of planes k - 1, k and k + 1 in registers: a node costs nine loads of x
+// (the new plane) instead of twenty-one.  The terms are summed in
+// fem_off_interior's order, so every residual is the same bit for bit.
+constexpr int RES_KC = 8;
+
+struct MarchMap {
+    NodeMap M;
+    int NB;      // boundary nodes
+    int nkc;     // z segments of the interior box
+    int items;   // NB + wi * wj * nkc
+};
+
+static MarchMap march_map(const int d[3]) {
+    MarchMap A;
+    A.M = node_map(d, 1);
+    A.NB = A.M.total - A.M.NI;
+    A.nkc = (d[2] - 2 + RES_KC - 1) / RES_KC;
+    A.items = A.NB + A.M.wi * A.M.wj * A.nkc;
+    return A;
+}
+
+// calls emit(c, j, k, r) for every node of work-item t
+template <typename T, typename E>
+__device__ __forceinline__ void resid_march(const MGLevel<T> &L, const Prd &P, const MarchMap &A, int t, E emit) {
+    if (t < A.NB) {
+        int i, j, k;
+        map_node(L, A.M, t, i, j, k);
+        int c = i + L.d[0] * (j + L.d[1] * k);
+        emit(c, j, k, resid_value(L, P, 1, false, i, j, k, c));
+        return;
+    }
+    const NodeMap &M = A.M;
+    int u = t - A.NB;
+    int row = qdiv(u, M.wi, M.rwi);
+    const int i = 1 + (u - row * M.wi);
+    int kc = qdiv(row, M.wj, M.rwj);
+    const int j = 1 + (row - kc * M.wj);
+    const int k0 = 1 + kc * RES_KC;
+    const int D0 = L.d[0], D01 = L.d[0] * L.d[1];
+    const int nk = min(RES_KC, L.d[2] - 1 - k0);
+    const T *x = L.x;
+    // q[a][b][e]: plane k - 1 + a, row j - 1 + b, column i - 1 + e
+    T q[3][3][3];
+    int c = i + D0 * j + D01 * k0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) q[a][b][e] = x[c + (a - 1) * D01 + (b - 1) * D0 + (e - 1)];
+#pragma unroll
+    for (int kk = 0; kk < RES_KC; kk++) {
+        if (kk >= nk) break;
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) q[2][b][e] = x[c + D01 + (b - 1) * D0 + (e - 1)];
+        const T bc = L.b[c];
+        T s = 0;
+        s += T(-1) * q[0][0][0]; s += T(-2) * q[0][0][1]; s += T(-1) * q[0][0][2];
+        s += T(-2) * q[0][1][0]; s += T(-2) * q[0][1][2];
+        s += T(-1) * q[0][2][0]; s += T(-2) * q[0][2][1]; s += T(-1) * q[0][2][2];
+        s += T(-2) * q[1][0][0]; s += T(-2) * q[1][0][2];
+        s += T(-2) * q[1][2][0]; s += T(-2) * q[1][2][2];
+        s += T(-1) * q[2][0][0]; s += T(-2) * q[2][0][1]; s += T(-1) * q[2][0][2];
+        s += T(-2) * q[2][1][0]; s += T(-2) * q[2][1][2];
+        s += T(-1) * q[2][2][0]; s += T(-2) * q[2][2][1]; s += T(-1) * q[2][2][2];
+        emit(c, j, k0 + kk, bc - (s + T(32) * q[1][1][1]));
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                q[0][b][e] = q[1][b][e];
+                q[1][b][e] = q[2][b][e];
+            }
+        c += D01;
+    }
+}
+
+template <typename T>
+__global__ void k_resid_march(MGLevel<T> L, Prd P, MarchMap A) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.items) return;
+    resid_march(L, P, A, t, [&](int c, int, int, T r) { L.r[c] = r; });
+}
+
 template <typename T>
 __global__ void k_resid(MGLevel<T> L, Prd P, NodeMap M) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1023,6 +1113,27 @@ __global__ void k_resid_to_f32(MGLevel<double> L, Prd P, NodeMap M, float *bf, f
     xf[c] = x0;
 }
 
+// k_resid_to_f32 for FEM levels with the interior marched along z
+template <bool FIRST>
+__global__ void k_resid_to_f32_march(MGLevel<double> L, Prd P, MarchMap A, float *bf, float *xf) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.items) return;
+    resid_march(L, P, A, t, [&](int c, int j, int k, double rd) {
+        float r = (float)rd;
+        bf[c] = r;
+        float x0 = 0.f;
+        if (FIRST && ((j | k) & 1) == 0) {
+            int i = c - L.d[0] * (j + L.d[1] * k);
+            float rdg = 0.03125f;
+            if (!P.p[0] && (i == 0 || i == L.d[0] - 1)) rdg *= 2.f;
+            if (!P.p[1] && (j == 0 || j == L.d[1] - 1)) rdg *= 2.f;
+            if (!P.p[2] && (k == 0 || k == L.d[2] - 1)) rdg *= 2.f;
+            x0 = r * rdg;
+        }
+        xf[c] = x0;
+    });
+}
+
 __global__ void k_cast_d2f(const double *a, float *b, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) b[t] = (float)a[t];
@@ -1233,6 +1344,15 @@ int mg_fem_pair_ok(const int d[3], const int prd[3], int esize) {
 
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
 
+// FEM residuals marched along z (RMT3D_RES_MARCH=0: one work-item per node)
+static bool res_march_on() {
+    static const bool on = [] {
+        const char *e = getenv("RMT3D_RES_MARCH");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 // Tiles of at most 17, 9 or 5 nodes per dimension: the largest that gives
 // 256 workgroups (one per CU) and whose region (tile plus halo, unknowns and
 // right-hand sides) fits in LDS, else 5.  A region holds at most
@@ -1291,6 +1411,11 @@ void launch_mg_copy(hipStream_t s, const T *a, T *b, int n) {
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]) {
     NodeMap M = node_map(L.d, fem);
+    if (!M.plain && res_march_on()) {
+        MarchMap A = march_map(L.d);
+        hipLaunchKernelGGL(k_resid_march<T>, flat_grid(A.items), dim3(256), 0, s, L, mkprd(prd), A);
+        return;
+    }
     hipLaunchKernelGGL(k_resid<T>, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M);
 }
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
@@ -1358,6 +1483,16 @@ void launch_mg_tail(hipStream_t s, int fem, const MGLevel<T> *L, int n, const in
 void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf,
                          int first_pass) {
     NodeMap M = node_map(L.d, fem);
+    if (!M.plain && res_march_on()) {
+        MarchMap A = march_map(L.d);
+        if (first_pass)
+            hipLaunchKernelGGL(k_resid_to_f32_march<true>, flat_grid(A.items), dim3(256), 0, s, L, mkprd(prd), A, bf,
+                               xf);
+        else
+            hipLaunchKernelGGL(k_resid_to_f32_march<false>, flat_grid(A.items), dim3(256), 0, s, L, mkprd(prd), A, bf,
+                               xf);
+        return;
+    }
     if (first_pass)
         hipLaunchKernelGGL(k_resid_to_f32<true>, flat_grid(M.total), dim3(256), 0, s, L, mkprd(prd), M, bf, xf);
     else
