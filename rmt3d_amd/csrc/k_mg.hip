@@ -149,8 +149,28 @@ __global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, 
     if (k >= L.d[2]) return;
     int i = 2 * ih + ((j + k + color) & 1);
     if (i >= L.d[0]) return;
-    T v = mac_smooth_at(L, P, i, j, k);
-    if (x64) x64[i + L.d[0] * (j + L.d[1] * k)] += (double)v;
+    const int c = i + L.d[0] * (j + L.d[1] * k);
+    T v;
+    if ((L.d[0] & 1) == 0) {
+        // Even rows: the node and its row partner c ^ 1 (the other colour,
+        // which this pass leaves unchanged) are written as one aligned pair,
+        // so that the stores fill whole memory sectors instead of every
+        // other word of them.  Only this work-item writes the partner, and
+        // with the value it already holds.
+        T dg;
+        T sm = mac_off(L, P, L.x, i, j, k, dg);
+        T xn = (L.b[c] - sm) / dg;
+        v = T(1 - MAC_OMEGA) * L.x[c] + T(MAC_OMEGA) * xn;
+        const T o = L.x[c ^ 1];
+        typedef T pair_t __attribute__((ext_vector_type(2)));
+        pair_t pv;
+        pv[0] = (c & 1) ? o : v;
+        pv[1] = (c & 1) ? v : o;
+        *reinterpret_cast<pair_t *>(L.x + (c & ~1)) = pv;
+    } else {
+        v = mac_smooth_at(L, P, i, j, k);
+    }
+    if (x64) x64[c] += (double)v;
 }
 
 template <typename T>
