@@ -244,8 +244,7 @@ int rmt3d_mg_dims(const rmt3d_ctx *ctx, int which, int dims[3]);
  * redrawn onto the primary grid (redraw_boundary, fluid_3d_ex.cc:336-394);
  * launch shapes picked by the multigrid autotuner at rmt3d_create (no
  * counterpart in the reference): 24/25 -> MAC/FEM finest-level colour-pass
- * workgroup size (0 = built-in default, -1 = two FEM colours per launch,
- * plane by plane), 26/27 -> MAC/FEM fast-diagonalisation
+ * workgroup size (0 = built-in default), 26/27 -> MAC/FEM fast-diagonalisation
  * outputs per work-item, pb_x + 10 pb_y + 100 pb_z, 28/29 -> 0 not tuned,
  * 1 tuned by measurement, 2 taken from the process's cache of that shape;
  * 30 -> object batches of the last band extrapolation (objects whose

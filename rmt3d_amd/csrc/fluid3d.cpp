@@ -324,15 +324,6 @@ void MGSolver::setup(int fem_, const int d[3], const int prd_[3], bool mixed_) {
             if (*p == ',') p++;
         }
     }
-    // RMT3D_FEM_PAIR=1: two FEM colours per launch on every level smoothed
-    // one pass per launch, where the form applies (the autotuner keeps it)
-    if (fem)
-        if (const char *fp = getenv("RMT3D_FEM_PAIR"))
-            if (fp[0] == '1') {
-                pair_fixed = true;
-                for (int l = 0; l < nlev; l++)
-                    if (npass[l] == 1 && mg_fem_pair_ok(L[l].d, prd, (int)sizeof(double))) smooth_bs[l] = -1;
-            }
     mg_sweep_enable();
     if (fdm) {
         // the FP32 fast-diagonalisation solve (mixed path) runs in three
@@ -419,30 +410,10 @@ double MGSolver::resid_max(hipStream_t s) {
 // level): the first sweep starts at colour first_col (the colours before it
 // were applied by the residual kernel), and every pass of the last sweep adds
 // its nodes' new values to the FP64 solution x64.
-// bs < 0 (FEM, one pass per launch): two colours per launch, plane by plane
-// (k_fem_pair); a sweep that starts at an odd colour applies that one alone.
 template <typename T>
 static void smooth(hipStream_t s, int fem, const MGLevel<T> &l, const int prd[3], int np, int nsweeps, KProf *kp,
                    int bs = 0, int first_col = 0, double *x64 = nullptr) {
     int ncol = fem ? 4 : 2;
-    if (fem && np == 1 && bs < 0) {
-        for (int sw = 0; sw < nsweeps; sw++) {
-            double *xa = sw == nsweeps - 1 ? x64 : nullptr;
-            for (int c0 = (sw == 0) ? first_col : 0; c0 < ncol;) {
-                hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (kp) kp->pair(e0, e1);
-                if ((c0 & 1) == 0) {
-                    launch_mg_fem_pair(s, l, prd, c0 / 2, e0, e1, xa);
-                    c0 += 2;
-                } else {
-                    launch_mg_smooth(s, fem, l, prd, c0, e0, e1, 0, xa);
-                    c0 += 1;
-                }
-            }
-        }
-        return;
-    }
-    if (bs < 0) bs = 0;
     // a halo no wider than a periodic extent wraps at most once
     for (int a = 0; a < 3; a++)
         if (prd[a] && l.d[a] < np) np = l.d[a];
@@ -611,13 +582,10 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
         std::vector<std::pair<int, int>> cand;
         static const int sizes[5] = {64, 128, 256, 512, 1024};
         for (int bsz : sizes) cand.push_back({1, bsz});
-        // FEM: two colours per launch, plane by plane (smooth_bs -1)
-        if (fem && mg_fem_pair_ok(LV[l].d, prd, (int)sizeof(T))) cand.push_back({1, -1});
         if (!sweep_fixed)
             for (int np = 2; np <= ncol; np *= 2) cand.push_back({np, 0});
         else if (npass[l] > 1)
             cand.assign(1, {npass[l], 0});
-        if (pair_fixed && smooth_bs[l] == -1) continue;
         int keep_np = npass[l], keep_bs = smooth_bs[l];
         std::vector<double> t = time_iterations(s, (int)cand.size(), [&](int c) {
             npass[l] = cand[c].first;
@@ -665,7 +633,6 @@ void MGSolver::tune(hipStream_t s) {
     }
     bool sweep_fixed = getenv("RMT3D_SWEEP") != nullptr;
     std::vector<int> key = {fem, mixed ? 4 : 8, sweep_fixed ? 1 : 0, fuse_off ? 1 : 0, fdm_fused_ok, fdm32.forms,
-                            pair_fixed ? 1 : 0,
                             mixed ? mixed_vcycle_env() : 0, nlev, fdm ? 1 : 0,
                             prd[0], prd[1], prd[2]};
     for (int l = 0; l < nlev; l++)

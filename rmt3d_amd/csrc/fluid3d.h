@@ -101,7 +101,6 @@ struct MGSolver {
     bool graph_failed = false;
     bool no_graph = false;
     bool fuse_off = false;                  // RMT3D_MIXED_FUSE=0 at set-up: separate launches
-    bool pair_fixed = false;                // RMT3D_FEM_PAIR=1 at set-up
     int fdm_fused_ok = 0;                   // the three-launch FDM fits: bit 0 FP64, bit 1 FP32
     void timed_cycle(hipStream_t s);
     double run(hipStream_t s);

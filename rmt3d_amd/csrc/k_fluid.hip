@@ -674,6 +674,21 @@ __global__ __launch_bounds__(256) void k_upwind(Geo g, Fluid U, Maps M, int *sta
     k += 1;
     int ind = IX(g, i, j, k);
     const int str[3] = {1, g.m4, g.mn4};
+    // The fluid face states this work-item upwinds (its cell's three upper
+    // faces and the matching lower faces of the neighbours) are written by
+    // this work-item alone and not by the map upwinding below, which writes
+    // map face states only: load all eighteen first, in one batch.
+    double fl[3][3], fr[3][3];
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+        int ff = 2 * dir + 1, nf = ff ^ 1;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int a = (dir + q) % 3;
+            fl[dir][q] = U.fvel[3 * ff + a][ind];
+            fr[dir][q] = U.fvel[3 * nf + a][ind + str[dir]];
+        }
+    }
     if ((M.r.c[ind] & OMASK) > 0)
         if (!upwind_map(g, U, M, ind, ind)) atomicOr(status, ST_FATAL_UPWIND);
     int es = M.b.start[ind], ne = M.b.cnt[ind];
@@ -686,12 +701,13 @@ __global__ __launch_bounds__(256) void k_upwind(Geo g, Fluid U, Maps M, int *sta
             }
         }
     }
+#pragma unroll
     for (int dir = 0; dir < 3; dir++) {
         int nb = ind + str[dir];
         int ff = 2 * dir + 1, nf = ff ^ 1;
         int a1 = (dir + 1) % 3, a2 = (dir + 2) % 3;
-        double nl = U.fvel[3 * ff + dir][ind], vl = U.fvel[3 * ff + a1][ind], wl = U.fvel[3 * ff + a2][ind];
-        double nr = U.fvel[3 * nf + dir][nb], vr = U.fvel[3 * nf + a1][nb], wr = U.fvel[3 * nf + a2][nb];
+        double nl = fl[dir][0], vl = fl[dir][1], wl = fl[dir][2];
+        double nr = fr[dir][0], vr = fr[dir][1], wr = fr[dir][2];
         if (nl + nr > 0 && nl > 0) {
             nr = nl; vr = vl; wr = wl;
         } else if (nl + nr < 0 && nr < 0) {
@@ -955,21 +971,16 @@ __global__ __launch_bounds__(64) void k_ustar_solid(Geo g, Phys P, const ObjC *O
 // FEM projection pieces (fluid_3d.cc:2524-2753), gather form of the element
 // scatter: each node sums its elements in the reference's element order
 // ---------------------------------------------------------------------------
+// (non-short-circuit forms, so that the tests compile to selects)
 __device__ __forceinline__ bool within_domain(const Geo &g, int ei, int ej, int ek) {
-    return (g.prd[0] || (ei >= 0 && ei <= g.m - 1)) && (g.prd[1] || (ej >= 0 && ej <= g.n - 1)) &&
-           (g.prd[2] || (ek >= 0 && ek <= g.o - 1));
+    return (g.prd[0] | ((ei >= 0) & (ei <= g.m - 1))) & (g.prd[1] | ((ej >= 0) & (ej <= g.n - 1))) &
+           (g.prd[2] | ((ek >= 0) & (ek <= g.o - 1)));
 }
 // validity of corner n of element (ei,ej,ek) (map_ref_to_local, fluid_3d.cc:2530-2545)
 __device__ __forceinline__ bool corner_ok(const Geo &g, int ei, int ej, int ek, int n) {
-    if (!within_domain(g, ei, ej, ek)) return false;
-    int ci = n % 2, cj = (n / 2) % 2, ck = n / 4;
-    if (ei + ci > g.fm - 1) return false;
-    if (ei < 0 && ci == 0) return false;
-    if (ej + cj > g.fn - 1) return false;
-    if (ej < 0 && cj == 0) return false;
-    if (ek + ck > g.fo - 1) return false;
-    if (ek < 0 && ck == 0) return false;
-    return true;
+    const int ci = n % 2, cj = (n / 2) % 2, ck = n / 4;
+    return within_domain(g, ei, ej, ek) & (ei + ci <= g.fm - 1) & !((ei < 0) & (ci == 0)) &
+           (ej + cj <= g.fn - 1) & !((ej < 0) & (cj == 0)) & (ek + ck <= g.fo - 1) & !((ek < 0) & (ck == 0));
 }
 
 struct GradPhi { double v[8][3]; };
@@ -995,16 +1006,24 @@ __global__ void k_pp_rhs(Geo g, Fluid U, GradPhi gp, Faces6 faces, int nwall, do
     int cz = fdivq(row, g.fn, rfn);
     int b = row - cz * g.fn;
     double acc = 0;
-    // elements in (k, j, i) ascending order: corner n = 7..0
+    // elements in (k, j, i) ascending order: corner n = 7..0.  Every
+    // element index is inside the padded grid (ei, ej, ek >= -1 and at most
+    // the last cell), so all 24 loads are issued unconditionally, in one
+    // batch, and an element that is not a valid corner leaves acc as it is.
+    double dv[8][3];
+#pragma unroll
+    for (int n = 7; n >= 0; n--) {
+        int e = IX(g, a - n % 2 + 2, b - (n / 2) % 2 + 2, cz - n / 4 + 2);
+#pragma unroll
+        for (int l = 0; l < 3; l++) dv[n][l] = U.dvel[l][e];
+    }
 #pragma unroll
     for (int n = 7; n >= 0; n--) {
         int ei = a - n % 2, ej = b - (n / 2) % 2, ek = cz - n / 4;
-        if (ei < -1 || ej < -1 || ek < -1) continue;
-        if (!corner_ok(g, ei, ej, ek, n)) continue;
-        int e = IX(g, ei + 2, ej + 2, ek + 2);
         double val = 0;
-        for (int l = 0; l < 3; l++) val += U.dvel[l][e] * gp.v[n][l];
-        acc += val;
+#pragma unroll
+        for (int l = 0; l < 3; l++) val += dv[n][l] * gp.v[n][l];
+        acc = corner_ok(g, ei, ej, ek, n) ? acc + val : acc;
     }
     // boundary flux of the wall faces (fluid_3d.cc:2591-2645): node N on face s
     // receives from the elements E = N - corner whose normal coordinate is the

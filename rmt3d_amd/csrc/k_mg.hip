@@ -1191,177 +1191,6 @@ void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd
                               F, x64);
     }
 }
-// ---- two FEM colours per launch, plane by plane -----------------------------
-// The colours are the (j, k) parities: 0 = (even, even), 1 = (odd, even),
-// 2 = (even, odd), 3 = (odd, odd).  Colours 0 and 1 update only the planes
-// k of one parity, and read the planes k +- 1 (the other two colours, not
-// changed by either) and, in their own plane, each other's rows.  So one
-// launch can apply colour 2 kc and then colour 2 kc + 1 to the planes of
-// parity kc: a workgroup owns a band of rows [j0, j1) of one plane, stages
-// the rows j0 - 2 .. j1 + 1 of that plane in LDS, applies the first colour
-// to its rows in [j0 - 1, j1] (one row beyond each side of the band,
-// recomputed where a neighbouring band owns it, from the same values), and
-// after a barrier the second colour to its rows in [j0, j1), reading its
-// plane from LDS and the planes k +- 1 from memory.  Every node gets the
-// value, bit for bit, that two single-colour launches give it.  Rows are
-// staged by their unwrapped index (a periodic y wraps when loading, which
-// needs an even periodic extent so that row parity is preserved).
-constexpr int FEM_PAIR_BAND = 8;
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_fem_pair(MGLevel<T> L, Prd P, int kc, double *x64) {
-    extern __shared__ unsigned char smem[];
-    T *rows = (T *)smem;
-    const int D0 = L.d[0], D1 = L.d[1], D2 = L.d[2], D01 = D0 * D1;
-    const int k = kc + 2 * blockIdx.y;
-    if (k >= D2) return;
-    const int j0 = blockIdx.x * FEM_PAIR_BAND, j1 = min(j0 + FEM_PAIR_BAND, D1);
-    const int rv0 = j0 - 2, nrv = j1 - j0 + 4;      // staged rows rv0 .. rv0 + nrv - 1
-    for (int e = threadIdx.x; e < nrv * D0; e += blockDim.x) {
-        int r = e / D0, i = e - r * D0;
-        int jg = rv0 + r;
-        if (P.p[1]) jg += (jg < 0) ? D1 : ((jg >= D1) ? -D1 : 0);
-        if (jg >= 0 && jg < D1) rows[e] = L.x[i + D0 * (jg + D1 * k)];
-    }
-    __syncthreads();
-    const int ka[3] = {nbw(k, -1, D2, P.p[2]), k, nbw(k, 1, D2, P.p[2])};
-    const bool w2 = !P.p[2] && (k == 0 || k == D2 - 1);
-    const T sz = w2 ? T(0.5) : T(1);
-#pragma unroll
-    for (int ph = 0; ph < 2; ph++) {
-        // rows of this phase's colour (parity ph) in [lo, hi), at most
-        // FEM_PAIR_BAND / 2 + 1 of them: first the interior ones (no wall,
-        // no wrap), then the others
-        const int lo = ph ? j0 : j0 - 1, hi = ph ? j1 : j1 + 1;
-        const int first = lo + ((lo - ph) & 1);
-        const int nr = hi > first ? (hi - first + 1) / 2 : 0;
-        const bool kint = k >= 1 && k <= D2 - 2;
-        // rows q of the phase (row first + 2 q): the in-domain ones are
-        // [q0, q1) and the interior ones a contiguous [qa, qb) inside it
-        // (walls and wraps only at the ends); all uniform over the workgroup
-        int q0 = nr, q1 = 0, qa = nr, qb = 0;
-        for (int q = 0; q < nr; q++) {
-            int j = first + 2 * q;
-            if (P.p[1]) j += (j < 0) ? D1 : ((j >= D1) ? -D1 : 0);
-            if (j < 0 || j >= D1) continue;
-            q0 = min(q0, q);
-            q1 = q + 1;
-            if (kint && j >= 1 && j <= D1 - 2) {
-                qa = min(qa, q);
-                qb = q + 1;
-            }
-        }
-        if (qb <= qa) qa = qb = q0;
-        const int nin = qb - qa, nlo = qa - q0, nbd = (q1 - q0) - nin;
-        // work-items: the interior nodes of interior rows, then the two end
-        // nodes of interior rows and every node of the other rows, so that
-        // a wavefront runs either the 20-load interior form or the general
-        // one (as k_fem_smooth orders its work-items)
-        const int wi = D0 - 2, nI = nin * wi, nE = 2 * nin, nB = nbd * D0;
-        for (int e = threadIdx.x; e < nI + nE + nB; e += blockDim.x) {
-            int rv, i;
-            bool interior = false;
-            if (e < nI) {
-                int q = e / wi;
-                rv = first + 2 * (qa + q);
-                i = 1 + (e - q * wi);
-                interior = true;
-            } else if (e < nI + nE) {
-                int t = e - nI;
-                rv = first + 2 * (qa + (t >> 1));
-                i = (t & 1) ? D0 - 1 : 0;
-            } else {
-                int t = e - nI - nE, bq = t / D0;
-                rv = first + 2 * (bq < nlo ? q0 + bq : qb + (bq - nlo));
-                i = t - bq * D0;
-            }
-            int j = rv;
-            if (P.p[1]) j += (j < 0) ? D1 : ((j >= D1) ? -D1 : 0);
-            const int c = i + D0 * (j + D1 * k);
-            const T *lr = rows + (rv - rv0) * D0;   // this row in LDS; rows +- 1 at lr -+ D0
-            const bool w0 = !P.p[0] && (i == 0 || i == D0 - 1);
-            const bool w1 = !P.p[1] && (j == 0 || j == D1 - 1);
-            T v;
-            if (interior) {
-                // fem_off_interior's order
-                const T *pm = L.x + D0 * (j + D1 * ka[0]) + i;
-                const T *pp = L.x + D0 * (j + D1 * ka[2]) + i;
-                T s = 0;
-                s += T(-1) * pm[-D0 - 1]; s += T(-2) * pm[-D0]; s += T(-1) * pm[-D0 + 1];
-                s += T(-2) * pm[-1];      s += T(-2) * pm[1];
-                s += T(-1) * pm[D0 - 1];  s += T(-2) * pm[D0];  s += T(-1) * pm[D0 + 1];
-                s += T(-2) * lr[-D0 + i - 1]; s += T(-2) * lr[-D0 + i + 1];
-                s += T(-2) * lr[D0 + i - 1];  s += T(-2) * lr[D0 + i + 1];
-                s += T(-1) * pp[-D0 - 1]; s += T(-2) * pp[-D0]; s += T(-1) * pp[-D0 + 1];
-                s += T(-2) * pp[-1];      s += T(-2) * pp[1];
-                s += T(-1) * pp[D0 - 1];  s += T(-2) * pp[D0];  s += T(-1) * pp[D0 + 1];
-                v = (L.b[c] - s) * T(0.03125);
-            } else {
-                // fem_off's general form, the own plane's rows from LDS
-                const int ia[3] = {nbw(i, -1, D0, P.p[0]), i, nbw(i, 1, D0, P.p[0])};
-                const int ja[3] = {nbw(j, -1, D1, P.p[1]), j, nbw(j, 1, D1, P.p[1])};
-                const T sx = w0 ? T(0.5) : T(1), sy = w1 ? T(0.5) : T(1);
-                const T xc = lr[i];
-                T s = 0;
-#pragma unroll
-                for (int ok = 0; ok < 3; ok++) {
-#pragma unroll
-                    for (int oj = 0; oj < 3; oj++) {
-                        bool vjk = ja[oj] >= 0 && ka[ok] >= 0;
-#pragma unroll
-                        for (int oi = 0; oi < 3; oi++) {
-                            const int nz = (oi != 1) + (oj != 1) + (ok != 1);
-                            if (nz < 2) continue;
-                            bool vv = vjk && ia[oi] >= 0;
-                            T w = (nz == 2) ? T(-2) : T(-1);
-                            if (oi == 1) w *= sx;
-                            if (oj == 1) w *= sy;
-                            if (ok == 1) w *= sz;
-                            T xv;
-                            if (!vv) xv = xc;
-                            else if (ok == 1) xv = lr[(oj - 1) * D0 + ia[oi]];
-                            else xv = L.x[ia[oi] + D0 * (ja[oj] + D1 * ka[ok])];
-                            s += (vv ? w : T(0)) * xv;
-                        }
-                    }
-                }
-                T rdg = T(0.03125);
-                if (w0) rdg *= T(2);
-                if (w1) rdg *= T(2);
-                if (w2) rdg *= T(2);
-                v = (L.b[c] - s) * rdg;
-            }
-            rows[(rv - rv0) * D0 + i] = v;
-            if (rv >= j0 && rv < j1) {
-                L.x[c] = v;
-                if (x64) x64[c] += (double)v;
-            }
-        }
-        __syncthreads();
-    }
-    (void)D01;
-}
-
-// kc: colours 2 kc and 2 kc + 1 (both on the planes of parity kc)
-template <typename T>
-void launch_mg_fem_pair(hipStream_t s, const MGLevel<T> &L, const int prd[3], int kc, hipEvent_t start,
-                        hipEvent_t stop, double *x64) {
-    Prd P = mkprd(prd);
-    dim3 grid((L.d[1] + FEM_PAIR_BAND - 1) / FEM_PAIR_BAND, (L.d[2] - kc + 1) / 2);
-    size_t lds = sizeof(T) * (size_t)(FEM_PAIR_BAND + 4) * L.d[0];
-    hipExtLaunchKernelGGL(k_fem_pair<T>, grid, dim3(256), lds, s, start, stop, 0, L, P, kc, x64);
-}
-// the plane-pair form needs FEM extents of at least 4, even periodic
-// extents (row and plane parities wrap consistently) and its row band in
-// LDS
-int mg_fem_pair_ok(const int d[3], const int prd[3], int esize) {
-    for (int a = 0; a < 3; a++) {
-        if (d[a] < 4) return 0;
-        if (prd[a] && (d[a] & 1)) return 0;
-    }
-    return (size_t)esize * (FEM_PAIR_BAND + 4) * d[0] <= 64 * 1024;
-}
-
 static inline dim3 flat_grid(int n) { return dim3((n + 255) / 256); }
 
 // FEM residuals marched along z (RMT3D_RES_MARCH=0: one work-item per node)
@@ -2268,8 +2097,6 @@ void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
     template void launch_mg_fdm<T>(hipStream_t, const MGLevel<T> &, const FdmDev<T> &);                     \
     template void launch_mg_smooth<T>(hipStream_t, int, const MGLevel<T> &, const int *, int, hipEvent_t,    \
                                       hipEvent_t, int, double *);                                            \
-    template void launch_mg_fem_pair<T>(hipStream_t, const MGLevel<T> &, const int *, int, hipEvent_t,       \
-                                        hipEvent_t, double *);                                               \
     template void launch_mg_fdm_pass<T>(hipStream_t, const MGLevel<T> &, int, int, const T *, const T *,    \
                                         const T *, T *);                                                     \
     template void launch_mg_sweep<T>(hipStream_t, int, const MGLevel<T> &, const int *, const SweepPlan &,   \

@@ -142,12 +142,6 @@ template <typename T>
 // value to the FP64 array x64 (the final sweep of the correction)
 void launch_mg_smooth(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3], int color,
                       hipEvent_t start = nullptr, hipEvent_t stop = nullptr, int bs = 0, double *x64 = nullptr);
-// FEM colours 2 kc and 2 kc + 1 in one launch (planes of parity kc); the
-// form needs mg_fem_pair_ok
-template <typename T>
-void launch_mg_fem_pair(hipStream_t s, const MGLevel<T> &L, const int prd[3], int kc, hipEvent_t start,
-                        hipEvent_t stop, double *x64);
-int mg_fem_pair_ok(const int d[3], const int prd[3], int esize);
 template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]);
 template <typename T>
