@@ -762,8 +762,7 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
     smooth(s, fem, L[lev], prd, npass[lev], v_down, kp, smooth_bs[lev]);
-    launch_mg_resid(s, fem, L[lev], prd);
-    launch_mg_restrict(s, fem, L[lev], L[lev + 1], prd);
+    launch_mg_resid_restrict(s, fem, L[lev], L[lev + 1], prd);
     vcycle64(s, lev + 1);
     launch_mg_prolong(s, fem, L[lev], L[lev + 1], prd);
     smooth(s, fem, L[lev], prd, npass[lev], v_up, kp, smooth_bs[lev]);
@@ -782,8 +781,7 @@ void MGSolver::vcycle32(hipStream_t s, int lev, bool fused) {
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
     smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp, smooth_bs[lev], fused ? 1 : 0);
-    launch_mg_resid(s, fem, Lf[lev], prd);
-    launch_mg_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
+    launch_mg_resid_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
     vcycle32(s, lev + 1, false);
     launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
     smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev], 0, fused ? L[0].x : nullptr);

@@ -1273,6 +1273,90 @@ void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, con
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(k_resid64_max, dim3(nb), dim3(256), 0, s, L, mkprd(prd), node_map(L.d, fem), maxbits);
 }
+// MAC residual and restriction in one pass: one work-item per coarse cell
+// computes the residuals of its eight fine cells (every fine cell has one
+// coarse parent, so nothing is computed twice) and sums them in
+// restrict_at's order; the fine residual array is not written (the V-cycle
+// reads only the coarse right-hand side).  A coarse cell away from walls and
+// wraps loads the 32 distinct unknowns of its children and their face
+// neighbours once and sums each child's terms in mac_off's order, so every
+// value is the two-launch form's bit for bit.
+template <typename T>
+__global__ void k_mac_resid_restrict(MGLevel<T> F, MGLevel<T> C, Prd P, float r0, float r1) {
+    FLAT_IJK(C.d[0], C.d[1], r0, r1)
+    if (k >= C.d[2]) return;
+    T s = 0;
+    const bool inner = i >= 1 && i <= C.d[0] - 2 && j >= 1 && j <= C.d[1] - 2 && k >= 1 && k <= C.d[2] - 2;
+    if (inner) {
+        const int D0 = F.d[0], D01 = F.d[0] * F.d[1];
+        const int c0 = 2 * i + D0 * (2 * j) + D01 * (2 * k);
+        const T *x = F.x;
+        T xc[2][2][2], bb[2][2][2], xm[2][2], xp[2][2], ym[2][2], yp[2][2], zm[2][2], zp[2][2];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const int row = c0 + b * D0 + c * D01;
+                xm[c][b] = x[row - 1];
+                xp[c][b] = x[row + 2];
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    xc[c][b][a] = x[row + a];
+                    bb[c][b][a] = F.b[row + a];
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                ym[u][a] = x[c0 + a + u * D01 - D0];        // (2i+a, 2j-1, 2k+u)
+                yp[u][a] = x[c0 + a + u * D01 + 2 * D0];    // (2i+a, 2j+2, 2k+u)
+                zm[u][a] = x[c0 + a + u * D0 - D01];        // (2i+a, 2j+u, 2k-1)
+                zp[u][a] = x[c0 + a + u * D0 + 2 * D01];    // (2i+a, 2j+u, 2k+2)
+            }
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    T sm = 0;
+                    sm += a ? xc[c][b][0] : xm[c][b];
+                    sm += a ? xp[c][b] : xc[c][b][1];
+                    sm += b ? xc[c][0][a] : ym[c][a];
+                    sm += b ? yp[c][a] : xc[c][1][a];
+                    sm += c ? xc[0][b][a] : zm[b][a];
+                    sm += c ? zp[b][a] : xc[1][b][a];
+                    s += bb[c][b][a] - (sm + T(-6.0) * xc[c][b][a]);
+                }
+    } else {
+        for (int c = 0; c < 2; c++)
+            for (int b = 0; b < 2; b++)
+                for (int a = 0; a < 2; a++) {
+                    const int fi = 2 * i + a, fj = 2 * j + b, fk = 2 * k + c;
+                    const int fc = fi + F.d[0] * (fj + F.d[1] * fk);
+                    s += resid_value(F, P, 0, false, fi, fj, fk, fc);
+                }
+    }
+    s *= T(0.5);
+    const int ci = i + C.d[0] * (j + C.d[1] * k);
+    C.b[ci] = s;
+    C.x[ci] = 0;
+}
+
+// r = b - A x on F, then C.b = R r and C.x = 0: one launch for MAC levels
+// (F.r is left as it was), the residual and restriction launches for FEM
+template <typename T>
+void launch_mg_resid_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
+    if (!fem) {
+        hipLaunchKernelGGL(k_mac_resid_restrict<T>, flat_grid(C.nn), dim3(256), 0, s, F, C, mkprd(prd),
+                           1.0f / C.d[0], 1.0f / C.d[1]);
+        return;
+    }
+    launch_mg_resid(s, fem, F, prd);
+    launch_mg_restrict(s, fem, F, C, prd);
+}
+
 template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
     hipLaunchKernelGGL(k_restrict<T>, flat_grid(C.nn), dim3(256), 0, s, F, C, mkprd(prd), fem, 1.0f / C.d[0],
@@ -2103,6 +2187,8 @@ void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
                                      const T *, T *, hipEvent_t, hipEvent_t);                                \
     template void launch_mg_copy<T>(hipStream_t, const T *, T *, int);                                      \
     template void launch_mg_resid<T>(hipStream_t, int, const MGLevel<T> &, const int *);                    \
+    template void launch_mg_resid_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &,      \
+                                              const int *);                                                 \
     template void launch_mg_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *); \
     template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *);  \
     template void launch_mg_tail<T>(hipStream_t, int, const MGLevel<T> *, int, const int *, int, int, int, int,  \

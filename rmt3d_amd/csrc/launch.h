@@ -146,6 +146,10 @@ template <typename T>
 void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[3]);
 template <typename T>
 void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
+// the residual of F restricted into C.b (C.x = 0): fused for MAC levels,
+// which then leave F.r untouched
+template <typename T>
+void launch_mg_resid_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
 template <typename T>
 void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
 // the rest of a V-cycle from L[0] (finest of the tail) to L[n-1] (coarsest);
