@@ -1127,11 +1127,15 @@ __device__ __forceinline__ void redraw_cell(const Geo &g, const Phys &P, const O
     double pen[3], pdet[3];
     uint32_t lflag = (uint32_t)(nlayers + 1) << LDIG;
     uint32_t pc = R.c[t];
-    double px[3] = {R.x[0][t], R.x[1][t], R.x[2][t]};
-    double pp[3] = {R.xpred[0][t], R.xpred[1][t], R.xpred[2][t]};
     bool occupied = false, changed = false, cleared = false;
     int take = -1;   // band entry that became the primary
     int oidp = live ? (int)(pc & OMASK) : 0;
+    // the primary map's values matter only where the cell has one (else they
+    // are overwritten before any use), so a cell without one, most of the
+    // grid, reads three words instead of nine
+    double px[3] = {0, 0, 0}, pp[3] = {0, 0, 0};
+    if (oidp > 0)
+        for (int a = 0; a < 3; a++) { px[a] = R.x[a][t]; pp[a] = R.xpred[a][t]; }
     if (oidp > 0) {
         double pphi = obj_phi(O, P.n_obj, oidp - 1, px[0], px[1], px[2]);
         if (pphi > 0) {
@@ -1240,17 +1244,49 @@ __device__ __forceinline__ void redraw_cell(const Geo &g, const Phys &P, const O
     }
 }
 
-// cells with at most RD_LIGHT - 1 band maps; the others go to hlist
-__global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status,
-                         int *hlist, int *hcount) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = t < g.Ng;
-    if (!live) t = 0;
-    if (live && B.cnt[t] > RD_LIGHT - 1) {
-        hlist[atomicAdd(hcount, 1)] = t;
-        live = false;
+// The cells that hold a map (a primary one or band entries): redraw_cell
+// does nothing on the others, most of the grid, so the register-heavy
+// redraw runs over this list only.  Block 0 also clears the staging and
+// heavy-list counters that the redraw launches use next.  A workgroup
+// gathers the listed cells of RD_SCAN_CHUNKS x 256 cells in LDS and
+// reserves their place in the list with one global atomic: one per 256
+// cells serialised on the counter (24 us at 128^3).
+constexpr int RD_SCAN_CHUNKS = 8;
+__global__ __launch_bounds__(256) void k_redraw_scan(Geo g, RMap R, Band B, int *list, int *count, int *st_n,
+                                                     int *hcount) {
+    __shared__ int buf[256 * RD_SCAN_CHUNKS];
+    __shared__ int nbuf, base;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *st_n = 0;
+        *hcount = 0;
     }
-    redraw_cell<RD_LIGHT, RD_LIGHT, false>(g, P, O, R, B, W, nlayers, status, t, live);
+    if (threadIdx.x == 0) nbuf = 0;
+    __syncthreads();
+    for (int r = 0; r < RD_SCAN_CHUNKS; r++) {
+        const int t = (blockIdx.x * RD_SCAN_CHUNKS + r) * 256 + threadIdx.x;
+        if (t < g.Ng && ((R.c[t] & OMASK) != 0 || B.cnt[t] > 0)) buf[atomicAdd(&nbuf, 1)] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base = nbuf ? atomicAdd(count, nbuf) : 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < nbuf; e += 256) list[base + e] = buf[e];
+}
+// the listed cells with at most RD_LIGHT - 1 band maps; the others go to
+// hlist.  The loop bound is uniform per workgroup, so every lane of a
+// wavefront reaches redraw_cell's reservation scan.
+__global__ void k_redraw(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status,
+                         const int *list, const int *count, int *hlist, int *hcount) {
+    const int n = *count;
+    for (int q0 = blockIdx.x * blockDim.x; q0 < n; q0 += gridDim.x * blockDim.x) {
+        const int q = q0 + threadIdx.x;
+        bool live = q < n;
+        const int t = live ? list[q] : 0;
+        if (live && B.cnt[t] > RD_LIGHT - 1) {
+            hlist[atomicAdd(hcount, 1)] = t;
+            live = false;
+        }
+        redraw_cell<RD_LIGHT, RD_LIGHT, false>(g, P, O, R, B, W, nlayers, status, t, live);
+    }
 }
 __global__ void k_redraw_heavy(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scratch W, int nlayers, int *status,
                                const int *hlist, const int *hcount) {
@@ -1260,9 +1296,11 @@ __global__ void k_redraw_heavy(Geo g, Phys P, const ObjC *O, RMap R, Band B, Scr
 }
 void launch_redraw(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, RMap R, Band B, Scratch &W, int nlayers,
                    int *status) {
-    launch_zero_i32(s, W.st_n, 1);
-    launch_zero_i32(s, W.cnt + 1, 1);
-    hipLaunchKernelGGL(k_redraw, g1(g.Ng), dim3(256), 0, s, g, P, O, R, B, W, nlayers, status, W.list[1], W.cnt + 1);
+    launch_zero_i32(s, W.cnt, 1);
+    hipLaunchKernelGGL(k_redraw_scan, dim3((g.Ng + 256 * RD_SCAN_CHUNKS - 1) / (256 * RD_SCAN_CHUNKS)), dim3(256), 0,
+                       s, g, R, B, W.list[0], W.cnt, W.st_n, W.cnt + 1);
+    hipLaunchKernelGGL(k_redraw, dim3(1024), dim3(256), 0, s, g, P, O, R, B, W, nlayers, status,
+                       (const int *)W.list[0], (const int *)W.cnt, W.list[1], W.cnt + 1);
     hipLaunchKernelGGL(k_redraw_heavy, dim3(64), dim3(64), 0, s, g, P, O, R, B, W, nlayers, status,
                        (const int *)W.list[1], (const int *)(W.cnt + 1));
 }
