@@ -755,7 +755,9 @@ __global__ void k_scan_blocks(const int *in, int *out, int *bsum, int n) {
 }
 // exclusive scan of the block sums in one workgroup of SCAN_BS threads, each
 // thread owning a contiguous run of ceil(nb / SCAN_BS) sums
-__global__ void k_scan_sums(int *bsum, int nb, int *total) {
+// cap_status (when not null): flags ST_BAND_OVERFLOW there if the total
+// exceeds cap
+__global__ void k_scan_sums(int *bsum, int nb, int *total, int cap = 0, int *cap_status = nullptr) {
     __shared__ int sh[SCAN_BS];
     int per = (nb + SCAN_BS - 1) / SCAN_BS;
     int b0 = threadIdx.x * per, b1 = min(b0 + per, nb);
@@ -775,7 +777,10 @@ __global__ void k_scan_sums(int *bsum, int nb, int *total) {
         bsum[b] = run;
         run += v;
     }
-    if (threadIdx.x == SCAN_BS - 1) *total = sh[SCAN_BS - 1];
+    if (threadIdx.x == SCAN_BS - 1) {
+        *total = sh[SCAN_BS - 1];
+        if (cap_status && sh[SCAN_BS - 1] > cap) atomicOr(cap_status, ST_BAND_OVERFLOW);
+    }
 }
 __global__ void k_scan_add(int *out, const int *bsum, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -851,8 +856,13 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0;
 }
-__global__ void k_check_cap(Band B, int *status) {
-    if (*B.n > B.cap) atomicOr(status, ST_BAND_OVERFLOW);
+// two int arrays of n zeroed in one launch
+__global__ void k_zero2_i32(int *p, int *q, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        p[t] = 0;
+        q[t] = 0;
+    }
 }
 
 void launch_zero_i32(hipStream_t s, int *p, int n) {
@@ -866,14 +876,12 @@ static inline dim3 g1(int n) { return dim3((n + 255) / 256); }
 static const int GRID_STRIDE_BLOCKS = 2048;
 
 void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, int nlayers, int *status, int carry) {
-    launch_zero_i32(s, B.cnt, g.Ng);
-    launch_zero_i32(s, W.fill, g.Ng);
+    hipLaunchKernelGGL(k_zero2_i32, g1(g.Ng), dim3(256), 0, s, B.cnt, W.fill, g.Ng);
     hipLaunchKernelGGL(k_count_cells, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
     int nb = (g.Ng + SCAN_BS - 1) / SCAN_BS;
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BS), 0, s, B.cnt, B.start, W.scan_tmp, g.Ng);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, s, W.scan_tmp, nb, B.n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, s, W.scan_tmp, nb, B.n, B.cap, status);
     hipLaunchKernelGGL(k_scan_add, g1(g.Ng), dim3(256), 0, s, B.start, W.scan_tmp, g.Ng);
-    hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(1), 0, s, B, status);
     hipLaunchKernelGGL(k_place, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
     hipLaunchKernelGGL(k_sort_cells, g1(g.Ng), dim3(256), 0, s, g, B, W, nlayers);
     hipLaunchKernelGGL(k_gather_entries, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B, W, carry);
@@ -1313,6 +1321,7 @@ __global__ void k_sum_blocks(const double *a, int n, double scale, double *parti
     double v = 0;
     int per = (n + gridDim.x - 1) / gridDim.x;
     int s = blockIdx.x * per, e = s + per < n ? s + per : n;
+#pragma unroll 8
     for (int t = s + threadIdx.x; t < e; t += blockDim.x) v += a[t] * scale;
     sh[threadIdx.x] = v;
     __syncthreads();
@@ -1322,17 +1331,23 @@ __global__ void k_sum_blocks(const double *a, int n, double scale, double *parti
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
+// the partials are loaded by all work-items at once, then added in order by
+// one (nb <= 256)
 __global__ void k_finish_sum(const double *partial, int nb, double *out) {
+    __shared__ double sh[256];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) sh[b] = partial[b];
+    __syncthreads();
     if (threadIdx.x != 0) return;
     double v = 0;
-    for (int b = 0; b < nb; b++) v += partial[b];
+#pragma unroll 16
+    for (int b = 0; b < nb; b++) v += sh[b];
     *out = v;
 }
 void launch_sum_blocks(hipStream_t s, const double *a, int n, double scale, double *partial) {
     hipLaunchKernelGGL(k_sum_blocks, dim3(256), dim3(256), 0, s, a, n, scale, partial);
 }
 void launch_finish_sum(hipStream_t s, const double *partial, double *out) {
-    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, partial, 256, out);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, s, partial, 256, out);
 }
 
 }  // namespace rmt
