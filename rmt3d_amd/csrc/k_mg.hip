@@ -550,6 +550,29 @@ __global__ void k_resid_march(MGLevel<T> L, Prd P, MarchMap A) {
     resid_march(L, P, A, t, [&](int c, int, int, T r) { L.r[c] = r; });
 }
 
+// k_resid64_max for FEM levels with the interior marched along z
+__global__ void k_resid64_max_march(MGLevel<double> L, Prd P, MarchMap A, unsigned long long *maxbits) {
+    __shared__ unsigned long long sh[4];
+    unsigned long long v = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < A.items; t += gridDim.x * blockDim.x)
+        resid_march(L, P, A, t, [&](int c, int, int, double r) {
+            L.r[c] = r;
+            unsigned long long b = (unsigned long long)__double_as_longlong(fabs(r));
+            v = b > v ? b : v;
+        });
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = sh[w] > m ? sh[w] : m;
+        atomicMax(maxbits, m);
+    }
+}
+
 template <typename T>
 __global__ void k_resid(MGLevel<T> L, Prd P, NodeMap M) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1269,9 +1292,17 @@ void launch_mg_resid(hipStream_t s, int fem, const MGLevel<T> &L, const int prd[
 }
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits) {
+    NodeMap M = node_map(L.d, fem);
+    if (!M.plain && res_march_on()) {
+        MarchMap A = march_map(L.d);
+        int nb = (A.items + 255) / 256;
+        if (nb > 1024) nb = 1024;
+        hipLaunchKernelGGL(k_resid64_max_march, dim3(nb), dim3(256), 0, s, L, mkprd(prd), A, maxbits);
+        return;
+    }
     int nb = (L.nn + 255) / 256;
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_resid64_max, dim3(nb), dim3(256), 0, s, L, mkprd(prd), node_map(L.d, fem), maxbits);
+    hipLaunchKernelGGL(k_resid64_max, dim3(nb), dim3(256), 0, s, L, mkprd(prd), M, maxbits);
 }
 // MAC residual and restriction in one pass: one work-item per coarse cell
 // computes the residuals of its eight fine cells (every fine cell has one
