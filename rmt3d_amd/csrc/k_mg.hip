@@ -167,6 +167,17 @@ __global__ void k_mac_smooth(MGLevel<T> L, Prd P, int color, int hx, float rhx, 
         pv[0] = (c & 1) ? o : v;
         pv[1] = (c & 1) ? v : o;
         *reinterpret_cast<pair_t *>(L.x + (c & ~1)) = pv;
+        if (x64) {
+            // the FP64 solution the same way: the pair read and written back
+            // whole, only this node's element changed
+            typedef double pair64_t __attribute__((ext_vector_type(2)));
+            pair64_t *xp = reinterpret_cast<pair64_t *>(x64 + (c & ~1));
+            pair64_t q = *xp;
+            if (c & 1) q[1] += (double)v;
+            else q[0] += (double)v;
+            *xp = q;
+        }
+        return;
     } else {
         v = mac_smooth_at(L, P, i, j, k);
     }
