@@ -98,9 +98,13 @@ def sweep_words(np_):
 # hierarchy of the 128^3 workload (six contractions reading and writing 1/8
 # as many unknowns: 1.5 words).  At the reference's 2 + 2 sweeps: MAC 19.9,
 # FEM 27.9 words.
-def vcycle_words(fem, sweeps):
+def vcycle_words(fem, down, up):
+    """Algorithmic words per finest unknown of one V-cycle: the sweeps, the
+    finest residual (not computed when no sweep precedes the coarse
+    correction: the right-hand side is restricted as it is), restriction,
+    prolongation and the coarse solve's share."""
     per_sweep = 4 * FEM_COLOUR_WORDS if fem else 2 * 1.5
-    return sweeps * per_sweep + 3.0 + 1.25 + 2.125 + 1.5
+    return (down + up) * per_sweep + (3.0 if down > 0 else 0.0) + 1.25 + 2.125 + 1.5
 # One FP64 defect-correction step of the mixed path adds the FP64 residual
 # (read x, b, write r) per finest unknown.
 DEFECT_WORDS64 = 3.0
@@ -185,7 +189,7 @@ def run_gpu(args, precision):
         ms_v, nv = prof[key + "_vcycle"]
         t_v = ms_v / max(nv, 1) * 1e-3
         sched = int(sim.diag(34 if key == "fem" else 33))
-        words = vcycle_words(key == "fem", sched // 10 + sched % 10)
+        words = vcycle_words(key == "fem", sched // 10, sched % 10)
         b = words * N * word + (DEFECT_WORDS64 * N * 8 if precision == "mixed" else 0)
         vc[key] = {"us": round(t_v * 1e6, 2), "gbs": round(b / t_v / 1e9, 1) if t_v > 0 else 0.0,
                    "frac": round(b / t_v / 1e9 / HBM_PEAK_GBS, 4) if t_v > 0 else 0.0, "cycles": nv}

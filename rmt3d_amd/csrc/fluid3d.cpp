@@ -617,7 +617,7 @@ void MGSolver::tune_levels(hipStream_t s, MGLevel<T> *LV, FdmDev<T> &F, bool swe
 static int mixed_vcycle_env() {
     const char *vc = getenv("RMT3D_MIXED_VCYCLE");
     int d = 0, u = 0;
-    if (vc && sscanf(vc, "%d,%d", &d, &u) == 2 && d >= 1 && u >= 1 && d <= 4 && u <= 4) return 10 * d + u;
+    if (vc && sscanf(vc, "%d,%d", &d, &u) == 2 && d >= 0 && u >= 1 && d <= 4 && u <= 4) return 10 * d + u;
     return 0;
 }
 
@@ -684,7 +684,10 @@ void MGSolver::tune(hipStream_t s) {
 // its iterates are the oracle's.  The mixed path's FP32 V-cycle is a
 // preconditioner inside the FP64 defect correction, whose stopping rule and
 // converged solution do not depend on it, so there the schedule is picked
-// here: for 2 + 2, 2 + 1 and 1 + 2 sweeps, the iterations that take a fixed
+// here: for 2 + 2, 2 + 1, 1 + 2 and 0 + 3 sweeps (with an exact or nearly
+// exact coarse solve the sweeps after it do the work of the ones before,
+// and without sweeps before it the finest level's own residual pass is not
+// needed), the iterations that take a fixed
 // zero-mean right-hand side (a 64-bit LCG) from zero below the solver's
 // FP64 tolerance, plus the two insurance iterations every solve runs after
 // that (mg_probs.cc:43-47), times the event-timed cost of one iteration;
@@ -696,7 +699,7 @@ void MGSolver::tune_schedule(hipStream_t s) {
         v_up = f % 10;
         return;
     }
-    static const int cand[3][2] = {{2, 2}, {2, 1}, {1, 2}};
+    static const int cand[4][2] = {{2, 2}, {2, 1}, {1, 2}, {0, 3}};
     const size_t n = L[0].nn;
     std::vector<double> h(n);
     unsigned long long st = 0x9E3779B97F4A7C15ull;
@@ -768,9 +771,12 @@ void MGSolver::vcycle64(hipStream_t s, int lev) {
     smooth(s, fem, L[lev], prd, npass[lev], v_up, kp, smooth_bs[lev]);
 }
 
-// fused (finest level only): the first colour pass came with the residual,
-// and the last sweep adds the correction into the FP64 solution
-void MGSolver::vcycle32(hipStream_t s, int lev, bool fused) {
+// first / last (finest level only): the first colour pass came with the
+// residual, and the last sweep adds the correction into the FP64 solution.
+// With no sweep before the coarse correction (v_down 0) the level's x is
+// still zero (the residual kernel or the restriction above cleared it), so
+// its residual is its right-hand side: that is restricted directly.
+void MGSolver::vcycle32(hipStream_t s, int lev, bool first, bool last) {
     if (fdm && lev == nlev - 1) {
         launch_mg_fdm(s, Lf[lev], fdm32);
         return;
@@ -780,11 +786,17 @@ void MGSolver::vcycle32(hipStream_t s, int lev, bool fused) {
         return;
     }
     KProf *kp = (lev == 0) ? prof : nullptr;
-    smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp, smooth_bs[lev], fused ? 1 : 0);
-    launch_mg_resid_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
-    vcycle32(s, lev + 1, false);
+    if (v_down > 0) {
+        smooth(s, fem, Lf[lev], prd, npass[lev], v_down, kp, smooth_bs[lev], first ? 1 : 0);
+        launch_mg_resid_restrict(s, fem, Lf[lev], Lf[lev + 1], prd);
+    } else {
+        MGLevel<float> rb = Lf[lev];
+        rb.r = rb.b;
+        launch_mg_restrict(s, fem, rb, Lf[lev + 1], prd);
+    }
+    vcycle32(s, lev + 1, false, false);
     launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
-    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev], 0, fused ? L[0].x : nullptr);
+    smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev], 0, last ? L[0].x : nullptr);
 }
 
 // The defect-correction step's FP64 pieces ride on the FP32 V-cycle's own
@@ -793,9 +805,14 @@ void MGSolver::vcycle32(hipStream_t s, int lev, bool fused) {
 // by the residual kernel, and the last sweep adds the correction into the
 // FP64 solution, so neither a separate colour pass nor an update pass runs.
 // The values are the same as those of the separate launches, bit for bit.
-// RMT3D_MIXED_FUSE=0 runs the separate launches.
-bool MGSolver::fused_finest() const {
-    return mixed && !fuse_off && tail_lev > 0 && !(fdm && nlev == 1) && npass[0] == 1 && v_down > 0 && v_up > 0;
+// RMT3D_MIXED_FUSE=0 runs the separate launches.  fuse_first: the residual
+// kernel writes the first colour pass (there is a sweep before the coarse
+// correction); fuse_last: the last sweep adds into the FP64 solution.
+bool MGSolver::fuse_first() const {
+    return mixed && !fuse_off && tail_lev > 0 && !(fdm && nlev == 1) && npass[0] == 1 && v_down > 0;
+}
+bool MGSolver::fuse_last() const {
+    return mixed && !fuse_off && tail_lev > 0 && !(fdm && nlev == 1) && npass[0] == 1 && v_up > 0;
 }
 
 // one iteration: an FP64 V-cycle, or (mixed) one FP64 defect-correction step
@@ -805,10 +822,10 @@ void MGSolver::cycle_body(hipStream_t s) {
         vcycle64(s, 0);
         return;
     }
-    bool fused = fused_finest();
-    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, Lf[0].x, fused ? 1 : 0);
-    vcycle32(s, 0, fused);
-    if (!fused) launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
+    const bool first = fuse_first(), last = fuse_last();
+    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, Lf[0].x, first ? 1 : 0);
+    vcycle32(s, 0, first, last);
+    if (!last) launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
 }
 
 // The kernel sequence of one iteration and all its arguments are fixed for

@@ -93,8 +93,9 @@ struct MGSolver {
     double check_wait();
     hipEvent_t ev_check = nullptr;
     void vcycle64(hipStream_t s, int lev);
-    void vcycle32(hipStream_t s, int lev, bool fused = false);
-    bool fused_finest() const;
+    void vcycle32(hipStream_t s, int lev, bool first = false, bool last = false);
+    bool fuse_first() const;
+    bool fuse_last() const;
     void cycle(hipStream_t s);              // one V-cycle (or one defect-correction step)
     void cycle_body(hipStream_t s);         // its launches
     hipGraphExec_t gexec = nullptr;         // the launches captured once
