@@ -684,7 +684,7 @@ void MGSolver::tune(hipStream_t s) {
 // its iterates are the oracle's.  The mixed path's FP32 V-cycle is a
 // preconditioner inside the FP64 defect correction, whose stopping rule and
 // converged solution do not depend on it, so there the schedule is picked
-// here: for 2 + 2, 2 + 1, 1 + 2 and 0 + 3 sweeps (with an exact or nearly
+// here: for 2 + 2, 2 + 1, 1 + 2 and (FEM) 0 + 3 sweeps (with an exact or nearly
 // exact coarse solve the sweeps after it do the work of the ones before,
 // and without sweeps before it the finest level's own residual pass is not
 // needed), the iterations that take a fixed
@@ -699,7 +699,12 @@ void MGSolver::tune_schedule(hipStream_t s) {
         v_up = f % 10;
         return;
     }
+    // 0 + 3 only for the FEM operator: for the red-black MAC smoother it
+    // scored close to 2 + 2 on this test but needed half as many iterations
+    // again on the warm-started solves of a step (1.18 -> 1.43 ms per
+    // 128^3 step)
     static const int cand[4][2] = {{2, 2}, {2, 1}, {1, 2}, {0, 3}};
+    const int ncand = fem ? 4 : 3;
     const size_t n = L[0].nn;
     std::vector<double> h(n);
     unsigned long long st = 0x9E3779B97F4A7C15ull;
@@ -718,7 +723,8 @@ void MGSolver::tune_schedule(hipStream_t s) {
     const double thr = max_norm_tol;
     double best = 1e300;
     int bd = v_down, bu = v_up;
-    for (const auto &c : cand) {
+    for (int ci = 0; ci < ncand; ci++) {
+        const int *c = cand[ci];
         v_down = c[0];
         v_up = c[1];
         hipMemsetAsync(L[0].x, 0, sizeof(double) * n, s);
@@ -795,7 +801,7 @@ void MGSolver::vcycle32(hipStream_t s, int lev, bool first, bool last) {
         launch_mg_restrict(s, fem, rb, Lf[lev + 1], prd);
     }
     vcycle32(s, lev + 1, false, false);
-    launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd);
+    launch_mg_prolong(s, fem, Lf[lev], Lf[lev + 1], prd, v_down == 0 ? 1 : 0);
     smooth(s, fem, Lf[lev], prd, npass[lev], v_up, kp, smooth_bs[lev], 0, last ? L[0].x : nullptr);
 }
 

@@ -685,7 +685,7 @@ __device__ __forceinline__ void vx_w(int f, int D, int prd, int &i0, int &i1, T 
 
 template <typename T>
 __device__ __forceinline__ void prolong_at(const MGLevel<T> &F, const MGLevel<T> &C, const Prd &P, int fem,
-                                           int i, int j, int k) {
+                                           int i, int j, int k, bool assign = false) {
     int i0, i1, j0, j1, k0, k1;
     T wi0, wi1, wj0, wj1, wk0, wk1;
     if (fem) {
@@ -703,14 +703,19 @@ __device__ __forceinline__ void prolong_at(const MGLevel<T> &F, const MGLevel<T>
     T v = wk0 * (wj0 * (wi0 * E(i0, j0, k0) + wi1 * E(i1, j0, k0)) + wj1 * (wi0 * E(i0, j1, k0) + wi1 * E(i1, j1, k0))) +
           wk1 * (wj0 * (wi0 * E(i0, j0, k1) + wi1 * E(i1, j0, k1)) + wj1 * (wi0 * E(i0, j1, k1) + wi1 * E(i1, j1, k1)));
 #undef E
-    F.x[i + F.d[0] * (j + F.d[1] * k)] += v;
+    T *xf = F.x + i + F.d[0] * (j + F.d[1] * k);
+    if (assign) *xf = v;
+    else *xf += v;
 }
 
-template <typename T>
+// ASSIGN: the fine x is known to be zero (no sweep before the coarse
+// correction), so the interpolated correction is stored instead of added
+// (no read of x; +0 instead of -0 where the correction is -0)
+template <typename T, bool ASSIGN>
 __global__ void k_prolong(MGLevel<T> F, MGLevel<T> C, Prd P, int fem, float r0, float r1) {
     FLAT_IJK(F.d[0], F.d[1], r0, r1)
     if (k >= F.d[2]) return;
-    prolong_at(F, C, P, fem, i, j, k);
+    prolong_at(F, C, P, fem, i, j, k, ASSIGN);
 }
 
 // (i, j, k) of a flat index e of dims (a, b, .) with float reciprocals ra, rb
@@ -1405,9 +1410,14 @@ void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLev
                        1.0f / C.d[1]);
 }
 template <typename T>
-void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]) {
-    hipLaunchKernelGGL(k_prolong<T>, flat_grid(F.nn), dim3(256), 0, s, F, C, mkprd(prd), fem, 1.0f / F.d[0],
-                       1.0f / F.d[1]);
+void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3],
+                       int assign) {
+    if (assign)
+        hipLaunchKernelGGL((k_prolong<T, true>), flat_grid(F.nn), dim3(256), 0, s, F, C, mkprd(prd), fem,
+                           1.0f / F.d[0], 1.0f / F.d[1]);
+    else
+        hipLaunchKernelGGL((k_prolong<T, false>), flat_grid(F.nn), dim3(256), 0, s, F, C, mkprd(prd), fem,
+                           1.0f / F.d[0], 1.0f / F.d[1]);
 }
 template <typename T>
 size_t mg_tail_lds_bytes(const MGLevel<T> *L, int n) {
@@ -2232,7 +2242,8 @@ void launch_mg_fdm(hipStream_t s, const MGLevel<T> &L, const FdmDev<T> &F) {
     template void launch_mg_resid_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &,      \
                                               const int *);                                                 \
     template void launch_mg_restrict<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *); \
-    template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *);  \
+    template void launch_mg_prolong<T>(hipStream_t, int, const MGLevel<T> &, const MGLevel<T> &, const int *,   \
+                                       int);                                                                  \
     template void launch_mg_tail<T>(hipStream_t, int, const MGLevel<T> *, int, const int *, int, int, int, int,  \
                                     const T *);                                                              \
     template size_t mg_tail_lds_bytes<T>(const MGLevel<T> *, int);                                          \

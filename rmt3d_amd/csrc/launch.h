@@ -151,7 +151,8 @@ void launch_mg_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLev
 template <typename T>
 void launch_mg_resid_restrict(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
 template <typename T>
-void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3]);
+void launch_mg_prolong(hipStream_t s, int fem, const MGLevel<T> &F, const MGLevel<T> &C, const int prd[3],
+                       int assign = 0);
 // the rest of a V-cycle from L[0] (finest of the tail) to L[n-1] (coarsest);
 // pinv (n_c x n_c, symmetric) solves the coarsest level exactly, or null for
 // csweeps smoothing sweeps there
