@@ -829,7 +829,10 @@ void MGSolver::cycle_body(hipStream_t s) {
         return;
     }
     const bool first = fuse_first(), last = fuse_last();
-    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, Lf[0].x, first ? 1 : 0);
+    // with no sweep before the coarse correction the prolongation stores
+    // every node of the FP32 correction, so it need not be cleared here
+    const bool x_stored = v_down == 0 && tail_lev > 0 && !(fdm && nlev == 1);
+    launch_resid_to_f32(s, fem, L[0], prd, Lf[0].b, x_stored ? nullptr : Lf[0].x, first ? 1 : 0);
     vcycle32(s, 0, first, last);
     if (!last) launch_axpy_f2d(s, L[0].x, Lf[0].x, L[0].nn);
 }

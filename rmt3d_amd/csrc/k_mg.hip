@@ -1169,7 +1169,7 @@ __global__ void k_resid_to_f32(MGLevel<double> L, Prd P, NodeMap M, float *bf, f
             x0 = float(MAC_OMEGA) * (r / dg);
         }
     }
-    xf[c] = x0;
+    if (xf) xf[c] = x0;
 }
 
 // k_resid_to_f32 for FEM levels with the interior marched along z
@@ -1189,7 +1189,7 @@ __global__ void k_resid_to_f32_march(MGLevel<double> L, Prd P, MarchMap A, float
             if (!P.p[2] && (k == 0 || k == L.d[2] - 1)) rdg *= 2.f;
             x0 = r * rdg;
         }
-        xf[c] = x0;
+        if (xf) xf[c] = x0;
     });
 }
 

@@ -169,8 +169,8 @@ int mg_tail_enable_lds(size_t bytes);
 void launch_mg_resid64_max(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3],
                            unsigned long long *maxbits);
 // mixed precision helpers
-// first_pass: also write the first colour pass of the V-cycle from zero
-// into xf (nodes of colour 0), zero elsewhere
+// bf = (float) r64; xf (when not null) = 0, or with first_pass the first
+// colour pass of the V-cycle from zero (nodes of colour 0), zero elsewhere
 void launch_resid_to_f32(hipStream_t s, int fem, const MGLevel<double> &L, const int prd[3], float *bf, float *xf,
                          int first_pass = 0);
 void launch_cast_d2f(hipStream_t s, const double *a, float *b, int n);
