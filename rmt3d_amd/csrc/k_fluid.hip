@@ -883,10 +883,21 @@ __global__ void k_periodic_band_fx(Geo g, Band B) {
 // ---------------------------------------------------------------------------
 // MAC projection pieces (fluid_3d.cc:2244-2304)
 // ---------------------------------------------------------------------------
+// interior cell (i, j, k) of work-item t of a flat grid over m x n x o,
+// x fastest (one 256-wide workgroup spans two rows of 128 cells)
+__device__ __forceinline__ bool flat_cell(const Geo &g, int &i, int &j, int &k) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.m * g.n * g.o) return false;
+    const int row = t / g.m;
+    i = t - row * g.m;
+    k = row / g.n;
+    j = row - k * g.n;
+    return true;
+}
+
 __global__ void k_mac_rhs(Geo g, Fluid U, double *b, double fac) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m) return;
+    int i, j, k;
+    if (!flat_cell(g, i, j, k)) return;
     int c = IX(g, i + 2, j + 2, k + 2);
     double r = 0;
     for (int f = 0; f < 3; f++) r += fac * (U.fvel[3 * (2 * f + 1) + f][c] - U.fvel[3 * (2 * f) + f][c]);
@@ -894,16 +905,14 @@ __global__ void k_mac_rhs(Geo g, Fluid U, double *b, double fac) {
 }
 
 __global__ void k_mac_extract(Geo g, Fluid U, const double *x) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m) return;
+    int i, j, k;
+    if (!flat_cell(g, i, j, k)) return;
     U.q[IX(g, i + 2, j + 2, k + 2)] = x[i + g.m * (j + g.n * k)];
 }
 
 __global__ void k_sub_qgrad(Geo g, Fluid U, double f2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m) return;
+    int i, j, k;
+    if (!flat_cell(g, i, j, k)) return;
     int c = IX(g, i + 2, j + 2, k + 2);
     const int str[3] = {1, g.m4, g.mn4};
     double q = U.q[c];
@@ -1070,9 +1079,8 @@ __global__ void k_pp_extract(Geo g, Fluid U, const double *x, const double *mean
 }
 
 __global__ void k_sub_pgrad(Geo g, Fluid U, double dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m) return;
+    int i, j, k;
+    if (!flat_cell(g, i, j, k)) return;
     int c = IX(g, i + 2, j + 2, k + 2);
     double npg[3];
     neg_pres_grad(g, U.dp, c, npg);
@@ -1341,13 +1349,13 @@ void launch_periodic_band_fx(hipStream_t s, const Geo &g, const Band &B) {
     hipLaunchKernelGGL(k_periodic_band_fx, dim3((tot + 255) / 256), dim3(256), 0, s, g, B);
 }
 void launch_mac_rhs(hipStream_t s, const Geo &g, const Fluid &U, double *b, double fac) {
-    hipLaunchKernelGGL(k_mac_rhs, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, U, b, fac);
+    hipLaunchKernelGGL(k_mac_rhs, dim3((g.m * g.n * g.o + 255) / 256), dim3(256), 0, s, g, U, b, fac);
 }
 void launch_mac_extract(hipStream_t s, const Geo &g, const Fluid &U, const double *x) {
-    hipLaunchKernelGGL(k_mac_extract, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, U, x);
+    hipLaunchKernelGGL(k_mac_extract, dim3((g.m * g.n * g.o + 255) / 256), dim3(256), 0, s, g, U, x);
 }
 void launch_sub_qgrad(hipStream_t s, const Geo &g, const Fluid &U, double f2) {
-    hipLaunchKernelGGL(k_sub_qgrad, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, U, f2);
+    hipLaunchKernelGGL(k_sub_qgrad, dim3((g.m * g.n * g.o + 255) / 256), dim3(256), 0, s, g, U, f2);
 }
 void launch_ustar(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, const Fluid &U, const Maps &M,
                   const int *list, const int *count) {
@@ -1379,7 +1387,7 @@ void launch_pp_extract(hipStream_t s, const Geo &g, const Fluid &U, const double
     hipLaunchKernelGGL(k_pp_extract, dim3((n + 255) / 256), dim3(256), 0, s, g, U, x, mean);
 }
 void launch_sub_pgrad(hipStream_t s, const Geo &g, const Fluid &U, double dt) {
-    hipLaunchKernelGGL(k_sub_pgrad, rows(g.m, g.n, g.o, 64), dim3(64), 0, s, g, U, dt);
+    hipLaunchKernelGGL(k_sub_pgrad, dim3((g.m * g.n * g.o + 255) / 256), dim3(256), 0, s, g, U, dt);
 }
 void launch_divu_terms(hipStream_t s, const Geo &g, const double *r, double *out) {
     hipLaunchKernelGGL(k_divu_terms, rows(g.fm, g.fn, g.fo, 64), dim3(64), 0, s, g, r, out);
