@@ -1006,10 +1006,12 @@ int band_extrapolate(hipStream_t s, const Geo &g, const Phys &P, const ObjC *O, 
 
 // ---- predictor, corrector, full update ----
 // update_reference_map predictor (fluid_3d.cc:1153-1192)
+// flat grid over the m x n x o interior, x fastest
 __global__ void k_xpred(Geo g, Phys P, Fluid U, RMap R) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y, k = blockIdx.z;
-    if (i >= g.m) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.m * g.n * g.o) return;
+    const int row = t / g.m, i = t - row * g.m;
+    const int k = row / g.n, j = row - k * g.n;
     int c = IXb(g, i + 2, j + 2, k + 2);
     if ((R.c[c] & OMASK) == 0) return;
     double cdt = P.dt;
@@ -1024,7 +1026,7 @@ __global__ void k_xpred(Geo g, Phys P, Fluid U, RMap R) {
     }
 }
 void launch_xpred(hipStream_t s, const Geo &g, const Phys &P, const Fluid &U, RMap R) {
-    hipLaunchKernelGGL(k_xpred, dim3((g.m + 63) / 64, g.n, g.o), dim3(64), 0, s, g, P, U, R);
+    hipLaunchKernelGGL(k_xpred, dim3((g.m * g.n * g.o + 255) / 256), dim3(256), 0, s, g, P, U, R);
 }
 
 // corrector (fluid_3d.cc:1205-1248) over every padded cell
