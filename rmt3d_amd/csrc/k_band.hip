@@ -738,20 +738,37 @@ __global__ void k_count_cells(Geo g, Band B, Scratch W) {
 
 // exclusive scan of B.cnt into B.start, total into *B.n (three phases)
 constexpr int SCAN_BS = 1024;
+// exclusive scan of SCAN_BS counts per workgroup: each wavefront scans its
+// 64 with lane shuffles, one wavefront scans the wavefront totals, and two
+// barriers join them (integer sums: the same values as any order)
 __global__ void k_scan_blocks(const int *in, int *out, int *bsum, int n) {
-    __shared__ int sh[SCAN_BS];
-    int t = blockIdx.x * SCAN_BS + threadIdx.x;
-    int v = (t < n) ? in[t] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < SCAN_BS; off <<= 1) {
-        int add = (threadIdx.x >= (unsigned)off) ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += add;
-        __syncthreads();
+    constexpr int NW = SCAN_BS / 64;
+    __shared__ int ws[NW];
+    const int t = blockIdx.x * SCAN_BS + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int v = (t < n) ? in[t] : 0;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
     }
-    if (t < n) out[t] = sh[threadIdx.x] - v;
-    if (threadIdx.x == SCAN_BS - 1) bsum[blockIdx.x] = sh[SCAN_BS - 1];
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        const int s0 = lane < NW ? ws[lane] : 0;
+        int si = s0;
+#pragma unroll
+        for (int off = 1; off < NW; off <<= 1) {
+            const int u = __shfl_up(si, off, 64);
+            if (lane >= off) si += u;
+        }
+        if (lane < NW) ws[lane] = si - s0;
+    }
+    __syncthreads();
+    const int excl = incl - v + ws[w];
+    if (t < n) out[t] = excl;
+    if (threadIdx.x == SCAN_BS - 1) bsum[blockIdx.x] = excl + v;
 }
 // exclusive scan of the block sums in one workgroup of SCAN_BS threads, each
 // thread owning a contiguous run of ceil(nb / SCAN_BS) sums
@@ -1272,10 +1289,15 @@ __global__ __launch_bounds__(256) void k_redraw_scan(Geo g, RMap R, Band B, int 
     }
     if (threadIdx.x == 0) nbuf = 0;
     __syncthreads();
+    bool f[RD_SCAN_CHUNKS];
+#pragma unroll
     for (int r = 0; r < RD_SCAN_CHUNKS; r++) {
         const int t = (blockIdx.x * RD_SCAN_CHUNKS + r) * 256 + threadIdx.x;
-        if (t < g.Ng && ((R.c[t] & OMASK) != 0 || B.cnt[t] > 0)) buf[atomicAdd(&nbuf, 1)] = t;
+        f[r] = t < g.Ng && ((R.c[t] & OMASK) != 0 || B.cnt[t] > 0);
     }
+#pragma unroll
+    for (int r = 0; r < RD_SCAN_CHUNKS; r++)
+        if (f[r]) buf[atomicAdd(&nbuf, 1)] = (blockIdx.x * RD_SCAN_CHUNKS + r) * 256 + threadIdx.x;
     __syncthreads();
     if (threadIdx.x == 0) base = nbuf ? atomicAdd(count, nbuf) : 0;
     __syncthreads();
