@@ -818,18 +818,22 @@ __global__ void k_place(Geo g, Band B, Scratch W) {
     if (n > W.st_cap) n = W.st_cap;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         int c = W.st_cell[t];
-        int slot = B.start[c] + atomicAdd(&W.fill[c], 1);
+        // the cell's start: its in-block prefix plus its block's offset
+        // (k_sort_cells writes the sum back into B.start)
+        int slot = B.start[c] + W.scan_tmp[c / SCAN_BS] + atomicAdd(&W.fill[c], 1);
         if (slot < B.cap) W.perm[slot] = t;
     }
 }
 
-// multimap order inside a cell: object order, a redrawn primary last
+// multimap order inside a cell: object order, a redrawn primary last.  Also
+// completes the scan: B.start gets its block's offset added.
 __global__ void k_sort_cells(Geo g, Band B, Scratch W, int nlayers) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.Ng) return;
+    const int s = B.start[t] + W.scan_tmp[t / SCAN_BS];
+    B.start[t] = s;
     int n = B.cnt[t];
     if (n < 2) return;
-    int s = B.start[t];
     if (s + n > B.cap) return;
     uint32_t moved = (uint32_t)(nlayers + 1);
     for (int a = 1; a < n; a++) {
@@ -898,7 +902,6 @@ void band_build_from_staging(hipStream_t s, const Geo &g, Band B, Scratch &W, in
     int nb = (g.Ng + SCAN_BS - 1) / SCAN_BS;
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_BS), 0, s, B.cnt, B.start, W.scan_tmp, g.Ng);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BS), 0, s, W.scan_tmp, nb, B.n, B.cap, status);
-    hipLaunchKernelGGL(k_scan_add, g1(g.Ng), dim3(256), 0, s, B.start, W.scan_tmp, g.Ng);
     hipLaunchKernelGGL(k_place, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, g, B, W);
     hipLaunchKernelGGL(k_sort_cells, g1(g.Ng), dim3(256), 0, s, g, B, W, nlayers);
     hipLaunchKernelGGL(k_gather_entries, dim3(GRID_STRIDE_BLOCKS), dim3(256), 0, s, B, W, carry);
